@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark: TAA resolve (+ fused CAS) throughput on one MI355X.
+
+Default workload: the 3840x2160 case (RGBA16F colour/history, RG16F velocity, D32F depth,
+8000-polygon seeded scene, variance clip + Catmull-Rom + closest-depth velocity + linear-depth
+rejection + Lottes + reject-outside + fused CAS 0.5). A step is one resolve of one 4K frame;
+inputs for 8 distinct frames are generated on the host, uploaded once and cycled, so the timed
+region reads inputs already resident in HBM.
+
+Prints one JSON line (rank 0). `--gpus N` with N > 1 exits with a message: this build measures
+one GPU; sessions shard across GPUs without any exchange.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "taa-star_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--arith", default="fp16", choices=["fp32", "fp16"])
+    ap.add_argument("--frames", type=int, default=8, help="distinct input frames cycled in the timed loop")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--autotune", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(w, frames_host, seconds):
+    """The FP32 scalar transcription (oracle, OpenMP over rows) on a bounded sample of the same
+    workload: whole frames, resolve + CAS, until `seconds` have elapsed (at least 2 frames)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import _oracle
+    from taastar.workloads import cas_for, decode, uniforms_for, velocity_as_read
+    threads = os.cpu_count() or 1
+    H, W = w["height"], w["width"]
+    hist = np.zeros((H, W, 4), np.float32)
+    k = cas_for(w)
+    t0 = time.perf_counter()
+    n = 0
+    prev_dep = frames_host[0][1]
+    while True:
+        col_s, dep, vel_s, jit = frames_host[n % len(frames_host)]
+        U = uniforms_for(w, n + 1, jit)
+        o = _oracle.resolve(U, decode(col_s), dep, velocity_as_read(vel_s), hist, prev_dep, threads=threads,
+                            debug=False, inspect=False)
+        scr = o["screen"]
+        if k is not None:
+            scr = _oracle.cas(scr, k.const1[0], threads=threads)
+        hist = o["history"]
+        prev_dep = dep
+        n += 1
+        el = time.perf_counter() - t0
+        if n >= 2 and el >= seconds:
+            break
+    return dict(value=round(n * W * H / el / 1e6, 3), unit="Mpix/s", cores=threads, kind="port",
+                sample=f"{n} full {W}x{H} frames of the same workload (resolve + CAS, FP32), {el:.1f} s wall, "
+                       f"OpenMP over rows on {threads} threads of {cpu_model()}")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU (asked for {a.gpus}): sessions are independent and shard across GPUs "
+              "with no exchange, so the per-GPU figure is the one reported.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import taastar
+    from taastar import FrameImages, check, lib
+    from taastar.workloads import (bytes_per_pixel, cas_for, encode_color, encode_velocity, scene_for,
+                                   uniforms_for, workload)
+
+    w = workload(a.workload)
+    W, H = w["width"], w["height"]
+    arith = taastar.ARITH_FP16 if a.arith == "fp16" else taastar.ARITH_FP32
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.Stream(device=dev)
+
+    # ---- inputs: F distinct frames, generated on the host, resident on the device
+    sc = scene_for(w)
+    host = []
+    for f in range(a.frames):
+        col, dep, vel4, jit = sc.render(f)
+        host.append((encode_color(col, w["fmt"]), dep, encode_velocity(vel4, w["vel"]), jit))
+    cols = torch.stack([torch.from_numpy(h[0]) for h in host]).to(dev)
+    deps = torch.stack([torch.from_numpy(h[1]) for h in host]).to(dev)
+    vels = torch.stack([torch.from_numpy(h[2]) for h in host]).to(dev)
+    cd = cols.dtype
+    hist = [torch.zeros(H, W, 4, dtype=cd, device=dev) for _ in range(2)]
+    screen = torch.zeros(H, W, 4, dtype=cd, device=dev)
+    cas = cas_for(w)
+    unis = [uniforms_for(w, 1, h[3]) for h in host]
+    u_reset = uniforms_for(w, 0, host[0][3])
+    torch.cuda.synchronize()
+
+    if a.autotune:
+        ctx = C.c_void_p()
+        check(lib().taa_ctx_create(0, 0, C.byref(ctx)))
+        bi, bus = C.c_int32(), C.c_float()
+        check(lib().taa_autotune(ctx, W, H, w["fmt"], arith, int(cas is not None), 20, C.byref(bi), C.byref(bus)))
+        lib().taa_ctx_destroy(ctx)
+
+    sptr = C.c_void_p(stream.cuda_stream)
+    imgs = []
+    for i in range(2 * a.frames):  # frame index i % F, history parity i % 2
+        f = i % a.frames
+        img = FrameImages()
+        img.color = cols[f].data_ptr()
+        img.depth = deps[f].data_ptr()
+        img.history = hist[(i + 1) & 1].data_ptr()
+        img.history_depth = deps[(f - 1) % a.frames].data_ptr()
+        img.result_screen = screen.data_ptr()
+        img.velocity = vels[f].data_ptr()
+        img.result_history = hist[i & 1].data_ptr()
+        img.width, img.height, img.pitch = W, H, W
+        img.color_format, img.velocity_format = w["fmt"], w["vel"]
+        imgs.append(img)
+    casp = C.byref(cas) if cas is not None else None
+    L = lib()
+
+    def launch(i, reset=False):
+        k = i % len(imgs)
+        U = u_reset if reset else unis[k % a.frames]
+        rc = L.taa_resolve_frame(C.byref(U), C.byref(imgs[k]), casp, arith, sptr)
+        if rc:
+            check(rc)
+
+    with torch.cuda.stream(stream):
+        launch(0, reset=True)
+        for i in range(1, a.warmup + 1):
+            launch(i)
+    stream.synchronize()
+
+    # ---- timed region: K back-to-back resolves, events between launches on the same stream
+    K = a.steps
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        ev[0].record(stream)
+        for i in range(K):
+            launch(a.warmup + 1 + i)
+            ev[i + 1].record(stream)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = ev[0].elapsed_time(ev[K])
+    per = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(K))
+    avg_ms = total_ms / K
+    med_ms = per[K // 2]
+    px = W * H
+    bpp = bytes_per_pixel(w)
+    achieved = bpp * px / (avg_ms * 1e-3) / 1e9
+    value = K * px / (total_ms * 1e-3) / 1e9
+    out = {
+        "metric": "TAA resolve Gpix/s & %HBM peak at 4K; p99 latency, 64 mixed-res sessions",
+        "value": round(value, 3),
+        "unit": "Gpix/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": round(avg_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16" if arith == taastar.ARITH_FP16 else "f32",
+        "data": "synthetic (seeded polygon scene, 8 frames cycled, resident in HBM)",
+        "config": {
+            "workload": f"{a.workload}: {W}x{H} TAA resolve + fused CAS 0.5, "
+                        f"{'RGBA16F' if w['fmt'] else 'RGBA32F'} colour/history, "
+                        f"{'RG16F' if w['vel'] == 1 else 'RG32F'} velocity, D32F depth",
+            "frame_pixels": px,
+            "arith": a.arith,
+            "tile_shape": L.taa_tile_shape_selected(W, H, w["fmt"], arith, int(cas is not None)),
+            "wall_s": round(wall, 4),
+            "median_step_ms": round(med_ms, 5),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_pixel": bpp,
+        },
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, host[:2], a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,152 @@
+// TEST INFRASTRUCTURE -- the oracle's C entry points (loaded by tests/ through ctypes and by
+// bench.py's cpu_baseline leg). Nothing in the product links or loads this library.
+//
+// Parity status of each oracle function (how it is pinned, see tests/test_oracle.py):
+//   halton / jitter selection  closed-form Halton KATs; jitter tables re-derived from them
+//   cas_setup                  oracle/_ref (the reference's own ffx_a.h + ffx_cas.h compiled
+//                              for the CPU) at many sharpness values; closed form -1/lerp(8,5,s)
+//   rgb<->YCoCg, tonemap       round trip on dyadic values; closed forms
+//   neighbourhood box          brute force on 3x3 patches (min/max order statistics; variance
+//                              against numpy mean/std); invariants min<=centre<=max
+//   closest-depth tap          brute force incl. all-equal ties -> tap 0
+//   velocity / reprojection    constant fields give historyUv = uv - v; identity matrices
+//   Catmull-Rom                9-tap (bilinear) form == 16-tap form; exact on linear ramps
+//                              and at texel centres; textbook Keys(a=-1/2) kernel in numpy
+//   clip / blend / rejection   closed forms: inside box -> identity, alpha=1 -> current, ...
+//   CasFilter                  numpy restatement with the same integer steps; amp = 0 case;
+//                              output in [0,1]
+// Parity unpinned (no reference run is possible here; documented definitions are used):
+//   sampler weight quantisation, out-of-bounds loads, GLSL sin() noise, FMA contraction.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "taa_oracle.hpp"
+
+namespace {
+
+template <class T>
+int resolve_frame(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+                  const float* vel4, const T* hist, const float* hdepth, T* screen, T* hist_out,
+                  T* debug, int32_t* inspect, int cr16, int threads) {
+    if (!U || W <= 0 || H <= 0 || !color || !depth || !vel4 || !hist || !hdepth || !screen || !hist_out) return -1;
+    oracle::Frame<T> F;
+    F.W = W; F.H = H; F.color = color; F.depth = depth; F.vel = vel4; F.hist = hist; F.hdepth = hdepth;
+    if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            oracle::PixelOut<T> o = oracle::resolve_pixel<T>(*U, F, oracle::iv2{x, y}, cr16 != 0);
+            size_t i = (size_t)y * W + x;
+            T* s = screen + 4 * i;
+            s[0] = o.screen.x; s[1] = o.screen.y; s[2] = o.screen.z; s[3] = o.screen.w;
+            T* h = hist_out + 4 * i;
+            h[0] = o.history.x; h[1] = o.history.y; h[2] = o.history.z; h[3] = o.history.w;
+            if (debug) {
+                T* d = debug + 4 * i;
+                d[0] = o.debug.x; d[1] = o.debug.y; d[2] = o.debug.z; d[3] = o.debug.w;
+            }
+            if (inspect) std::memcpy(inspect + (size_t)TAA_INSPECT_WORDS * i, o.inspect, sizeof(o.inspect));
+        }
+    }
+    return 0;
+}
+
+template <class T>
+int cas_frame(const T* src, T* dst, int W, int H, uint32_t const1_x, int threads) {
+    if (!src || !dst || W <= 0 || H <= 0) return -1;
+    if (threads <= 0) threads = 1;
+    auto load = [&](int x, int y) {
+        x = oracle::iclamp(x, 0, W - 1);
+        y = oracle::iclamp(y, 0, H - 1);
+        const T* p = src + 4 * ((size_t)y * W + x);
+        return oracle::v3<T>{p[0], p[1], p[2]};
+    };
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            oracle::v3<T> c = oracle::cas_filter<T>(load, x, y, const1_x);
+            T* d = dst + 4 * ((size_t)y * W + x);
+            d[0] = c.x; d[1] = c.y; d[2] = c.z; d[3] = T(1);
+        }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_resolve_f32(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+                       const float* vel4, const float* hist, const float* hdepth, float* screen,
+                       float* hist_out, float* debug, int32_t* inspect, int cr16, int threads) {
+    return resolve_frame<float>(U, W, H, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, cr16, threads);
+}
+
+int oracle_resolve_f64(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+                       const float* vel4, const double* hist, const float* hdepth, double* screen,
+                       double* hist_out, double* debug, int32_t* inspect, int cr16, int threads) {
+    return resolve_frame<double>(U, W, H, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, cr16, threads);
+}
+
+int oracle_cas_f32(const float* src, float* dst, int W, int H, uint32_t const1_x, int threads) {
+    return cas_frame<float>(src, dst, W, H, const1_x, threads);
+}
+
+int oracle_cas_f64(const double* src, double* dst, int W, int H, uint32_t const1_x, int threads) {
+    return cas_frame<double>(src, dst, W, H, const1_x, threads);
+}
+
+void oracle_cas_setup(uint32_t c0[4], uint32_t c1[4], float s, float inX, float inY, float outX, float outY) {
+    oracle::cas_setup(c0, c1, s, inX, inY, outX, outY);
+}
+
+float oracle_halton(int i, int b) { return oracle::halton(i, b); }
+
+uint32_t oracle_half_bits_trunc(float f) { return oracle::half_bits_trunc(f); }
+
+// get_jitter_offset_for_frame (source/taa.hpp:150-233) with the pixel size of (W, H).
+int oracle_jitter(int dist, uint64_t frame, int slow_motion, int fixed_index, float rotate_deg,
+                  float extra_scale, int n_user, const float* user, uint32_t W, uint32_t H,
+                  float out[2], uint32_t* n_out) {
+    float px = 2.0f / float(W), py = 2.0f / float(H);
+    std::vector<float> ox, oy;
+    auto add = [&](float x, float y) { ox.push_back(px * x); oy.push_back(py * y); };
+    const float e = 1.f / 8.f;
+    float sx = 1.f, sy = 1.f;
+    switch (dist) {
+        case 0: add(-0.25f, -0.25f); add(0.25f, -0.25f); add(0.25f, 0.25f); add(-0.25f, 0.25f); break;
+        case 1: add(-0.25f, -0.25f); add(0.25f, 0.25f); add(0.25f, -0.25f); add(-0.25f, 0.25f); break;
+        case 2:
+        case 3: {
+            int n = dist == 2 ? 8 : 16;
+            for (int i = 0; i < n; ++i) add(oracle::halton(i + 1, 2) - 0.5f, oracle::halton(i + 1, 3) - 0.5f);
+            break;
+        }
+        case 4:
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c) add(float(2 * c - 3) * e, float(2 * r - 3) * e);
+            break;
+        case 5:
+            if (n_user <= 0 || !user) return -1;
+            for (int i = 0; i < n_user; ++i) { ox.push_back(user[2 * i]); oy.push_back(user[2 * i + 1]); }
+            sx = px; sy = py;
+            break;
+        default: return -1;
+    }
+    if (slow_motion > 1) frame /= (uint64_t)slow_motion;
+    if (fixed_index >= 0) frame = (uint64_t)fixed_index;
+    size_t k = (size_t)(frame % ox.size());
+    float x = ox[k] * sx, y = oy[k] * sy;
+    if (rotate_deg != 0.f) {
+        float rad = rotate_deg * 0.01745329251994329576923690768489f;
+        float s = std::sin(rad), c = std::cos(rad);
+        float nx = x * c - y * s, ny = x * s + y * c;
+        x = nx; y = ny;
+    }
+    out[0] = x * extra_scale;
+    out[1] = y * extra_scale;
+    if (n_out) *n_out = (uint32_t)ox.size();
+    return 0;
+}
+
+}  // extern "C"

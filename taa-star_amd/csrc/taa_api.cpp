@@ -1,0 +1,758 @@
+// Host runtime behind the C ABI (include/taastar/taa_abi.h): argument checking, the per-frame
+// launch, sessions with HBM-budget admission, batched and graph-captured launches, autotuner.
+// The MI355X counterpart of the host driver taa<CF> (source/taa.hpp).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "taa_internal.hpp"
+
+// ---- layout of the reference structs ------------------------------------------------------
+static_assert(sizeof(taa_params_t) == 176, "Parameters must be 176 bytes (source/taa.hpp:30-77)");
+static_assert(offsetof(taa_params_t, mVarClipGamma) == 48, "layout");
+static_assert(offsetof(taa_params_t, mRejectOutside) == 64, "layout");
+static_assert(offsetof(taa_params_t, mInterpolationMode) == 76, "layout");
+static_assert(offsetof(taa_params_t, mRayTraceHistoryCount) == 140, "layout");
+static_assert(offsetof(taa_params_t, mDebugMask) == 144, "layout");
+static_assert(offsetof(taa_params_t, mDebugToScreenOutput) == 172, "layout");
+static_assert(sizeof(taa_uniforms_t) == 544, "uniforms_for_taa must be 544 bytes (source/taa.hpp:113-129)");
+static_assert(offsetof(taa_uniforms_t, param) == 128, "layout");
+static_assert(offsetof(taa_uniforms_t, mJitterNdc) == 480, "layout");
+static_assert(offsetof(taa_uniforms_t, mSinTime) == 496, "layout");
+static_assert(offsetof(taa_uniforms_t, splitScreen) == 512, "layout");
+static_assert(offsetof(taa_uniforms_t, mResetHistory) == 528, "layout");
+static_assert(offsetof(taa_uniforms_t, mCamNearPlane) == 532, "layout");
+static_assert(offsetof(taa_uniforms_t, pad1) == 540, "layout");
+static_assert(sizeof(taa_cas_consts_t) == 32, "push_constants_for_cas must be 32 bytes (source/taa.hpp:88-91)");
+static_assert(sizeof(taastar::BatchEntry) % 16 == 0, "BatchEntry alignment");
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+int ok() { return TAA_OK; }
+int hip_fail(hipError_t e, const char* what) { return fail(TAA_ERR_HIP, "%s: %s", what, hipGetErrorString(e)); }
+
+size_t color_bpp(int fmt) { return fmt == TAA_FMT_RGBA16F ? 8 : 16; }
+size_t vel_bpp(int fmt) {
+    switch (fmt) {
+        case TAA_VEL_RG32F: return 8;
+        case TAA_VEL_RG16F: return 4;
+        case TAA_VEL_RGBA32F: return 16;
+        default: return 8;
+    }
+}
+bool valid_color_fmt(int f) { return f == TAA_FMT_RGBA32F || f == TAA_FMT_RGBA16F; }
+bool valid_vel_fmt(int f) { return f >= TAA_VEL_RG32F && f <= TAA_VEL_RGBA16F; }
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// ---- process-wide tile shape table ----------------------------------------------------------
+using ClassKey = std::tuple<int, int, int, int, int>;  // W, H, fmt, arith, cas
+std::mutex g_tune_mu;
+std::map<ClassKey, int> g_tune;
+
+int selected_shape(int W, int H, int fmt, int arith, bool cas) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(ClassKey{W, H, fmt, arith, cas ? 1 : 0});
+    if (it != g_tune.end()) return it->second;
+    return cas ? taastar::kDefaultShapeCas : taastar::kDefaultShapePlain;
+}
+
+taastar::TileShapeDef shape_def(int idx, bool cas) { return cas ? taastar::kShapesCas[idx] : taastar::kShapesPlain[idx]; }
+
+// Fill the kernel-side frame descriptor; validates the images.
+int make_desc(const taa_frame_images_t* img, const taa_cas_consts_t* cas, taastar::FrameDesc& fd) {
+    if (!img) return fail(TAA_ERR_INVALID_ARG, "images is NULL");
+    if (img->width <= 0 || img->height <= 0 || img->width > 32768 || img->height > 32768)
+        return fail(TAA_ERR_INVALID_ARG, "bad frame size %dx%d", img->width, img->height);
+    const int pitch = img->pitch == 0 ? img->width : img->pitch;
+    if (pitch < img->width) return fail(TAA_ERR_INVALID_ARG, "pitch %d < width %d", pitch, img->width);
+    if (!valid_color_fmt(img->color_format)) return fail(TAA_ERR_INVALID_ARG, "bad color_format %d", img->color_format);
+    if (!valid_vel_fmt(img->velocity_format)) return fail(TAA_ERR_INVALID_ARG, "bad velocity_format %d", img->velocity_format);
+    if (img->reserved != 0) return fail(TAA_ERR_INVALID_ARG, "reserved must be 0");
+    if (!img->color || !img->depth || !img->history || !img->history_depth || !img->result_screen || !img->velocity ||
+        !img->result_history)
+        return fail(TAA_ERR_INVALID_ARG, "a required image pointer is NULL");
+    if (img->history == img->result_history) return fail(TAA_ERR_INVALID_ARG, "history and result_history alias");
+    if (img->result_screen == img->result_history) return fail(TAA_ERR_INVALID_ARG, "result_screen and result_history alias");
+    if (cas && (img->result_screen == img->color || img->result_screen == img->history))
+        return fail(TAA_ERR_INVALID_ARG, "result_screen aliases an input");
+    std::memset(&fd, 0, sizeof(fd));
+    fd.color = img->color;
+    fd.depth = img->depth;
+    fd.history = img->history;
+    fd.hdepth = img->history_depth;
+    fd.screen = img->result_screen;
+    fd.debug = img->debug;
+    fd.velocity = img->velocity;
+    fd.history_out = img->result_history;
+    fd.inspect = img->inspect;
+    fd.W = img->width;
+    fd.H = img->height;
+    fd.pitch = pitch;
+    fd.color_fmt = img->color_format;
+    fd.vel_fmt = img->velocity_format;
+    fd.x_begin = 0;
+    fd.x_end = img->width;
+    fd.param_idx = 0;
+    fd.cas_on = cas ? 1 : 0;
+    fd.cas_peak_bits = cas ? cas->const1[0] : 0u;
+    return TAA_OK;
+}
+
+int check_uniforms(const taa_uniforms_t* u) {
+    if (!u) return fail(TAA_ERR_INVALID_ARG, "uniforms is NULL");
+    if (u->mUpsampling) return fail(TAA_ERR_UNSUPPORTED, "mUpsampling (TAAU) is not built");
+    for (int i = 0; i < 2; ++i) {
+        if (u->param[i].mRayTraceAugment) return fail(TAA_ERR_UNSUPPORTED, "mRayTraceAugment is not built");
+        if (u->param[i].mInterpolationMode < 0 || u->param[i].mInterpolationMode > 2)
+            return fail(TAA_ERR_INVALID_ARG, "mInterpolationMode %d", u->param[i].mInterpolationMode);
+    }
+    return TAA_OK;
+}
+
+// Launch the resolve over [x_begin, x_end) with the given parameter block.
+int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, int x_end, int param_idx, int arith, void* stream) {
+    if (x_end <= x_begin) return TAA_OK;
+    const bool cas = fd.cas_on != 0;
+    const int shape = selected_shape(fd.W, fd.H, fd.color_fmt, arith, cas);
+    const taastar::TileShapeDef s = shape_def(shape, cas);
+    fd.x_begin = x_begin;
+    fd.x_end = x_end;
+    fd.param_idx = param_idx;
+    fd.tiles_x = taastar::tiles_for(x_end - x_begin, s.tw);
+    fd.tiles_y = taastar::tiles_for(fd.H, s.th);
+    int e = taastar::launch_resolve(fd, U, shape, arith, stream);
+    if (e != 0) return hip_fail((hipError_t)e, "resolve launch");
+    return TAA_OK;
+}
+
+int resolve_impl(const taa_uniforms_t* u, const taa_frame_images_t* img, const taa_cas_consts_t* cas, int arith, void* stream) {
+    int rc = check_uniforms(u);
+    if (rc) return rc;
+    if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith %d", arith);
+    taastar::FrameDesc fd;
+    rc = make_desc(img, cas, fd);
+    if (rc) return rc;
+    if (!u->splitScreen) return launch_range(fd, *u, 0, fd.W, 0, arith, stream);
+    if (cas) return fail(TAA_ERR_UNSUPPORTED, "split screen with CAS: resolve without CAS, then taa_cas_apply");
+    // paramsIdx = (splitScreen && x > splitX) ? 1 : 0 (taa.comp:717)
+    const int split = std::min(std::max(u->splitX + 1, 0), fd.W);
+    rc = launch_range(fd, *u, 0, split, 0, arith, stream);
+    if (rc) return rc;
+    return launch_range(fd, *u, split, fd.W, 1, arith, stream);
+}
+
+}  // namespace
+
+// ---- contexts and sessions ----------------------------------------------------------------
+
+struct taa_session;
+
+struct taa_ctx {
+    int device = 0;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> used{0};
+    std::mutex mu;
+    // batch staging ring: pinned host mirror + device copy of entries and uniforms
+    static constexpr int kSlots = 4;
+    static constexpr int kCap = 1024;
+    struct Slot {
+        void* host = nullptr;
+        void* dev = nullptr;
+        hipEvent_t done = nullptr;
+        bool used = false;
+    } slots[kSlots];
+    int next_slot = 0;
+    hipStream_t tune_stream = nullptr;
+};
+
+struct taa_session {
+    taa_ctx* ctx = nullptr;
+    int W = 0, H = 0, pitch = 0, cf = 0, vf = 0;
+    uint64_t bytes = 0;
+    void* base = nullptr;
+    void* hist[2] = {nullptr, nullptr};
+    float* depth[2] = {nullptr, nullptr};
+    void* color = nullptr;
+    void* vel = nullptr;
+    uint64_t frames = 0;
+    bool force_reset = true;
+};
+
+namespace {
+
+size_t staging_bytes() { return align256(sizeof(taastar::BatchEntry) * taa_ctx::kCap) + sizeof(taa_uniforms_t) * taa_ctx::kCap; }
+
+// Frame descriptor for the session's next frame (frame k reads depth[k&1], history[(k+1)&1] and
+// writes history[k&1]; the previous depth is depth[(k+1)&1]).
+void session_desc(taa_session* s, void* screen, const taa_cas_consts_t* cas, taastar::FrameDesc& fd) {
+    const int k = (int)(s->frames & 1);
+    std::memset(&fd, 0, sizeof(fd));
+    fd.color = s->color;
+    fd.depth = s->depth[k];
+    fd.history = s->hist[k ^ 1];
+    fd.hdepth = s->depth[k ^ 1];
+    fd.screen = screen;
+    fd.velocity = s->vel;
+    fd.history_out = s->hist[k];
+    fd.W = s->W;
+    fd.H = s->H;
+    fd.pitch = s->pitch;
+    fd.color_fmt = s->cf;
+    fd.vel_fmt = s->vf;
+    fd.x_begin = 0;
+    fd.x_end = s->W;
+    fd.cas_on = cas ? 1 : 0;
+    fd.cas_peak_bits = cas ? cas->const1[0] : 0u;
+}
+
+void session_advance(taa_session* s) {
+    s->frames++;
+    s->force_reset = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t taa_abi_version(void) { return TAASTAR_ABI_VERSION; }
+const char* taa_last_error(void) { return g_err.c_str(); }
+
+int32_t taa_params_defaults(taa_params_t* p) {
+    if (!p) return fail(TAA_ERR_INVALID_ARG, "out is NULL");
+    std::memset(p, 0, sizeof(*p));
+    // source/taa.hpp:31-76
+    p->mAlpha = 0.05f;
+    p->mColorClampingOrClipping = 1;
+    p->mUnjitterFactor = 1.0f;
+    p->mVarClipGamma = 1.0f;
+    p->mMinAlpha = 1.0f - 0.97f;
+    p->mMaxAlpha = 1.0f - 0.88f;
+    p->mRejectionAlpha = 1.0f;
+    p->mUseVelocityVectors = 1;
+    p->mNoiseFactor = 1.f / 510.f;
+    p->mVelBasedAlphaMax = 0.2f;
+    p->mVelBasedAlphaFactor = 1.f / 40.f;
+    p->mRayTraceAugmentFlags = 0xffffffffu & ~(0x0080u | 0x0100u);  // ~(TAA_RTFLAG_ALL | TAA_RTFLAG_FXD)
+    p->mRayTraceAugment_WNrm = 0.5f;
+    p->mRayTraceAugment_WDpt = 0.015f;
+    p->mRayTraceAugment_WMId = 0.25f;
+    p->mRayTraceAugment_WLum = 0.5f;
+    p->mRayTraceAugment_Thresh = 0.5f;
+    p->mRayTraceHistoryCount = -1;
+    p->mDebugMask[0] = p->mDebugMask[1] = p->mDebugMask[2] = 1.0f;
+    p->mDebugScale = 1.0f;
+    return ok();
+}
+
+int32_t taa_resolve_frame(const taa_uniforms_t* uniforms, const taa_frame_images_t* images, const taa_cas_consts_t* cas,
+                          int32_t arith, void* stream) {
+    return resolve_impl(uniforms, images, cas, arith, stream);
+}
+
+int32_t taa_cas_apply(const void* src, void* dst, int32_t width, int32_t height, int32_t pitch, int32_t format,
+                      const taa_cas_consts_t* cas, void* stream) {
+    if (!src || !dst || !cas) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
+    if (src == dst) return fail(TAA_ERR_INVALID_ARG, "src and dst alias");
+    if (width <= 0 || height <= 0) return fail(TAA_ERR_INVALID_ARG, "bad size");
+    if (!valid_color_fmt(format)) return fail(TAA_ERR_INVALID_ARG, "bad format");
+    if (pitch == 0) pitch = width;
+    if (pitch < width) return fail(TAA_ERR_INVALID_ARG, "pitch < width");
+    int e = taastar::launch_cas(src, dst, width, height, pitch, format, cas->const1[0], stream);
+    if (e) return hip_fail((hipError_t)e, "cas launch");
+    return ok();
+}
+
+int32_t taa_ctx_create(int32_t device, uint64_t hbm_budget_bytes, taa_ctx_t** out) {
+    if (!out) return fail(TAA_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(TAA_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= n) return fail(TAA_ERR_INVALID_ARG, "device %d of %d", device, n);
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(TAA_ERR_NO_DEVICE, "device %d is %s; this build targets gfx950 only", device, prop.gcnArchName);
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    taa_ctx* c = new (std::nothrow) taa_ctx();
+    if (!c) return fail(TAA_ERR_OUT_OF_MEMORY, "host allocation");
+    c->device = device;
+    if (hbm_budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        if ((e = hipMemGetInfo(&fr, &tot)) != hipSuccess) { delete c; return hip_fail(e, "hipMemGetInfo"); }
+        hbm_budget_bytes = (uint64_t)(0.9 * (double)fr);
+    }
+    c->budget = hbm_budget_bytes;
+    for (auto& s : c->slots) {
+        if (hipHostMalloc(&s.host, staging_bytes(), hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&s.dev, staging_bytes()) != hipSuccess || hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+            taa_ctx_destroy(c);
+            return fail(TAA_ERR_OUT_OF_MEMORY, "batch staging allocation");
+        }
+    }
+    *out = c;
+    return ok();
+}
+
+int32_t taa_ctx_destroy(taa_ctx_t* c) {
+    if (!c) return fail(TAA_ERR_INVALID_ARG, "ctx is NULL");
+    hipSetDevice(c->device);
+    for (auto& s : c->slots) {
+        if (s.done) { hipEventSynchronize(s.done); hipEventDestroy(s.done); }
+        if (s.dev) hipFree(s.dev);
+        if (s.host) hipHostFree(s.host);
+    }
+    if (c->tune_stream) hipStreamDestroy(c->tune_stream);
+    delete c;
+    return ok();
+}
+
+int32_t taa_ctx_usage(const taa_ctx_t* c, uint64_t* used, uint64_t* budget) {
+    if (!c) return fail(TAA_ERR_INVALID_ARG, "ctx is NULL");
+    if (used) *used = c->used.load();
+    if (budget) *budget = c->budget;
+    return ok();
+}
+
+uint64_t taa_session_bytes(int32_t w, int32_t h, int32_t cf, int32_t vf) {
+    if (w <= 0 || h <= 0 || !valid_color_fmt(cf) || !valid_vel_fmt(vf)) return 0;
+    const size_t px = (size_t)w * (size_t)h;
+    return 3 * align256(px * color_bpp(cf)) + 2 * align256(px * 4) + align256(px * vel_bpp(vf));
+}
+
+int32_t taa_session_open(taa_ctx_t* c, int32_t w, int32_t h, int32_t cf, int32_t vf, taa_session_t** out) {
+    if (!c || !out) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    if (w <= 0 || h <= 0 || w > 32768 || h > 32768) return fail(TAA_ERR_INVALID_ARG, "bad size %dx%d", w, h);
+    if (!valid_color_fmt(cf) || !valid_vel_fmt(vf)) return fail(TAA_ERR_INVALID_ARG, "bad format");
+    const uint64_t bytes = taa_session_bytes(w, h, cf, vf);
+    // admission: reserve the bytes against the budget first
+    uint64_t cur = c->used.load();
+    do {
+        if (cur + bytes > c->budget)
+            return fail(TAA_ERR_BUDGET, "HBM budget: %llu + %llu > %llu bytes", (unsigned long long)cur,
+                        (unsigned long long)bytes, (unsigned long long)c->budget);
+    } while (!c->used.compare_exchange_weak(cur, cur + bytes));
+    taa_session* s = new (std::nothrow) taa_session();
+    if (!s) { c->used -= bytes; return fail(TAA_ERR_OUT_OF_MEMORY, "host allocation"); }
+    hipSetDevice(c->device);
+    hipError_t e = hipMalloc(&s->base, bytes);
+    if (e != hipSuccess) {
+        c->used -= bytes;
+        delete s;
+        return fail(TAA_ERR_OUT_OF_MEMORY, "hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
+    }
+    // zero everything once so the first (reset) frame never blends non-finite garbage
+    if ((e = hipMemset(s->base, 0, bytes)) != hipSuccess) {
+        hipFree(s->base);
+        c->used -= bytes;
+        delete s;
+        return hip_fail(e, "hipMemset");
+    }
+    const size_t px = (size_t)w * (size_t)h;
+    char* p = static_cast<char*>(s->base);
+    s->hist[0] = p; p += align256(px * color_bpp(cf));
+    s->hist[1] = p; p += align256(px * color_bpp(cf));
+    s->color = p; p += align256(px * color_bpp(cf));
+    s->depth[0] = reinterpret_cast<float*>(p); p += align256(px * 4);
+    s->depth[1] = reinterpret_cast<float*>(p); p += align256(px * 4);
+    s->vel = p;
+    s->ctx = c;
+    s->W = w; s->H = h; s->pitch = w; s->cf = cf; s->vf = vf;
+    s->bytes = bytes;
+    *out = s;
+    return ok();
+}
+
+int32_t taa_session_close(taa_session_t* s) {
+    if (!s) return fail(TAA_ERR_INVALID_ARG, "session is NULL");
+    hipSetDevice(s->ctx->device);
+    hipDeviceSynchronize();  // the session's buffers may still be read by queued work
+    hipFree(s->base);
+    s->ctx->used -= s->bytes;
+    delete s;
+    return ok();
+}
+
+int32_t taa_session_io(taa_session_t* s, taa_session_io_t* out) {
+    if (!s || !out) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
+    const int k = (int)(s->frames & 1);
+    out->color = s->color;
+    out->depth = s->depth[k];
+    out->velocity = s->vel;
+    out->history = s->hist[k ^ 1];
+    out->width = s->W; out->height = s->H; out->pitch = s->pitch;
+    out->color_format = s->cf; out->velocity_format = s->vf;
+    out->frames_resolved = s->frames;
+    return ok();
+}
+
+int32_t taa_session_reset(taa_session_t* s) {
+    if (!s) return fail(TAA_ERR_INVALID_ARG, "session is NULL");
+    s->force_reset = true;
+    return ok();
+}
+
+int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* u, const taa_cas_consts_t* cas, void* screen,
+                            int32_t arith, void* stream) {
+    if (!s || !u || !screen) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
+    taa_uniforms_t U = *u;
+    if (s->force_reset) U.mResetHistory = 1;
+    const int k = (int)(s->frames & 1);
+    taa_frame_images_t img;
+    std::memset(&img, 0, sizeof(img));
+    img.color = s->color;
+    img.depth = s->depth[k];
+    img.history = s->hist[k ^ 1];
+    img.history_depth = s->depth[k ^ 1];
+    img.result_screen = screen;
+    img.velocity = s->vel;
+    img.result_history = s->hist[k];
+    img.width = s->W; img.height = s->H; img.pitch = s->pitch;
+    img.color_format = s->cf; img.velocity_format = s->vf;
+    int rc = resolve_impl(&U, &img, cas, arith, stream);
+    if (rc) return rc;
+    session_advance(s);
+    return ok();
+}
+
+// ---- batches ------------------------------------------------------------------------------
+
+static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int arith, void* host, void* dev, int& shape,
+                       bool& cas, bool& s16, int& total_tiles) {
+    if (n <= 0 || n > taa_ctx::kCap) return fail(TAA_ERR_CAPACITY, "batch of %d items (capacity %d)", n, taa_ctx::kCap);
+    auto* ents = static_cast<taastar::BatchEntry*>(host);
+    auto* unis = reinterpret_cast<taa_uniforms_t*>(static_cast<char*>(host) + align256(sizeof(taastar::BatchEntry) * taa_ctx::kCap));
+    auto* dunis = reinterpret_cast<taa_uniforms_t*>(static_cast<char*>(dev) + align256(sizeof(taastar::BatchEntry) * taa_ctx::kCap));
+    cas = items[0].cas != nullptr;
+    s16 = items[0].session && items[0].session->cf == TAA_FMT_RGBA16F;
+    // one tile shape per launch: the one tuned for the largest frame class in the batch
+    int best_px = -1;
+    for (int i = 0; i < n; ++i) {
+        const taa_batch_item_t& it = items[i];
+        if (!it.session || !it.uniforms || !it.result_screen) return fail(TAA_ERR_INVALID_ARG, "item %d: NULL field", i);
+        if (it.session->ctx != c) return fail(TAA_ERR_INVALID_ARG, "item %d: session of another context", i);
+        if ((it.cas != nullptr) != cas) return fail(TAA_ERR_INVALID_ARG, "item %d: CAS on/off must match the batch", i);
+        if ((it.session->cf == TAA_FMT_RGBA16F) != s16) return fail(TAA_ERR_INVALID_ARG, "item %d: colour format must match", i);
+        for (int j = 0; j < i; ++j)
+            if (items[j].session == it.session) return fail(TAA_ERR_INVALID_ARG, "item %d: session repeated", i);
+        int rc = check_uniforms(it.uniforms);
+        if (rc) return rc;
+        if (it.uniforms->splitScreen) return fail(TAA_ERR_UNSUPPORTED, "item %d: split screen in a batch", i);
+        const int px = it.session->W * it.session->H;
+        if (px > best_px) {
+            best_px = px;
+            shape = selected_shape(it.session->W, it.session->H, it.session->cf, arith, cas);
+        }
+    }
+    const taastar::TileShapeDef sd = shape_def(shape, cas);
+    int tiles = 0;
+    for (int i = 0; i < n; ++i) {
+        taa_session* s = items[i].session;
+        taastar::BatchEntry& e = ents[i];
+        std::memset(&e, 0, sizeof(e));
+        session_desc(s, items[i].result_screen, items[i].cas, e.fd);
+        e.fd.tiles_x = taastar::tiles_for(s->W, sd.tw);
+        e.fd.tiles_y = taastar::tiles_for(s->H, sd.th);
+        unis[i] = *items[i].uniforms;
+        if (s->force_reset) unis[i].mResetHistory = 1;
+        e.U = dunis + i;
+        e.tile_begin = tiles;
+        tiles += e.fd.tiles_x * e.fd.tiles_y;
+    }
+    total_tiles = tiles;
+    return TAA_OK;
+}
+
+int32_t taa_batch_submit(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items, int32_t arith, void* stream) {
+    if (!c || !items) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
+    if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith");
+    std::lock_guard<std::mutex> lk(c->mu);
+    taa_ctx::Slot& slot = c->slots[c->next_slot];
+    if (slot.used) {
+        hipError_t e = hipEventSynchronize(slot.done);  // the slot's previous batch has been consumed
+        if (e != hipSuccess) return hip_fail(e, "staging slot wait");
+    }
+    int shape = 0, total = 0;
+    bool cas = false, s16 = false;
+    int rc = build_batch(c, n, items, arith, slot.host, slot.dev, shape, cas, s16, total);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemcpyAsync(slot.dev, slot.host, staging_bytes(), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+    const int grid = taastar::batch_grid_size(shape, cas, s16, arith, c->device);
+    int le = taastar::launch_batch(static_cast<const taastar::BatchEntry*>(slot.dev), n, total, shape, cas, s16, arith, grid, stream);
+    if (le) return hip_fail((hipError_t)le, "batch launch");
+    if ((e = hipEventRecord(slot.done, st)) != hipSuccess) return hip_fail(e, "event record");
+    slot.used = true;
+    c->next_slot = (c->next_slot + 1) % taa_ctx::kSlots;
+    for (int i = 0; i < n; ++i) session_advance(items[i].session);
+    return ok();
+}
+
+}  // extern "C"
+
+// ---- graph-captured batch plans -----------------------------------------------------------
+
+struct taa_batch_plan {
+    taa_ctx* ctx = nullptr;
+    int n = 0, arith = 0;
+    std::vector<taa_batch_item_t> items;
+    std::vector<taa_uniforms_t> uniforms;  // per-item copy refreshed from the caller each launch
+    struct Buf {
+        void* host = nullptr;
+        void* dev = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t done = nullptr;
+        bool used = false;
+    } buf[2];
+    int next = 0;
+};
+
+namespace {
+
+void plan_free(taa_batch_plan* p) {
+    for (auto& b : p->buf) {
+        if (b.done) { hipEventSynchronize(b.done); hipEventDestroy(b.done); }
+        if (b.exec) hipGraphExecDestroy(b.exec);
+        if (b.dev) hipFree(b.dev);
+        if (b.host) hipHostFree(b.host);
+    }
+    delete p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t taa_batch_plan_create(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items, int32_t arith, taa_batch_plan_t** out) {
+    if (!c || !items || !out) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith");
+    taa_batch_plan* p = new (std::nothrow) taa_batch_plan();
+    if (!p) return fail(TAA_ERR_OUT_OF_MEMORY, "host allocation");
+    p->ctx = c;
+    p->n = n;
+    p->arith = arith;
+    p->items.assign(items, items + n);
+    hipSetDevice(c->device);
+    // validate once and fix the launch geometry (it does not change between launches)
+    int shape = 0, total = 0;
+    bool cas = false, s16 = false;
+    std::vector<char> scratch(staging_bytes());
+    std::vector<char> dscratch(staging_bytes());
+    int rc = build_batch(c, n, items, arith, scratch.data(), dscratch.data(), shape, cas, s16, total);
+    if (rc) { delete p; return rc; }
+    const int grid = taastar::batch_grid_size(shape, cas, s16, arith, c->device);
+    hipStream_t cap = nullptr;
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) { delete p; return fail(TAA_ERR_HIP, "stream"); }
+    for (auto& b : p->buf) {
+        hipError_t e;
+        if ((e = hipHostMalloc(&b.host, staging_bytes(), hipHostMallocDefault)) != hipSuccess ||
+            (e = hipMalloc(&b.dev, staging_bytes())) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming)) != hipSuccess) {
+            hipStreamDestroy(cap);
+            plan_free(p);
+            return fail(TAA_ERR_OUT_OF_MEMORY, "plan buffers: %s", hipGetErrorString(e));
+        }
+        hipGraph_t g = nullptr;
+        if ((e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+            hipStreamDestroy(cap);
+            plan_free(p);
+            return hip_fail(e, "begin capture");
+        }
+        hipMemcpyAsync(b.dev, b.host, staging_bytes(), hipMemcpyHostToDevice, cap);
+        taastar::launch_batch(static_cast<const taastar::BatchEntry*>(b.dev), n, total, shape, cas, s16, arith, grid, cap);
+        if ((e = hipStreamEndCapture(cap, &g)) != hipSuccess) {
+            hipStreamDestroy(cap);
+            plan_free(p);
+            return hip_fail(e, "end capture");
+        }
+        e = hipGraphInstantiate(&b.exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            hipStreamDestroy(cap);
+            plan_free(p);
+            return hip_fail(e, "graph instantiate");
+        }
+    }
+    hipStreamDestroy(cap);
+    *out = p;
+    return ok();
+}
+
+int32_t taa_batch_plan_launch(taa_batch_plan_t* p, void* stream) {
+    if (!p) return fail(TAA_ERR_INVALID_ARG, "plan is NULL");
+    auto& b = p->buf[p->next];
+    if (b.used) {
+        hipError_t e = hipEventSynchronize(b.done);  // this buffer's previous replay has finished
+        if (e != hipSuccess) return hip_fail(e, "plan buffer wait");
+    }
+    int shape = 0, total = 0;
+    bool cas = false, s16 = false;
+    int rc = build_batch(p->ctx, p->n, p->items.data(), p->arith, b.host, b.dev, shape, cas, s16, total);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = hipGraphLaunch(b.exec, st);
+    if (e != hipSuccess) return hip_fail(e, "graph launch");
+    if ((e = hipEventRecord(b.done, st)) != hipSuccess) return hip_fail(e, "event record");
+    b.used = true;
+    p->next ^= 1;
+    for (int i = 0; i < p->n; ++i) session_advance(p->items[i].session);
+    return ok();
+}
+
+int32_t taa_batch_plan_destroy(taa_batch_plan_t* p) {
+    if (!p) return fail(TAA_ERR_INVALID_ARG, "plan is NULL");
+    plan_free(p);
+    return ok();
+}
+
+// ---- autotuner ----------------------------------------------------------------------------
+
+int32_t taa_tile_shape_count(void) { return taastar::kNumShapes; }
+
+int32_t taa_tile_shape_get(int32_t index, taa_bool32 cas, taa_tile_shape_t* out) {
+    if (!out || index < 0 || index >= taastar::kNumShapes) return fail(TAA_ERR_INVALID_ARG, "bad index");
+    taastar::TileShapeDef d = shape_def(index, cas != 0);
+    out->tile_w = d.tw;
+    out->tile_h = d.th;
+    out->lds_pad = d.pad;
+    return ok();
+}
+
+int32_t taa_tile_shape_select(int32_t w, int32_t h, int32_t fmt, int32_t arith, taa_bool32 cas, int32_t index) {
+    if (index < -1 || index >= taastar::kNumShapes) return fail(TAA_ERR_INVALID_ARG, "bad index");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    ClassKey k{w, h, fmt, arith, cas ? 1 : 0};
+    if (index < 0) g_tune.erase(k);
+    else g_tune[k] = index;
+    return ok();
+}
+
+int32_t taa_tile_shape_selected(int32_t w, int32_t h, int32_t fmt, int32_t arith, taa_bool32 cas) {
+    return selected_shape(w, h, fmt, arith, cas != 0);
+}
+
+int32_t taa_autotune(taa_ctx_t* c, int32_t w, int32_t h, int32_t fmt, int32_t arith, taa_bool32 cas, int32_t iters,
+                     int32_t* best_index, float* best_us) {
+    if (!c) return fail(TAA_ERR_INVALID_ARG, "ctx is NULL");
+    if (w <= 0 || h <= 0 || !valid_color_fmt(fmt)) return fail(TAA_ERR_INVALID_ARG, "bad class");
+    if (iters <= 0) iters = 20;
+    hipSetDevice(c->device);
+    if (!c->tune_stream && hipStreamCreateWithFlags(&c->tune_stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(TAA_ERR_HIP, "stream");
+    hipStream_t st = c->tune_stream;
+    const size_t px = (size_t)w * h, cb = px * color_bpp(fmt);
+    // synthetic frame: colour ramps and a depth step so every branch of the config runs
+    std::vector<char> hc(cb), hv(px * 8);
+    std::vector<float> hd(px);
+    for (size_t i = 0; i < px; ++i) {
+        const int x = (int)(i % w), y = (int)(i / w);
+        float rgb[4] = {0.1f + 0.8f * (float)((x * 7 + y * 3) % 64) / 64.f, 0.3f + 0.01f * (float)(y % 17), 0.5f, 1.0f};
+        if (fmt == TAA_FMT_RGBA16F) {
+            for (int k = 0; k < 4; ++k) {  // fp16 of values in [0.1, 1]: exact enough for timing
+                uint16_t hb = 0;
+                float v = rgb[k];
+                uint32_t u;
+                std::memcpy(&u, &v, 4);
+                hb = (uint16_t)(((u >> 16) & 0x8000u) | ((((u >> 23) & 0xff) - 112) << 10) | ((u >> 13) & 0x3ff));
+                std::memcpy(&hc[i * 8 + 2 * k], &hb, 2);
+            }
+        } else {
+            std::memcpy(&hc[i * 16], rgb, 16);
+        }
+        hd[i] = ((x / 37 + y / 23) & 1) ? 0.4f : 0.7f;
+        float v2[2] = {(float)((x % 13) - 6) / (float)w, (float)((y % 11) - 5) / (float)h};
+        std::memcpy(&hv[i * 8], v2, 8);
+    }
+    void *dc = nullptr, *dh = nullptr, *dho = nullptr, *ds = nullptr, *dv = nullptr;
+    float *dd = nullptr, *dhd = nullptr;
+    bool alloc_ok = hipMalloc(&dc, cb) == hipSuccess && hipMalloc(&dh, cb) == hipSuccess && hipMalloc(&dho, cb) == hipSuccess &&
+                    hipMalloc(&ds, cb) == hipSuccess && hipMalloc(&dv, px * 8) == hipSuccess &&
+                    hipMalloc((void**)&dd, px * 4) == hipSuccess && hipMalloc((void**)&dhd, px * 4) == hipSuccess;
+    auto release = [&]() {
+        hipStreamSynchronize(st);
+        for (void* p : {dc, dh, dho, ds, dv, (void*)dd, (void*)dhd})
+            if (p) hipFree(p);
+    };
+    if (!alloc_ok) { release(); return fail(TAA_ERR_OUT_OF_MEMORY, "autotune buffers"); }
+    hipMemcpy(dc, hc.data(), cb, hipMemcpyHostToDevice);
+    hipMemcpy(dh, hc.data(), cb, hipMemcpyHostToDevice);
+    hipMemcpy(dv, hv.data(), px * 8, hipMemcpyHostToDevice);
+    hipMemcpy(dd, hd.data(), px * 4, hipMemcpyHostToDevice);
+    hipMemcpy(dhd, hd.data(), px * 4, hipMemcpyHostToDevice);
+    // the full feature set (variance clip, Catmull-Rom, closest-depth velocity, rejections, Lottes)
+    taa_uniforms_t U;
+    std::memset(&U, 0, sizeof(U));
+    taa_params_defaults(&U.param[0]);
+    taa_params_t& P = U.param[0];
+    P.mVarianceClipping = 1; P.mVarClipGamma = 1.0f; P.mColorClampingOrClipping = 2; P.mInterpolationMode = 2;
+    P.mAlpha = 0.1f; P.mUseVelocityVectors = 2; P.mVelocitySampleMode = 2; P.mUseYCoCg = 1;
+    P.mRejectOutside = 1; P.mDepthCulling = 1; P.mLumaWeightingLottes = 1;
+    U.param[1] = P;
+    U.mCamNearPlane = 0.1f; U.mCamFarPlane = 100.f; U.pad1 = 0.01f;
+    taa_cas_consts_t K;
+    taa_cas_setup(K.const0, K.const1, 0.5f, (float)w, (float)h, (float)w, (float)h);
+    taastar::FrameDesc fd;
+    taa_frame_images_t img;
+    std::memset(&img, 0, sizeof(img));
+    img.color = dc; img.depth = dd; img.history = dh; img.history_depth = dhd; img.result_screen = ds;
+    img.velocity = dv; img.result_history = dho; img.width = w; img.height = h; img.pitch = w;
+    img.color_format = fmt; img.velocity_format = TAA_VEL_RG32F;
+    int rc = make_desc(&img, cas ? &K : nullptr, fd);
+    if (rc) { release(); return rc; }
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    int best = -1;
+    float best_t = 1e30f;
+    for (int s = 0; s < taastar::kNumShapes; ++s) {
+        const taastar::TileShapeDef sd = shape_def(s, cas != 0);
+        fd.tiles_x = taastar::tiles_for(w, sd.tw);
+        fd.tiles_y = taastar::tiles_for(h, sd.th);
+        for (int i = 0; i < 3; ++i) taastar::launch_resolve(fd, U, s, arith, st);
+        hipEventRecord(e0, st);
+        for (int i = 0; i < iters; ++i) taastar::launch_resolve(fd, U, s, arith, st);
+        hipEventRecord(e1, st);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        const float us = 1000.f * ms / (float)iters;
+        if (us < best_t) { best_t = us; best = s; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipError_t le = hipGetLastError();
+    release();
+    if (le != hipSuccess) return hip_fail(le, "autotune launch");
+    taa_tile_shape_select(w, h, fmt, arith, cas, best);
+    if (best_index) *best_index = best;
+    if (best_us) *best_us = best_t;
+    return ok();
+}
+
+}  // extern "C"

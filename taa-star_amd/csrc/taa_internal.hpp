@@ -1,0 +1,64 @@
+// Internal types shared by the gfx950 kernels (taa_kernels.hip) and the host runtime
+// (taa_api.cpp). Not part of the C ABI.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "taastar/taa_abi.h"
+
+namespace taastar {
+
+// One frame as the kernels see it. Mirrors the descriptor set of taa.comp
+// (shaders/taa.comp:24-38) plus the tiling of this launch.
+struct FrameDesc {
+    const void* color;        // uCurrentFrame
+    const float* depth;       // uCurrentDepth
+    const void* history;      // uHistoryFrame
+    const float* hdepth;      // uHistoryDepth
+    void* screen;             // uResultScreen (sharpened when cas_on)
+    void* debug;              // uDebug (RGBA16F) or null
+    const void* velocity;     // uCurrentVelocity
+    void* history_out;        // uResultHistory
+    void* inspect;            // parity-test intermediates or null
+    int32_t W, H, pitch;
+    int32_t color_fmt, vel_fmt;
+    int32_t x_begin, x_end;   // columns owned by this launch (split screen), [x_begin, x_end)
+    int32_t tiles_x, tiles_y; // output tiles covering [x_begin, x_end) x [0, H)
+    int32_t param_idx;        // which Parameters block of the uniforms applies
+    int32_t cas_on;
+    uint32_t cas_peak_bits;   // const1[0] of CasSetup
+    int32_t pad0;
+};
+
+// One frame of a batched launch: frame + device copy of its uniforms + first flattened tile.
+struct BatchEntry {
+    FrameDesc fd;
+    const taa_uniforms_t* U;  // device pointer
+    int32_t tile_begin;       // prefix sum of tiles of the entries before this one
+    int32_t pad;
+};
+
+// Tile shapes the kernels are instantiated for (output tile, 256 threads per workgroup).
+struct TileShapeDef {
+    int tw, th, pad;
+};
+constexpr int kNumShapes = 4;
+// Shapes used without fused CAS (resolve region == output tile)
+constexpr TileShapeDef kShapesPlain[kNumShapes] = {{16, 16, 1}, {32, 8, 1}, {64, 4, 1}, {32, 16, 1}};
+// Shapes used with fused CAS (resolve region == output tile + 1-px apron)
+constexpr TileShapeDef kShapesCas[kNumShapes] = {{14, 14, 1}, {30, 14, 1}, {30, 30, 1}, {62, 14, 1}};
+constexpr int kDefaultShapePlain = 0;
+constexpr int kDefaultShapeCas = 2;
+
+inline int tiles_for(int extent, int t) { return (extent + t - 1) / t; }
+
+// Launchers (taa_kernels.hip). All return hipError_t as int.
+int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int arith, void* stream);
+int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uint32_t peak_bits, void* stream);
+int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas,
+                 bool s16, int arith, int grid, void* stream);
+// workgroups of one batched launch that keep every CU busy
+int batch_grid_size(int shape, bool cas, bool s16, int arith, int device);
+
+}  // namespace taastar

@@ -1,0 +1,758 @@
+// gfx950 kernels of the temporal resolve (shaders/taa.comp) and FidelityFX CAS
+// (shaders/sharpen_cas.comp, shaders/ffx_cas.h:424-537).
+//
+// Work decomposition (one 256-thread workgroup per output tile TW x TH):
+//   1. stage the current colour (tone-mapped, converted to YCoCg once per texel) and depth of the
+//      tile plus an apron into LDS: 1 px for the 3x3 neighbourhood, 2 px when CAS is fused;
+//   2. resolve every texel of the resolve region (the tile, plus a 1-px ring when CAS is fused)
+//      -- neighbourhood box from LDS, closest-depth tap from LDS, velocity and the 4x4
+//      Catmull-Rom history footprint gathered from global memory (L2 / MALL resident);
+//   3. with CAS fused, keep the resolved screen colour of the region in LDS and sharpen the tile
+//      from it, so the unsharpened image never goes to HBM.
+// Tiles are mapped to workgroups so that each XCD (blockIdx % 8) works on one contiguous band of
+// tiles, which keeps apron and history re-reads inside that XCD's L2.
+//
+// This file is compiled with -ffp-contract=off: the FP32 arithmetic path evaluates every
+// expression of taa.comp in source order with correctly rounded division and square root, so it
+// reproduces the scalar FP32 transcription bit for bit.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "taa_internal.hpp"
+
+namespace taastar {
+namespace {
+
+// ------------------------------------------------------------------------------------------
+// storage
+
+__device__ __forceinline__ float h2f(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
+__device__ __forceinline__ uint32_t f2h(float f) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f); }
+__device__ __forceinline__ float round_f16(float f) { return (float)(_Float16)f; }
+
+template <bool S16>
+__device__ __forceinline__ float4 ld_rgba(const void* p, size_t i) {
+    if constexpr (S16) {
+        uint2 v = static_cast<const uint2*>(p)[i];
+        return make_float4(h2f(v.x & 0xffffu), h2f(v.x >> 16), h2f(v.y & 0xffffu), h2f(v.y >> 16));
+    } else {
+        return static_cast<const float4*>(p)[i];
+    }
+}
+template <bool S16>
+__device__ __forceinline__ void st_rgba(void* p, size_t i, float4 v) {
+    if constexpr (S16) {
+        uint2 o;
+        o.x = f2h(v.x) | (f2h(v.y) << 16);
+        o.y = f2h(v.z) | (f2h(v.w) << 16);
+        static_cast<uint2*>(p)[i] = o;
+    } else {
+        static_cast<float4*>(p)[i] = v;
+    }
+}
+__device__ __forceinline__ void st_rgba16(void* p, size_t i, float4 v) { st_rgba<true>(p, i, v); }
+
+// Velocity texel; RG formats read back as (r, g, 0, 1).
+__device__ __forceinline__ float4 ld_vel(const void* p, int fmt, size_t i) {
+    switch (fmt) {
+        case TAA_VEL_RG32F: { float2 v = static_cast<const float2*>(p)[i]; return make_float4(v.x, v.y, 0.f, 1.f); }
+        case TAA_VEL_RG16F: { uint32_t v = static_cast<const uint32_t*>(p)[i]; return make_float4(h2f(v & 0xffffu), h2f(v >> 16), 0.f, 1.f); }
+        case TAA_VEL_RGBA32F: return static_cast<const float4*>(p)[i];
+        default: {
+            uint2 v = static_cast<const uint2*>(p)[i];
+            return make_float4(h2f(v.x & 0xffffu), h2f(v.x >> 16), h2f(v.y & 0xffffu), h2f(v.y >> 16));
+        }
+    }
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// int(x): NaN -> 0, saturating, truncation toward zero
+__device__ __forceinline__ int to_int(float v) {
+    if (v != v) return 0;
+    if (v >= 2147483647.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (-2147483647 - 1);
+    return (int)v;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP32 arithmetic in taa.comp order
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, f3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 vmin(f3 a, f3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+__device__ __forceinline__ f3 vmax(f3 a, f3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+__device__ __forceinline__ float gmix(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+__device__ __forceinline__ float gclamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+
+// taa.comp:158-174
+__device__ __forceinline__ f3 rgb_to_ycocg(f3 c) {
+    return {.25f * c.x + .5f * c.y + .25f * c.z, .5f * c.x - .5f * c.z, -.25f * c.x + .5f * c.y - .25f * c.z};
+}
+__device__ __forceinline__ f3 ycocg_to_rgb(f3 c) {
+    float tmp = c.x - c.z;
+    return {tmp + c.y, c.x + c.z, tmp - c.y};
+}
+// taa.comp:182-197
+__device__ __forceinline__ f3 tonemap_rgb(const taa_params_t& P, f3 hdr) {
+    if (P.mToneMapLumaKaris) {
+        float luma = fmaxf(fmaxf(hdr.x, hdr.y), hdr.z);
+        return hdr / (1.0f + luma);
+    }
+    return hdr;
+}
+__device__ __forceinline__ f3 un_tonemap_rgb(const taa_params_t& P, f3 ldr) {
+    if (P.mToneMapLumaKaris) {
+        float luma = fmaxf(fmaxf(ldr.x, ldr.y), ldr.z);
+        return ldr / (1.0f - luma);
+    }
+    return ldr;
+}
+__device__ __forceinline__ f3 stage_convert(const taa_params_t& P, f3 c) {
+    c = tonemap_rgb(P, c);
+    return P.mUseYCoCg ? rgb_to_ycocg(c) : c;
+}
+__device__ __forceinline__ float luminance(const taa_params_t& P, f3 c) { return P.mUseYCoCg ? c.x : rgb_to_ycocg(c).x; }
+__device__ __forceinline__ float linearize_depth(float d, float n, float f) { return n * f / (f + d * (n - f)); }
+
+// Bilinear (clamp to edge, exact weights) over an RGBA image -- the sampler of taa.hpp:274 at a
+// position that is not a texel centre.
+template <bool S16>
+__device__ float4 bilinear_rgba(const void* img, int W, int H, int pitch, float u, float v) {
+    float px = u * (float)W - 0.5f, py = v * (float)H - 0.5f;
+    float fx = floorf(px), fy = floorf(py);
+    float tx = px - fx, ty = py - fy;
+    int x0 = clampi(to_int(fx), -1, W), y0 = clampi(to_int(fy), -1, H);
+    int xa = clampi(x0, 0, W - 1), xb = clampi(x0 + 1, 0, W - 1);
+    int ya = clampi(y0, 0, H - 1), yb = clampi(y0 + 1, 0, H - 1);
+    float4 a = ld_rgba<S16>(img, (size_t)ya * pitch + xa), b = ld_rgba<S16>(img, (size_t)ya * pitch + xb);
+    float4 c = ld_rgba<S16>(img, (size_t)yb * pitch + xa), d = ld_rgba<S16>(img, (size_t)yb * pitch + xb);
+    float ux = 1.0f - tx, uy = 1.0f - ty;
+    float4 top = make_float4(a.x * ux + b.x * tx, a.y * ux + b.y * tx, a.z * ux + b.z * tx, a.w * ux + b.w * tx);
+    float4 bot = make_float4(c.x * ux + d.x * tx, c.y * ux + d.y * tx, c.z * ux + d.z * tx, c.w * ux + d.w * tx);
+    return make_float4(top.x * uy + bot.x * ty, top.y * uy + bot.y * ty, top.z * uy + bot.z * ty, top.w * uy + bot.w * ty);
+}
+
+// Catmull-Rom history fetch (taa.comp:441-514), 16-tap texelFetch form with the edge clamp
+// at size-1; identical to the 9-tap bilinear form up to rounding.
+template <bool S16>
+__device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
+    const int W = fd.W, H = fd.H;
+    float iTx = hu * (float)W, iTy = hv * (float)H;
+    float fbx = floorf(iTx - 0.5f), fby = floorf(iTy - 0.5f);
+    float tcx = fbx + 0.5f, tcy = fby + 0.5f;
+    float fx = iTx - tcx, fy = iTy - tcy;
+    float f2x = fx * fx, f2y = fy * fy, f3x = f2x * fx, f3y = f2y * fy;
+    float wx[4] = {-0.5f * f3x + f2x - 0.5f * fx, 1.5f * f3x - 2.5f * f2x + 1.0f,
+                   -1.5f * f3x + 2.0f * f2x + 0.5f * fx, 0.5f * f3x - 0.5f * f2x};
+    float wy[4] = {-0.5f * f3y + f2y - 0.5f * fy, 1.5f * f3y - 2.5f * f2y + 1.0f,
+                   -1.5f * f3y + 2.0f * f2y + 0.5f * fy, 0.5f * f3y - 0.5f * f2y};
+    int kx = clampi(to_int(fbx), -2, W + 1), ky = clampi(to_int(fby), -2, H + 1);
+    int xs[4], ys[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        xs[i] = clampi(kx - 1 + i, 0, W - 1);
+        ys[i] = clampi(ky - 1 + i, 0, H - 1) * fd.pitch;
+    }
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float4 t = ld_rgba<S16>(fd.history, (size_t)ys[j] + xs[i]);
+            float a = wx[i], b = wy[j];
+            float4 s = make_float4(t.x * a * b, t.y * a * b, t.z * a * b, t.w * a * b);
+            if (i == 0 && j == 0) acc = s;
+            else acc = make_float4(acc.x + s.x, acc.y + s.y, acc.z + s.z, acc.w + s.w);
+        }
+    return acc;
+}
+
+// B-spline history fetch (taa.comp:517-543): four bilinear taps.
+template <bool S16>
+__device__ float4 history_bspline(const FrameDesc& fd, float hu, float hv) {
+    const int W = fd.W, H = fd.H;
+    float sx = (float)W, sy = (float)H;
+    float ix = 1.0f / sx, iy = 1.0f / sy;
+    float iTx = hu * sx, iTy = hv * sy;
+    float tcx = floorf(iTx - 0.5f) + 0.5f, tcy = floorf(iTy - 0.5f) + 0.5f;
+    float fx = iTx - tcx, fy = iTy - tcy;
+    float f2x = fx * fx, f2y = fy * fy, f3x = f2x * fx, f3y = f2y * fy;
+    float w0x = f2x - 0.5f * (f3x + fx), w0y = f2y - 0.5f * (f3y + fy);
+    float w1x = 1.5f * f3x - 2.5f * f2x + 1.0f, w1y = 1.5f * f3y - 2.5f * f2y + 1.0f;
+    float w3x = 0.5f * (f3x - f2x), w3y = 0.5f * (f3y - f2y);
+    float w2x = 1.0f - w0x - w1x - w3x, w2y = 1.0f - w0y - w1y - w3y;
+    float s0x = w0x + w1x, s0y = w0y + w1y, s1x = w2x + w3x, s1y = w2y + w3y;
+    float f0x = w1x / (w0x + w1x), f0y = w1y / (w0y + w1y);
+    float f1x = w3x / (w2x + w3x), f1y = w3y / (w2y + w3y);
+    float t0x = (tcx - 1.0f + f0x) * ix, t0y = (tcy - 1.0f + f0y) * iy;
+    float t1x = (tcx + 1.0f + f1x) * ix, t1y = (tcy + 1.0f + f1y) * iy;
+    float4 a = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, t0x, t0y);
+    float4 b = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, t1x, t0y);
+    float4 c = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, t0x, t1y);
+    float4 d = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, t1x, t1y);
+    float4 r0 = make_float4(a.x * s0x + b.x * s1x, a.y * s0x + b.y * s1x, a.z * s0x + b.z * s1x, a.w * s0x + b.w * s1x);
+    float4 r1 = make_float4(c.x * s0x + d.x * s1x, c.y * s0x + d.y * s1x, c.z * s0x + d.z * s1x, c.w * s0x + d.w * s1x);
+    return make_float4(r0.x * s0y + r1.x * s1y, r0.y * s0y + r1.y * s1y, r0.z * s0y + r1.z * s1y, r0.w * s0y + r1.w * s1y);
+}
+
+// taa.comp:323-345 with p = (0,0,0,1)
+__device__ __forceinline__ f3 clipAabb(f3 mn, f3 mx, f3 q) {
+    const float eps = 1e-7f;
+    f3 pClip = 0.5f * (mx + mn);
+    f3 eClip = 0.5f * (mx - mn) + f3{eps, eps, eps};
+    f3 vClip = q - pClip;
+    f3 vUnit = vClip / eClip;
+    float ma = fmaxf(fabsf(vUnit.x), fmaxf(fabsf(vUnit.y), fabsf(vUnit.z)));
+    if (ma > 1.0f) return pClip + vClip / ma;
+    return q;
+}
+// taa.comp:348-369
+__device__ __forceinline__ f3 clipAabbSlow(f3 mn, f3 mx, f3 p, f3 q) {
+    f3 r = q - p, rmax = mx - p, rmin = mn - p;
+    const float eps = 1e-7f;
+    if (r.x > rmax.x + eps) r = r * (rmax.x / r.x);
+    if (r.y > rmax.y + eps) r = r * (rmax.y / r.y);
+    if (r.z > rmax.z + eps) r = r * (rmax.z / r.z);
+    if (r.x < rmin.x - eps) r = r * (rmin.x / r.x);
+    if (r.y < rmin.y - eps) r = r * (rmin.y / r.y);
+    if (r.z < rmin.z - eps) r = r * (rmin.z / r.z);
+    return p + r;
+}
+
+// ------------------------------------------------------------------------------------------
+// CAS (ffx_cas.h:424-537, sharpen only, green weight for all channels)
+
+__device__ __forceinline__ float prx_lo_sqrt(float a) { return __uint_as_float((__float_as_uint(a) >> 1u) + 0x1fbc4639u); }
+__device__ __forceinline__ float prx_lo_rcp(float a) { return __uint_as_float(0x7ef07ebbu - __float_as_uint(a)); }
+__device__ __forceinline__ float prx_med_rcp(float a) {
+    float b = __uint_as_float(0x7ef19fffu - __float_as_uint(a));
+    return b * (-b * a + 2.0f);
+}
+__device__ __forceinline__ float sat(float a) { return fminf(fmaxf(a, 0.0f), 1.0f); }
+__device__ __forceinline__ float min3(float x, float y, float z) { return fminf(x, fminf(y, z)); }
+__device__ __forceinline__ float max3(float x, float y, float z) { return fmaxf(x, fmaxf(y, z)); }
+
+__device__ __forceinline__ f3 cas_filter(f3 b, f3 d, f3 e, f3 f, f3 h, float peak) {
+    float mxG = max3(max3(d.y, e.y, f.y), b.y, h.y);
+    float mnG = min3(min3(d.y, e.y, f.y), b.y, h.y);
+    float rcpMG = prx_lo_rcp(mxG);
+    float ampG = sat(fminf(mnG, 1.0f - mxG) * rcpMG);
+    ampG = prx_lo_sqrt(ampG);
+    float wG = ampG * peak;
+    float rcpWeight = prx_med_rcp(1.0f + 4.0f * wG);
+    f3 o;
+    o.x = sat((b.x * wG + d.x * wG + f.x * wG + h.x * wG + e.x) * rcpWeight);
+    o.y = sat((b.y * wG + d.y * wG + f.y * wG + h.y * wG + e.y) * rcpWeight);
+    o.z = sat((b.z * wG + d.z * wG + f.z * wG + h.z * wG + e.z) * rcpWeight);
+    return o;
+}
+
+// ------------------------------------------------------------------------------------------
+// tile geometry and LDS
+
+template <int TW_, int TH_, int PAD_, bool CAS_>
+struct Geo {
+    static constexpr int TW = TW_, TH = TH_, PAD = PAD_;
+    static constexpr bool CAS = CAS_;
+    static constexpr int NT = 256;
+    static constexpr int A = CAS ? 2 : 1;  // input apron
+    static constexpr int SW = TW + 2 * A, SH = TH + 2 * A, SP = SW + PAD;
+    static constexpr int RA = CAS ? 1 : 0;  // resolve apron (CAS needs the resolved cross)
+    static constexpr int RW = TW + 2 * RA, RH = TH + 2 * RA, RP = RW + PAD;
+    struct Smem {
+        float c0[SH * SP], c1[SH * SP], c2[SH * SP], d[SH * SP];
+        float r0[CAS ? RH * RP : 1], r1[CAS ? RH * RP : 1], r2[CAS ? RH * RP : 1];
+    };
+};
+
+struct PixelResult {
+    f3 screen;       // un-tonemapped rgb (alpha 1)
+    float4 history;  // antiAliased, dynamic mask
+};
+
+// One output texel of taa.comp main (taa.comp:707-960), FP32 arithmetic. `l` indexes the texel
+// in the staged LDS planes (row pitch SP).
+template <bool S16, int SP>
+__device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
+                                         int x, int y, int l, const float* __restrict__ s0,
+                                         const float* __restrict__ s1, const float* __restrict__ s2,
+                                         const float* __restrict__ sd, bool write_aux) {
+    const int W = fd.W, H = fd.H;
+    const size_t gi = (size_t)y * fd.pitch + x;
+    PixelResult R;
+    const float uvx = ((float)x + 0.5f) / (float)W, uvy = ((float)y + 0.5f) / (float)H;  // taa.comp:131
+
+    if (P.mPassThrough || U.mBypassHistoryUpdate) {  // taa.comp:726-737
+        float4 h = ld_rgba<S16>(fd.history, gi);
+        f3 s = P.mPassThrough ? xyz(ld_rgba<S16>(fd.color, gi)) : xyz(h);
+        R.screen = s;
+        R.history = make_float4(h.x, h.y, h.z, 1.0f);
+        if (write_aux && fd.debug) st_rgba16(fd.debug, gi, make_float4(0.f, 0.f, 0.f, 0.f));
+        if (write_aux && fd.inspect) {
+            int32_t* w = static_cast<int32_t*>(fd.inspect) + gi * TAA_INSPECT_WORDS;
+#pragma unroll
+            for (int k = 0; k < TAA_INSPECT_WORDS; ++k) w[k] = (k == TAA_INSPECT_CLOSEST_TAP) ? -1 : 0;
+        }
+        return R;
+    }
+
+    // ---- neighbourhood (taa.comp:199-213, 259-319)
+    const float jux = U.mJitterNdc[0] * 0.5f * P.mUnjitterFactor, juy = U.mJitterNdc[1] * 0.5f * P.mUnjitterFactor;
+    f3 c[9];  // centre, then (-1,-1) (0,-1) (1,-1) (-1,0) (1,0) (-1,1) (0,1) (1,1)
+    const int ox[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1};
+    const int oy[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
+    if (!P.mUnjitterNeighbourhood) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            int li = l + oy[k] * SP + ox[k];
+            c[k] = {s0[li], s1[li], s2[li]};
+        }
+    } else {
+        const float ix = 1.0f / (float)W, iy = 1.0f / (float)H;
+        for (int k = 0; k < 9; ++k) {
+            float u = jux + ((float)(x + ox[k]) + 0.5f) * ix, v = juy + ((float)(y + oy[k]) + 0.5f) * iy;
+            c[k] = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, W, H, fd.pitch, u, v)));
+        }
+    }
+    f3 colMin, colMax, clipTowards;
+    if (P.mVarianceClipping) {
+        const float N = 9.0f;
+        f3 m1 = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
+        f3 m2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + c[3] * c[3] + c[4] * c[4] + c[5] * c[5] + c[6] * c[6] +
+                c[7] * c[7] + c[8] * c[8];
+        f3 mean = m1 / N;
+        f3 var = m2 / N - mean * mean;
+        f3 sigma = {sqrtf(fmaxf(0.0f, var.x)), sqrtf(fmaxf(0.0f, var.y)), sqrtf(fmaxf(0.0f, var.z))};
+        colMin = mean - P.mVarClipGamma * sigma;
+        colMax = mean + P.mVarClipGamma * sigma;
+        clipTowards = mean;
+    } else if (P.mShapedNeighbourhood) {
+        f3 mn33 = vmin(vmin(vmin(vmin(vmin(vmin(vmin(vmin(c[0], c[1]), c[2]), c[3]), c[4]), c[5]), c[6]), c[7]), c[8]);
+        f3 mx33 = vmax(vmax(vmax(vmax(vmax(vmax(vmax(vmax(c[0], c[1]), c[2]), c[3]), c[4]), c[5]), c[6]), c[7]), c[8]);
+        f3 mn5 = vmin(vmin(vmin(vmin(c[0], c[2]), c[4]), c[5]), c[7]);
+        f3 mx5 = vmax(vmax(vmax(vmax(c[0], c[2]), c[4]), c[5]), c[7]);
+        colMin = (mn33 + mn5) * 0.5f;
+        colMax = (mx33 + mx5) * 0.5f;
+        clipTowards = c[0];
+    } else {
+        colMin = vmin(vmin(vmin(vmin(vmin(vmin(vmin(vmin(c[0], c[1]), c[2]), c[3]), c[4]), c[5]), c[6]), c[7]), c[8]);
+        colMax = vmax(vmax(vmax(vmax(vmax(vmax(vmax(vmax(c[0], c[1]), c[2]), c[3]), c[4]), c[5]), c[6]), c[7]), c[8]);
+        clipTowards = c[0];
+    }
+    if (P.mUseYCoCg && P.mShrinkChromaAxis) {
+        const f3 s = {0.5f * 1.0f, 0.5f * 0.5f, 0.5f * 0.5f};
+        f3 halfSize = s * (colMax - colMin);
+        f3 center = (colMin + colMax) * 0.5f;
+        colMin = center - halfSize;
+        colMax = center + halfSize;
+        bool out = clipTowards.x < colMin.x || clipTowards.y < colMin.y || clipTowards.z < colMin.z ||
+                   clipTowards.x > colMax.x || clipTowards.y > colMax.y || clipTowards.z > colMax.z;
+        if (out) clipTowards = center;
+    }
+    // getCurrentColor (taa.comp:216-220)
+    f3 currentColor = c[0];
+    if (P.mUnjitterCurrentSample) {
+        float u = jux + ((float)x + 0.5f) * (1.0f / (float)W), v = juy + ((float)y + 0.5f) * (1.0f / (float)H);
+        currentColor = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, W, H, fd.pitch, u, v)));
+    }
+    const float depth = sd[l];  // taa.comp:756
+
+    // ---- getHistoryPosition (taa.comp:391-438)
+    int flags = 0, closest = -1;
+    float hu, hv, expectedHistoryDepth;
+    {
+        float4 vs = ld_vel(fd.velocity, fd.vel_fmt, gi);
+        bool canUseVelocity = !(P.mUseVelocityVectors == 0 || (P.mUseVelocityVectors == 1 && vs.w < 0.5f));
+        if (canUseVelocity) {
+            flags |= TAA_FLAG_USED_VELOCITY;
+            if (P.mVelocitySampleMode == 1) {
+                // taa.comp:398-412: the longest vector is tracked but the last tap is assigned
+                float4 last = ld_vel(fd.velocity, fd.vel_fmt, (size_t)min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1));
+                vs.x = last.x;
+                vs.y = last.y;
+            } else if (P.mVelocitySampleMode == 2) {
+                // findClosestUvAndZ_3x3 (taa.comp:371-389): scan order row-major, strict <
+                float dC = sd[l - SP - 1];
+                int best = 0;
+#pragma unroll
+                for (int k = 1; k < 9; ++k) {
+                    float d = sd[l + (k / 3 - 1) * SP + (k % 3 - 1)];
+                    if (d < dC) { dC = d; best = k; }
+                }
+                closest = best;
+                int vx = clampi(x + best % 3 - 1, 0, W - 1), vy = clampi(y + best / 3 - 1, 0, H - 1);
+                vs = ld_vel(fd.velocity, fd.vel_fmt, (size_t)vy * fd.pitch + vx);
+            }
+            hu = uvx - vs.x;
+            hv = uvy - vs.y;
+            expectedHistoryDepth = depth - vs.z;
+        } else {
+            const float* Mi = U.mInverseViewProjMatrix;
+            const float* Mh = U.mHistoryViewProjMatrix;
+            float cs[4] = {uvx * 2.0f - 1.0f, uvy * 2.0f - 1.0f, depth, 1.0f};
+            float w[4], h[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) w[r] = Mi[r] * cs[0] + Mi[4 + r] * cs[1] + Mi[8 + r] * cs[2] + Mi[12 + r] * cs[3];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) h[r] = Mh[r] * w[0] + Mh[4 + r] * w[1] + Mh[8 + r] * w[2] + Mh[12 + r] * w[3];
+            hu = (h[0] / h[3]) * 0.5f + 0.5f;
+            hv = (h[1] / h[3]) * 0.5f + 0.5f;
+            expectedHistoryDepth = h[2] / h[3];
+        }
+    }
+    const float vux = uvx - hu, vuy = uvy - hv;
+    const float pixelSpeed = sqrtf(vux * vux + vuy * vuy);
+
+    // ---- sample_history_rgba (taa.comp:545-549)
+    float4 historyRaw;
+    if (P.mInterpolationMode == 0) historyRaw = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, hu, hv);
+    else if (P.mInterpolationMode == 1) historyRaw = history_bspline<S16>(fd, hu, hv);
+    else historyRaw = history_catmullrom<S16>(fd, hu, hv);
+    f3 historyColor = xyz(historyRaw);
+    if (P.mUseYCoCg) historyColor = rgb_to_ycocg(historyColor);
+    float alpha = P.mAlpha, beta = 1.0f;
+    bool rejected = false;
+
+    // ---- rejection (taa.comp:787-823)
+    if (P.mRejectOutside) {
+        if (hu < 0.0f || hv < 0.0f || hu >= 1.0f || hv >= 1.0f) {
+            alpha = P.mRejectionAlpha;
+            rejected = true;
+            flags |= TAA_FLAG_OUTSIDE;
+        }
+    }
+    float writeDynamicMask = 0.0f;
+    if (P.mDynamicAntiGhosting) {
+        const float eps = 1e-5f;
+        auto mov = [&](int dx, int dy) {
+            float4 v = ld_vel(fd.velocity, fd.vel_fmt, (size_t)clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1));
+            return (fabsf(v.x) > eps || fabsf(v.y) > eps) && (fabsf(v.w) >= 0.5f);
+        };
+        bool movL = mov(-1, 0), movR = mov(1, 0), movT = mov(0, -1), movB = mov(0, 1), movC = mov(0, 0);
+        if (!(movL || movR || movT || movB || movC) && historyRaw.w > 0.0f) {
+            rejected = true;
+            flags |= TAA_FLAG_GHOST_REJECT;
+        }
+        writeDynamicMask = movC ? 1.0f : 0.0f;
+    }
+    int dtx = 0, dty = 0;
+    if (P.mDepthCulling) {
+        dtx = to_int(hu * (float)W);
+        dty = to_int(hv * (float)H);
+        float historyDepth = fd.hdepth[(size_t)clampi(dty, 0, H - 1) * fd.pitch + clampi(dtx, 0, W - 1)];
+        bool rej;
+        if (U.pad1 > 0.0f) {
+            float n = U.mCamNearPlane, f = U.mCamFarPlane;
+            rej = fabsf(linearize_depth(historyDepth, n, f) - linearize_depth(expectedHistoryDepth, n, f)) > U.pad1;
+        } else {
+            float depthEpsilon = 0.1f * (1.0f - historyDepth);
+            rej = fabsf(historyDepth - expectedHistoryDepth) > depthEpsilon;
+        }
+        if (rej) {
+            rejected = true;
+            flags |= TAA_FLAG_DEPTH_REJECT;
+        }
+    }
+
+    // ---- rectification (taa.comp:826-845)
+    const f3 origHistoryColor = historyColor;
+    if (P.mColorClampingOrClipping == 1) historyColor = vmin(vmax(historyColor, colMin), colMax);
+    else if (P.mColorClampingOrClipping == 2) historyColor = clipAabb(colMin, colMax, historyColor);
+    else if (P.mColorClampingOrClipping == 3) historyColor = clipAabbSlow(colMin, colMax, clipTowards, historyColor);
+    const f3 rdiff = historyColor - origHistoryColor;
+    const bool rectified = fabsf(rdiff.x) > 0.001f || fabsf(rdiff.y) > 0.001f || fabsf(rdiff.z) > 0.001f;
+
+    // ---- blending (taa.comp:850-900)
+    if (rejected) {
+        alpha = P.mRejectionAlpha;
+        beta = 1.0f;
+    } else {
+        if (P.mVelBasedAlpha)
+            alpha = fmaxf(alpha, gmix(alpha, P.mVelBasedAlphaMax, gclamp(pixelSpeed * P.mVelBasedAlphaFactor, 0.0f, 1.0f)));
+        if (P.mLumaWeightingLottes) {
+            float lc = luminance(P, currentColor), lh = luminance(P, historyColor);
+            float diff = fabsf(lc - lh) / fmaxf(fmaxf(lc, lh), 0.2f);
+            float w = 1.0f - diff;
+            float ww = w * w;
+            alpha = gmix(P.mMaxAlpha, P.mMinAlpha, ww);
+        }
+        if (P.mReduceBlendNearClamp) {
+            float mnl = luminance(P, colMin), mxl = luminance(P, colMax), hl = luminance(P, origHistoryColor);
+            float distToClamp = 2.0f * fabsf(fminf(hl - mnl, mxl - hl)) / (mxl - mnl);
+            if (mxl - mnl < 0.001f) distToClamp = 1.0f;
+            alpha *= gclamp(4.0f * distToClamp, 0.0f, 1.0f);
+        }
+    }
+    if (U.mResetHistory) { alpha = 1.0f; beta = 1.0f; }
+    const float ab = alpha * beta;
+    f3 aa = {gmix(historyColor.x, currentColor.x, ab), gmix(historyColor.y, currentColor.y, ab), gmix(historyColor.z, currentColor.z, ab)};
+    if (P.mUseYCoCg) aa = ycocg_to_rgb(aa);
+    if (P.mAddNoise) {  // taa.comp:551-556 (GLSL sin(): outside the parity contract)
+        float sx = uvx + U.mSinTime[0] + 0.6959174f, sy = uvy + U.mSinTime[0] + 0.6959174f;
+        float s = sinf(sx * 12.9898f + sy * 78.233f);
+        float n0 = s * 43758.5453f, n1 = s * 28001.8384f, n2 = s * 50849.4141f;
+        aa.x = aa.x + ((n0 - floorf(n0)) * 2.0f - 1.0f) * P.mNoiseFactor;
+        aa.y = aa.y + ((n1 - floorf(n1)) * 2.0f - 1.0f) * P.mNoiseFactor;
+        aa.z = aa.z + ((n2 - floorf(n2)) * 2.0f - 1.0f) * P.mNoiseFactor;
+    }
+    R.history = make_float4(aa.x, aa.y, aa.z, writeDynamicMask);
+    R.screen = un_tonemap_rgb(P, aa);
+
+    if (write_aux && fd.debug) {  // taa.comp:912-942
+        float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+        switch (P.mDebugMode) {
+            case 0: { f3 t = colMax - colMin; g = make_float4(t.x, t.y, t.z, 0.f); break; }
+            case 1: { f3 t = colMax - colMin; float p = t.x * t.y * t.z; g = make_float4(p, p, p, 0.f); break; }
+            case 2: g = make_float4(rejected ? 1.f : 0.f, sqrtf(rdiff.x * rdiff.x + rdiff.y * rdiff.y + rdiff.z * rdiff.z), 0.f, 0.f); break;
+            case 3: g = make_float4(alpha, alpha, alpha, 0.f); break;
+            case 4: g = ld_vel(fd.velocity, fd.vel_fmt, gi); break;
+            case 5: g = make_float4(pixelSpeed, 0.f, 0.f, 0.f); break;
+            case 6: g = make_float4(R.screen.x, R.screen.y, R.screen.z, 1.0f); break;
+            case 7: g = R.history; break;
+            case 8: g = make_float4(0.f, 0.f, 1.f, 0.f); break;
+            default: break;
+        }
+        const float ds = P.mDebugScale;
+        g = make_float4(g.x * ds * P.mDebugMask[0], g.y * ds * P.mDebugMask[1], g.z * ds * P.mDebugMask[2], g.w * ds * P.mDebugMask[3]);
+        if (P.mDebugCenter) g = make_float4(g.x * 0.5f + 0.5f, g.y * 0.5f + 0.5f, g.z * 0.5f + 0.5f, g.w * 0.5f + 0.5f);
+        st_rgba16(fd.debug, gi, g);
+    }
+    if (write_aux && fd.inspect) {
+        if (rejected) flags |= TAA_FLAG_REJECTED;
+        if (rectified) flags |= TAA_FLAG_RECTIFIED;
+        int32_t* w = static_cast<int32_t*>(fd.inspect) + gi * TAA_INSPECT_WORDS;
+        w[TAA_INSPECT_HIST_TC_X] = to_int(floorf(hu * (float)W - 0.5f));
+        w[TAA_INSPECT_HIST_TC_Y] = to_int(floorf(hv * (float)H - 0.5f));
+        w[TAA_INSPECT_CLOSEST_TAP] = closest;
+        w[TAA_INSPECT_FLAGS] = flags;
+        w[TAA_INSPECT_COLMIN + 0] = __float_as_int(colMin.x);
+        w[TAA_INSPECT_COLMIN + 1] = __float_as_int(colMin.y);
+        w[TAA_INSPECT_COLMIN + 2] = __float_as_int(colMin.z);
+        w[TAA_INSPECT_COLMAX + 0] = __float_as_int(colMax.x);
+        w[TAA_INSPECT_COLMAX + 1] = __float_as_int(colMax.y);
+        w[TAA_INSPECT_COLMAX + 2] = __float_as_int(colMax.z);
+        w[TAA_INSPECT_BLEND] = __float_as_int(ab);
+        w[TAA_INSPECT_DEPTH_TC_X] = dtx;
+        w[TAA_INSPECT_DEPTH_TC_Y] = dty;
+        w[TAA_INSPECT_HIST_UV_X] = __float_as_int(hu);
+        w[TAA_INSPECT_HIST_UV_Y] = __float_as_int(hv);
+        w[15] = 0;
+    }
+    return R;
+}
+
+// ------------------------------------------------------------------------------------------
+// one tile
+
+template <class G, bool S16>
+__device__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm) {
+    const taa_params_t& P = U.param[fd.param_idx];
+    const int W = fd.W, H = fd.H, pitch = fd.pitch;
+    const int ty = tile / fd.tiles_x, tx = tile - ty * fd.tiles_x;
+    const int x0 = fd.x_begin + tx * G::TW, y0 = ty * G::TH;
+    const int gx0 = x0 - G::A, gy0 = y0 - G::A;  // image coordinates of LDS texel (0,0)
+    const int tid = threadIdx.x;
+
+    // 1. stage colour (converted once per texel) and depth, clamped to the edge
+    for (int i = tid; i < G::SW * G::SH; i += G::NT) {
+        const int sy = i / G::SW, sx = i - sy * G::SW;
+        const int gx = clampi(gx0 + sx, 0, W - 1), gy = clampi(gy0 + sy, 0, H - 1);
+        const size_t gi = (size_t)gy * pitch + gx;
+        f3 t = stage_convert(P, xyz(ld_rgba<S16>(fd.color, gi)));
+        const int li = sy * G::SP + sx;
+        sm.c0[li] = t.x;
+        sm.c1[li] = t.y;
+        sm.c2[li] = t.z;
+        sm.d[li] = fd.depth[gi];
+    }
+    __syncthreads();
+
+    // 2. resolve the region
+    for (int i = tid; i < G::RW * G::RH; i += G::NT) {
+        const int ry = i / G::RW, rx = i - ry * G::RW;
+        const int px = x0 - G::RA + rx, py = y0 - G::RA + ry;
+        const int cx = clampi(px, 0, W - 1), cy = clampi(py, 0, H - 1);
+        const bool owned = rx >= G::RA && rx < G::RA + G::TW && ry >= G::RA && ry < G::RA + G::TH && px < fd.x_end && py < H;
+        const int l = (cy - gy0) * G::SP + (cx - gx0);
+        PixelResult r = resolve_pixel_f32<S16, G::SP>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, owned);
+        if (owned) {
+            const size_t gi = (size_t)cy * pitch + cx;
+            st_rgba<S16>(fd.history_out, gi, r.history);
+            if constexpr (!G::CAS) st_rgba<S16>(fd.screen, gi, make_float4(r.screen.x, r.screen.y, r.screen.z, 1.0f));
+        }
+        if constexpr (G::CAS) {
+            // the CAS pass reads the stored screen image: round to the storage format first
+            f3 s = r.screen;
+            if constexpr (S16) s = {round_f16(s.x), round_f16(s.y), round_f16(s.z)};
+            const int li = ry * G::RP + rx;
+            sm.r0[li] = s.x;
+            sm.r1[li] = s.y;
+            sm.r2[li] = s.z;
+        }
+    }
+
+    // 3. fused CAS from the resolved region
+    if constexpr (G::CAS) {
+        __syncthreads();
+        const float peak = __uint_as_float(fd.cas_peak_bits);
+        for (int i = tid; i < G::TW * G::TH; i += G::NT) {
+            const int oy = i / G::TW, ox = i - oy * G::TW;
+            const int px = x0 + ox, py = y0 + oy;
+            if (px >= fd.x_end || py >= H) continue;
+            const int li = (oy + 1) * G::RP + (ox + 1);
+            auto at = [&](int k) { return f3{sm.r0[k], sm.r1[k], sm.r2[k]}; };
+            f3 o = cas_filter(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak);
+            st_rgba<S16>(fd.screen, (size_t)py * pitch + px, make_float4(o.x, o.y, o.z, 1.0f));
+        }
+    }
+}
+
+// Workgroup -> tile so that XCD k (blockIdx % 8) gets the contiguous k-th eighth of the tiles.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n >> 3, r = n & 7;
+    const int xcd = b & 7, slot = b >> 3;
+    return xcd * q + min(xcd, r) + slot;
+}
+
+struct ResolveArgs {
+    FrameDesc fd;
+    taa_uniforms_t U;
+};
+
+template <int TW, int TH, int PAD, bool CAS, bool S16>
+__global__ __launch_bounds__(256) void k_resolve(const ResolveArgs a) {
+    using G = Geo<TW, TH, PAD, CAS>;
+    __shared__ typename G::Smem sm;
+    const int n = a.fd.tiles_x * a.fd.tiles_y;
+    resolve_tile<G, S16>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
+}
+
+// Persistent batched resolve: the grid is sized to the machine and walks the flattened tile
+// space of every frame in the batch. XCD k walks the k-th eighth of the tiles.
+template <int TW, int TH, int PAD, bool CAS, bool S16>
+__global__ __launch_bounds__(256) void k_resolve_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
+    using G = Geo<TW, TH, PAD, CAS>;
+    __shared__ typename G::Smem sm;
+    const int per = (total_tiles + 7) >> 3;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
+    for (int j = slot; j < per; j += slots) {
+        const int t = xcd * per + j;
+        if (t >= total_tiles) break;
+        int lo = 0, hi = n_entries - 1;  // last entry with tile_begin <= t
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (entries[mid].tile_begin <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchEntry& e = entries[lo];
+        resolve_tile<G, S16>(e.fd, *e.U, t - e.tile_begin, sm);
+        __syncthreads();  // LDS is reused by the next tile
+    }
+}
+
+// sharpen_cas.comp on its own (one thread per pixel, clamped loads)
+template <bool S16>
+__global__ __launch_bounds__(256) void k_cas(const void* __restrict__ src, void* __restrict__ dst, int W, int H, int pitch, uint32_t peak_bits) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
+    auto ld = [&](int xx, int yy) {
+        return xyz(ld_rgba<S16>(src, (size_t)clampi(yy, 0, H - 1) * pitch + clampi(xx, 0, W - 1)));
+    };
+    f3 o = cas_filter(ld(x, y - 1), ld(x - 1, y), ld(x, y), ld(x + 1, y), ld(x, y + 1), __uint_as_float(peak_bits));
+    st_rgba<S16>(dst, (size_t)y * pitch + x, make_float4(o.x, o.y, o.z, 1.0f));
+}
+
+// ------------------------------------------------------------------------------------------
+// dispatch tables
+
+template <bool CAS, bool S16>
+hipError_t launch_resolve_t(const FrameDesc& fd, const taa_uniforms_t& U, int shape, hipStream_t st) {
+    ResolveArgs a;
+    a.fd = fd;
+    a.U = U;
+    const int n = fd.tiles_x * fd.tiles_y;
+    constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
+    switch (shape) {
+        case 0: hipLaunchKernelGGL((k_resolve<S[0].tw, S[0].th, S[0].pad, CAS, S16>), dim3(n), dim3(256), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_resolve<S[1].tw, S[1].th, S[1].pad, CAS, S16>), dim3(n), dim3(256), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_resolve<S[2].tw, S[2].th, S[2].pad, CAS, S16>), dim3(n), dim3(256), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_resolve<S[3].tw, S[3].th, S[3].pad, CAS, S16>), dim3(n), dim3(256), 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+template <bool CAS, bool S16>
+hipError_t launch_batch_t(const BatchEntry* e, int n, int total, int shape, int grid, hipStream_t st) {
+    constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
+    switch (shape) {
+        case 0: hipLaunchKernelGGL((k_resolve_batch<S[0].tw, S[0].th, S[0].pad, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 1: hipLaunchKernelGGL((k_resolve_batch<S[1].tw, S[1].th, S[1].pad, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 2: hipLaunchKernelGGL((k_resolve_batch<S[2].tw, S[2].th, S[2].pad, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        default: hipLaunchKernelGGL((k_resolve_batch<S[3].tw, S[3].th, S[3].pad, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+    }
+    return hipGetLastError();
+}
+
+template <bool CAS, bool S16>
+const void* batch_kernel_ptr(int shape) {
+    constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
+    switch (shape) {
+        case 0: return (const void*)&k_resolve_batch<S[0].tw, S[0].th, S[0].pad, CAS, S16>;
+        case 1: return (const void*)&k_resolve_batch<S[1].tw, S[1].th, S[1].pad, CAS, S16>;
+        case 2: return (const void*)&k_resolve_batch<S[2].tw, S[2].th, S[2].pad, CAS, S16>;
+        default: return (const void*)&k_resolve_batch<S[3].tw, S[3].th, S[3].pad, CAS, S16>;
+    }
+}
+
+}  // namespace
+
+int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int arith, void* stream) {
+    (void)arith;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
+    hipError_t e;
+    if (fd.cas_on) e = s16 ? launch_resolve_t<true, true>(fd, U, shape, st) : launch_resolve_t<true, false>(fd, U, shape, st);
+    else e = s16 ? launch_resolve_t<false, true>(fd, U, shape, st) : launch_resolve_t<false, false>(fd, U, shape, st);
+    return (int)e;
+}
+
+int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uint32_t peak_bits, void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    dim3 grid((W + 15) / 16, (H + 15) / 16);
+    if (fmt == TAA_FMT_RGBA16F) hipLaunchKernelGGL(k_cas<true>, grid, dim3(256), 0, st, src, dst, W, H, pitch, peak_bits);
+    else hipLaunchKernelGGL(k_cas<false>, grid, dim3(256), 0, st, src, dst, W, H, pitch, peak_bits);
+    return (int)hipGetLastError();
+}
+
+int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, int arith,
+                 int grid, void* stream) {
+    (void)arith;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    if (cas) e = s16 ? launch_batch_t<true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
+                     : launch_batch_t<true, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+    else e = s16 ? launch_batch_t<false, true>(d_entries, n_entries, total_tiles, shape, grid, st)
+                 : launch_batch_t<false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+    return (int)e;
+}
+
+int batch_grid_size(int shape, bool cas, bool s16, int arith, int device) {
+    (void)arith;
+    const void* k = cas ? (s16 ? batch_kernel_ptr<true, true>(shape) : batch_kernel_ptr<true, false>(shape))
+                        : (s16 ? batch_kernel_ptr<false, true>(shape) : batch_kernel_ptr<false, false>(shape));
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
+    int g = per_cu * cus;
+    return (g + 7) & ~7;  // whole XCD slots
+}
+
+}  // namespace taastar

@@ -1,0 +1,138 @@
+"""The resolve workloads: parameter sets, scene settings and storage encodings.
+
+c1  640x360,   16 frames, FP32 storage, variance clip + Catmull-Rom, simple velocity
+c2  1920x1080, 64 frames, FP32 storage, + closest-depth velocity, linear-depth rejection,
+    Lottes luma weighting, reject-outside, fused CAS 0.5
+c3  3840x2160, 64 frames, FP16 storage (RGBA16F / RG16F / D32F), c2 feature set, sigma 2.0
+c4  48 streams of mixed resolution in one persistent launch (FP32 and FP16 storage)
+c5  served load: 256 sessions, Poisson arrivals, FP16 storage, HBM-budget admission
+"""
+import numpy as np
+
+from . import (FMT_RGBA16F, FMT_RGBA32F, VEL_RG16F, VEL_RG32F, VEL_RGBA16F, VEL_RGBA32F, Scene, Uniforms,
+               cas_setup, default_params)
+
+NEAR, FAR = 0.1, 100.0          # camera planes for the linear-depth test (main.cpp:3780 uses near 0.1)
+LINEAR_DEPTH_THRESHOLD = 0.01   # c2/c3: reject when |linear(hd) - linear(expected)| > 0.01
+FP16_COLOR_MAX = 32768.0        # scene colour clamp for FP16 storage (keeps HDR fireflies finite)
+
+RESOLUTIONS = {"720p": (1280, 720), "1080p": (1920, 1080), "1440p": (2560, 1440), "4k": (3840, 2160)}
+SERVE_MIX = (("720p", 0.40), ("1080p", 0.35), ("1440p", 0.15), ("4k", 0.10))
+
+
+def params_c1():
+    p = default_params()
+    p.mVarianceClipping = 1
+    p.mVarClipGamma = 1.0
+    p.mColorClampingOrClipping = 2
+    p.mInterpolationMode = 2
+    p.mAlpha = 0.1
+    p.mUseVelocityVectors = 2
+    p.mVelocitySampleMode = 0
+    p.mUseYCoCg = 1
+    return p
+
+
+def params_c2():
+    p = params_c1()
+    p.mVelocitySampleMode = 2
+    p.mDepthCulling = 1
+    p.mLumaWeightingLottes = 1
+    p.mRejectOutside = 1
+    return p
+
+
+WORKLOADS = {
+    "c1": dict(width=640, height=360, frames=16, polygons=200, seed=0x7AA50001, lum_sigma=1.5,
+               fmt=FMT_RGBA32F, vel=VEL_RG32F, params=params_c1, cas=None, linear_depth=0.0),
+    "c2": dict(width=1920, height=1080, frames=64, polygons=2000, seed=0x7AA50002, lum_sigma=1.5,
+               fmt=FMT_RGBA32F, vel=VEL_RG32F, params=params_c2, cas=0.5, linear_depth=LINEAR_DEPTH_THRESHOLD),
+    "c3": dict(width=3840, height=2160, frames=64, polygons=8000, seed=0x7AA50003, lum_sigma=2.0,
+               fmt=FMT_RGBA16F, vel=VEL_RG16F, params=params_c2, cas=0.5, linear_depth=LINEAR_DEPTH_THRESHOLD),
+}
+
+
+def workload(name, **over):
+    w = dict(WORKLOADS[name])
+    w.update(over)
+    return w
+
+
+def scene_for(w, seed=None, width=None, height=None, polygons=None, pan_mode=0, pan=8.0, jitter_phase=0):
+    width = width or w["width"]
+    height = height or w["height"]
+    polygons = w["polygons"] if polygons is None else polygons
+    # keep the polygon density of the workload when a test shrinks the frame
+    if width != w["width"] or height != w["height"]:
+        polygons = max(1, int(round(polygons * width * height / (w["width"] * w["height"]))))
+    return Scene(width=width, height=height, num_polygons=polygons, seed=w["seed"] if seed is None else seed,
+                 lum_sigma=w["lum_sigma"], pan_mode=pan_mode, pan=pan, jitter_phase=jitter_phase,
+                 color_max=FP16_COLOR_MAX if w["fmt"] == FMT_RGBA16F else 0.0)
+
+
+def uniforms_for(w, frame, jitter_ndc, reset=None, params=None):
+    U = Uniforms()
+    p = params if params is not None else w["params"]()
+    U.param[0] = p
+    U.param[1] = p
+    U.mJitterNdc[0], U.mJitterNdc[1] = jitter_ndc
+    U.mCamNearPlane = NEAR
+    U.mCamFarPlane = FAR
+    U.pad1 = w.get("linear_depth", 0.0)
+    U.mResetHistory = int(frame == 0 if reset is None else reset)
+    for i in (0, 5, 10, 15):  # identity matrices (only read by matrix reprojection)
+        U.mHistoryViewProjMatrix[i] = 1.0
+        U.mInverseViewProjMatrix[i] = 1.0
+    return U
+
+
+def cas_for(w, width=None, height=None):
+    if w.get("cas") is None:
+        return None
+    return cas_setup(w["cas"], width or w["width"], height or w["height"])
+
+
+# ---- storage encodings --------------------------------------------------------------------
+
+def encode_color(rgba, fmt):
+    """float32 [H,W,4] -> storage array ([H,W,4] float32 or float16)."""
+    return np.ascontiguousarray(rgba.astype(np.float16 if fmt == FMT_RGBA16F else np.float32))
+
+
+def decode(arr):
+    return np.ascontiguousarray(arr.astype(np.float32))
+
+
+def encode_velocity(vel4, vfmt):
+    if vfmt == VEL_RG32F:
+        return np.ascontiguousarray(vel4[..., :2].astype(np.float32))
+    if vfmt == VEL_RG16F:
+        return np.ascontiguousarray(vel4[..., :2].astype(np.float16))
+    if vfmt == VEL_RGBA32F:
+        return np.ascontiguousarray(vel4.astype(np.float32))
+    return np.ascontiguousarray(vel4.astype(np.float16))
+
+
+def velocity_as_read(vel_stored):
+    """What the resolve reads from a velocity image: RG formats come back as (r, g, 0, 1)."""
+    v = vel_stored.astype(np.float32)
+    if v.shape[-1] == 2:
+        out = np.zeros(v.shape[:-1] + (4,), np.float32)
+        out[..., :2] = v
+        out[..., 3] = 1.0
+        return out
+    return np.ascontiguousarray(v)
+
+
+def bytes_per_pixel(w, cas=True):
+    """Algorithmic HBM bytes per output pixel: each input read once, each output written once."""
+    cb = 8 if w["fmt"] == FMT_RGBA16F else 16
+    vb = {VEL_RG32F: 8, VEL_RG16F: 4, VEL_RGBA32F: 16, VEL_RGBA16F: 8}[w["vel"]]
+    p = w["params"]()
+    reads = cb + vb + cb  # colour, velocity, history
+    if p.mVelocitySampleMode == 2 or p.mDepthCulling:
+        reads += 4        # current depth
+    if p.mDepthCulling:
+        reads += 4        # previous depth
+    writes = cb + cb      # history, screen (sharpened when CAS is fused)
+    return reads + writes

@@ -1,0 +1,101 @@
+"""ctypes wrapper of the oracle (oracle/liboracle.so): the checker the tests compare against."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libffx_ref.so")
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle/liboracle.so is missing: run `make -C oracle`")
+        L = C.CDLL(LIB)
+        fp, dp, ip = C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.oracle_resolve_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp, fp, fp, ip, C.c_int, C.c_int]
+        L.oracle_resolve_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, fp, fp, fp, dp, fp, dp, dp, dp, ip, C.c_int, C.c_int]
+        L.oracle_cas_f32.argtypes = [fp, fp, C.c_int, C.c_int, C.c_uint32, C.c_int]
+        L.oracle_cas_f64.argtypes = [dp, dp, C.c_int, C.c_int, C.c_uint32, C.c_int]
+        L.oracle_cas_setup.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)] + [C.c_float] * 5
+        L.oracle_halton.restype = C.c_float
+        L.oracle_halton.argtypes = [C.c_int, C.c_int]
+        L.oracle_half_bits_trunc.restype = C.c_uint32
+        L.oracle_half_bits_trunc.argtypes = [C.c_float]
+        L.oracle_jitter.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, fp,
+                                    C.c_uint32, C.c_uint32, fp, C.POINTER(C.c_uint32)]
+        _L = L
+    return _L
+
+
+def ref_lib():
+    """oracle/_ref/libffx_ref.so (the reference's own ffx_a.h/ffx_cas.h built for the CPU) or None."""
+    if not os.path.exists(REF):
+        return None
+    L = C.CDLL(REF)
+    L.ref_cas_setup.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)] + [C.c_float] * 5
+    L.ref_half_bits.restype = C.c_uint32
+    L.ref_half_bits.argtypes = [C.c_float]
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def resolve(U, color4, depth, vel4, hist4, hdepth, dtype=np.float32, cr16=True, threads=0, debug=True, inspect=True):
+    """One frame of the oracle. Inputs are float32 arrays as the resolve reads them (vel4 already
+    expanded to 4 channels); hist4 in `dtype`. Returns dict(screen, history, debug, inspect)."""
+    H, W = depth.shape
+    threads = threads or os.cpu_count() or 1
+    f = np.float32
+    color4 = np.ascontiguousarray(color4, f)
+    depth = np.ascontiguousarray(depth, f)
+    vel4 = np.ascontiguousarray(vel4, f)
+    hdepth = np.ascontiguousarray(hdepth, f)
+    hist4 = np.ascontiguousarray(hist4, dtype)
+    scr = np.empty((H, W, 4), dtype)
+    his = np.empty((H, W, 4), dtype)
+    dbg = np.empty((H, W, 4), dtype) if debug else None
+    ins = np.empty((H, W, 16), np.int32) if inspect else None
+    T = C.c_float if dtype == np.float32 else C.c_double
+    fn = lib().oracle_resolve_f32 if dtype == np.float32 else lib().oracle_resolve_f64
+    rc = fn(C.byref(U), W, H, _p(color4, C.c_float), _p(depth, C.c_float), _p(vel4, C.c_float), _p(hist4, T),
+            _p(hdepth, C.c_float), _p(scr, T), _p(his, T), _p(dbg, T) if debug else None,
+            _p(ins, C.c_int32) if inspect else None, int(cr16), threads)
+    assert rc == 0
+    return dict(screen=scr, history=his, debug=dbg, inspect=ins)
+
+
+def cas(src4, const1_x, dtype=np.float32, threads=0):
+    H, W = src4.shape[:2]
+    threads = threads or os.cpu_count() or 1
+    src4 = np.ascontiguousarray(src4, dtype)
+    dst = np.empty_like(src4)
+    T = C.c_float if dtype == np.float32 else C.c_double
+    fn = lib().oracle_cas_f32 if dtype == np.float32 else lib().oracle_cas_f64
+    assert fn(_p(src4, T), _p(dst, T), W, H, const1_x, threads) == 0
+    return dst
+
+
+def cas_setup(s, w, h):
+    c0, c1 = (C.c_uint32 * 4)(), (C.c_uint32 * 4)()
+    lib().oracle_cas_setup(c0, c1, s, w, h, w, h)
+    return list(c0), list(c1)
+
+
+def jitter(dist, frame, W, H, slow=0, fixed=-1, rot=0.0, scale=1.0, user=None):
+    out = (C.c_float * 2)()
+    n = C.c_uint32()
+    u = None
+    if user is not None:
+        u = np.ascontiguousarray(np.asarray(user, np.float32).reshape(-1))
+    rc = lib().oracle_jitter(dist, frame, slow, fixed, rot, scale, 0 if u is None else u.size // 2,
+                             None if u is None else _p(u, C.c_float), W, H, out, C.byref(n))
+    assert rc == 0
+    return (out[0], out[1]), n.value

@@ -1,0 +1,160 @@
+"""GPU FP32 path vs the FP32 transcription of taa.comp + CasFilter, through the C ABI.
+
+Bar (FP32 arithmetic): reprojected texel indices, closest-depth taps, rejection flags and the
+neighbourhood box bit-exact; history and screen within 1e-5 relative per channel
+(|d| <= 1e-5 * max(|ref|, 1e-3)).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from gpu_helpers import Sequence, assert_parity, rel_err  # noqa: E402
+from taastar import FMT_RGBA16F, FMT_RGBA32F, VEL_RG16F, VEL_RG32F, VEL_RGBA16F, VEL_RGBA32F  # noqa: E402
+from taastar.workloads import params_c1, params_c2, workload  # noqa: E402
+
+
+@pytest.fixture(autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def test_c1_full_sequence_bit_exact(oracle):
+    """640x360, 16 frames, FP32 storage: the first workload at its real size, every frame."""
+    print("case c1 640x360 x16 fp32", flush=True)
+    seq = Sequence(workload("c1"), cas=False)
+    for f in range(16):
+        out = seq.step(oracle)
+        assert_parity(out)
+        assert np.array_equal(out["gpu_history"], out["oracle_history"]), f"frame {f}: history not bit-identical"
+        assert np.array_equal(out["gpu_screen"], out["oracle_screen"]), f"frame {f}: screen not bit-identical"
+
+
+@pytest.mark.parametrize("fmt,vfmt", [(FMT_RGBA32F, VEL_RG32F), (FMT_RGBA16F, VEL_RG16F)])
+def test_c2_features_fused_cas(oracle, fmt, vfmt):
+    """c2 feature set (closest-depth velocity, linear depth rejection, Lottes, reject outside,
+    fused CAS 0.5) at 480x272, 8 frames, FP32 and FP16 storage."""
+    print(f"case c2 480x272 x8 fmt={fmt}", flush=True)
+    seq = Sequence(workload("c2", fmt=fmt, vel=vfmt), width=480, height=272, cas=True)
+    for f in range(8):
+        out = seq.step(oracle)
+        assert_parity(out)
+
+
+FEATURES = {
+    "minmax_rgb": dict(mVarianceClipping=0, mUseYCoCg=0, mColorClampingOrClipping=1),
+    "shaped_shrink": dict(mVarianceClipping=0, mShapedNeighbourhood=1, mShrinkChromaAxis=1),
+    "slow_clip": dict(mColorClampingOrClipping=3),
+    "bilinear_hist": dict(mInterpolationMode=0),
+    "bspline_hist": dict(mInterpolationMode=1),
+    "longest_vel": dict(mVelocitySampleMode=1),
+    "movers_only": dict(mUseVelocityVectors=1),
+    "matrix_reproj": dict(mUseVelocityVectors=0),
+    "raw_depth_cull": dict(mDepthCulling=1),
+    "reduce_near_clamp": dict(mReduceBlendNearClamp=1),
+    "vel_alpha": dict(mVelBasedAlpha=1),
+    "karis": dict(mToneMapLumaKaris=1, mVarClipGamma=1.25),
+    "antighost": dict(mDynamicAntiGhosting=1),
+    "unjitter": dict(mUnjitterNeighbourhood=1, mUnjitterCurrentSample=1),
+    "passthrough": dict(mPassThrough=1),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FEATURES))
+@pytest.mark.parametrize("vfmt", [VEL_RG32F, VEL_RGBA32F])
+def test_feature_matrix(oracle, name, vfmt):
+    """Every option of Parameters the kernel implements, on 160x96 frames, 4 frames each."""
+    print(f"case feature {name} vfmt={vfmt}", flush=True)
+    p = params_c2()
+    for k, v in FEATURES[name].items():
+        setattr(p, k, v)
+    w = workload("c2", vel=vfmt, params=lambda p=p: p)
+    seq = Sequence(w, width=160, height=96, cas=True)
+    for f in range(4):
+        out = seq.step(oracle)
+        assert_parity(out)
+
+
+def test_debug_modes_and_bypass(oracle):
+    """uDebug output (taa.comp:912-942) for modes 0..8, and mBypassHistoryUpdate."""
+    from taastar.workloads import uniforms_for
+    w = workload("c2")
+    seq = Sequence(w, width=96, height=64, cas=False)
+    seq.step(oracle)
+    for mode in range(9):
+        p = params_c2()
+        p.mDebugMode = mode
+        p.mDebugScale = 2.0
+        p.mDebugCenter = mode % 2
+        _, _, _, jit = seq.scene.render(seq.frame)
+        U = uniforms_for(w, seq.frame, jit, params=p)
+        U.mBypassHistoryUpdate = int(mode == 8)
+        out = seq.step(oracle, uniforms=U)
+        assert_parity(out)
+        g, o = out["gpu_debug"], out["oracle_debug"]
+        assert np.array_equal(np.isfinite(g), np.isfinite(o))
+        fin = np.isfinite(o)
+        assert rel_err(g[fin], o[fin], floor=1e-2).max() <= 2e-3, f"debug mode {mode}"
+
+
+def test_split_screen(oracle):
+    """splitScreen: columns x > splitX use param[1] (taa.comp:717), two launches."""
+    from taastar.workloads import uniforms_for
+    w = workload("c2")
+    seq = Sequence(w, width=200, height=80, cas=False)
+    for f in range(3):
+        col, dep, vel4, jit = seq.scene.render(seq.frame)
+        U = uniforms_for(w, seq.frame, jit)
+        U.splitScreen = 1
+        U.splitX = 77
+        U.param[1].mVarianceClipping = 0
+        U.param[1].mInterpolationMode = 0
+        U.param[1].mUseYCoCg = 0
+        out = seq.step(oracle, uniforms=U)
+        assert_parity(out)
+
+
+def test_fused_cas_equals_separate_pass(oracle):
+    """The fused CAS writes exactly what resolve + sharpen_cas.comp would."""
+    from taastar.device import FrameBuffers, cas_apply
+    from taastar.workloads import cas_for, encode_color, encode_velocity, scene_for, uniforms_for
+    for fmt, vfmt in [(FMT_RGBA32F, VEL_RG32F), (FMT_RGBA16F, VEL_RG16F)]:
+        w = workload("c2", fmt=fmt, vel=vfmt)
+        W, H = 333, 201  # ragged: partial tiles on both axes
+        sc = scene_for(w, width=W, height=H)
+        a = FrameBuffers(W, H, fmt, vfmt)
+        b = FrameBuffers(W, H, fmt, vfmt)
+        tmp = torch.zeros_like(b.screen)
+        K = cas_for(w, W, H)
+        for f in range(4):
+            col, dep, vel4, jit = sc.render(f)
+            for bb in (a, b):
+                bb.upload(encode_color(col, fmt), dep, encode_velocity(vel4, vfmt))
+            U = uniforms_for(w, f, jit)
+            a.resolve(U, K)
+            b.resolve(U, None, screen=tmp)
+            cas_apply(tmp, b.screen, fmt, K)
+            torch.cuda.synchronize()
+            assert torch.equal(a.screen, b.screen), f"fmt {fmt} frame {f}"
+            assert torch.equal(a.hist_in, b.hist_in)
+
+
+def test_deterministic_rerun(oracle):
+    seqs = [Sequence(workload("c2"), width=257, height=130, cas=True) for _ in range(2)]
+    for f in range(3):
+        o = [s.step(oracle, run_oracle=False) for s in seqs]
+        assert np.array_equal(o[0]["gpu_screen"], o[1]["gpu_screen"])
+        assert np.array_equal(o[0]["gpu_history"], o[1]["gpu_history"])
+
+
+@pytest.mark.parametrize("name", ["c2", "c3"])
+def test_full_size_against_oracle(oracle, name):
+    """The real 1080p / 4K frames: 3 frames through the oracle, index/box bit-exact."""
+    print(f"case full-size {name}", flush=True)
+    seq = Sequence(workload(name), cas=True)
+    for f in range(3):
+        out = seq.step(oracle)
+        assert_parity(out)
