@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="after the timed loop, copy MIB MiB device-to-device (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -210,6 +212,14 @@ def main():
             "bytes_per_pixel": bpp,
         },
     }
+    if a.copy_calib > 0:
+        n = a.copy_calib * (1 << 20)
+        src = torch.empty(n, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["config"]["copy_calib_bytes"] = n
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, host[:2], a.cpu_seconds)
     print(json.dumps(out), flush=True)

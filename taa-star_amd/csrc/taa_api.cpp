@@ -143,7 +143,9 @@ int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, in
     fd.param_idx = param_idx;
     fd.tiles_x = taastar::tiles_for(x_end - x_begin, s.tw);
     fd.tiles_y = taastar::tiles_for(fd.H, s.th);
-    int e = taastar::launch_resolve(fd, U, shape, arith, stream);
+    int e = (arith == TAA_ARITH_FP16 && taastar::fast_path_supported(fd, U, param_idx))
+                ? taastar::launch_resolve_fast(fd, U, shape, stream)
+                : taastar::launch_resolve(fd, U, shape, arith, stream);
     if (e != 0) return hip_fail((hipError_t)e, "resolve launch");
     return TAA_OK;
 }
@@ -176,7 +178,7 @@ struct taa_ctx {
     std::atomic<uint64_t> used{0};
     std::mutex mu;
     // batch staging ring: pinned host mirror + device copy of entries and uniforms
-    static constexpr int kSlots = 4;
+    static constexpr int kSlots = 16;
     static constexpr int kCap = 1024;
     struct Slot {
         void* host = nullptr;
@@ -442,7 +444,7 @@ int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* u, const taa
 // ---- batches ------------------------------------------------------------------------------
 
 static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int arith, void* host, void* dev, int& shape,
-                       bool& cas, bool& s16, int& total_tiles) {
+                       bool& cas, bool& s16, int& total_tiles, bool& fast) {
     if (n <= 0 || n > taa_ctx::kCap) return fail(TAA_ERR_CAPACITY, "batch of %d items (capacity %d)", n, taa_ctx::kCap);
     auto* ents = static_cast<taastar::BatchEntry*>(host);
     auto* unis = reinterpret_cast<taa_uniforms_t*>(static_cast<char*>(host) + align256(sizeof(taastar::BatchEntry) * taa_ctx::kCap));
@@ -484,6 +486,8 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
         tiles += e.fd.tiles_x * e.fd.tiles_y;
     }
     total_tiles = tiles;
+    fast = arith == TAA_ARITH_FP16;
+    for (int i = 0; i < n && fast; ++i) fast = taastar::fast_path_supported(ents[i].fd, unis[i], 0);
     return TAA_OK;
 }
 
@@ -497,14 +501,17 @@ int32_t taa_batch_submit(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items,
         if (e != hipSuccess) return hip_fail(e, "staging slot wait");
     }
     int shape = 0, total = 0;
-    bool cas = false, s16 = false;
-    int rc = build_batch(c, n, items, arith, slot.host, slot.dev, shape, cas, s16, total);
+    bool cas = false, s16 = false, fast = false;
+    int rc = build_batch(c, n, items, arith, slot.host, slot.dev, shape, cas, s16, total, fast);
     if (rc) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e = hipMemcpyAsync(slot.dev, slot.host, staging_bytes(), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "descriptor upload");
-    const int grid = taastar::batch_grid_size(shape, cas, s16, arith, c->device);
-    int le = taastar::launch_batch(static_cast<const taastar::BatchEntry*>(slot.dev), n, total, shape, cas, s16, arith, grid, stream);
+    const auto* d_ents = static_cast<const taastar::BatchEntry*>(slot.dev);
+    int le = fast ? taastar::launch_batch_fast(d_ents, n, total, shape, cas, s16,
+                                               taastar::batch_grid_size_fast(shape, cas, s16, c->device), stream)
+                  : taastar::launch_batch(d_ents, n, total, shape, cas, s16, arith,
+                                          taastar::batch_grid_size(shape, cas, s16, arith, c->device), stream);
     if (le) return hip_fail((hipError_t)le, "batch launch");
     if ((e = hipEventRecord(slot.done, st)) != hipSuccess) return hip_fail(e, "event record");
     slot.used = true;
@@ -521,15 +528,19 @@ struct taa_batch_plan {
     taa_ctx* ctx = nullptr;
     int n = 0, arith = 0;
     std::vector<taa_batch_item_t> items;
-    std::vector<taa_uniforms_t> uniforms;  // per-item copy refreshed from the caller each launch
+    // two descriptor buffers alternate between launches; each holds one captured graph per kernel
+    // family (FP32 kernel, FP16 kernel), captured the first time that family is needed
     struct Buf {
         void* host = nullptr;
         void* dev = nullptr;
-        hipGraphExec_t exec = nullptr;
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
+        int shape[2] = {-1, -1};
+        int total[2] = {0, 0};
         hipEvent_t done = nullptr;
         bool used = false;
     } buf[2];
     int next = 0;
+    hipStream_t cap = nullptr;
 };
 
 namespace {
@@ -537,11 +548,33 @@ namespace {
 void plan_free(taa_batch_plan* p) {
     for (auto& b : p->buf) {
         if (b.done) { hipEventSynchronize(b.done); hipEventDestroy(b.done); }
-        if (b.exec) hipGraphExecDestroy(b.exec);
+        for (auto& e : b.exec)
+            if (e) hipGraphExecDestroy(e);
         if (b.dev) hipFree(b.dev);
         if (b.host) hipHostFree(b.host);
     }
+    if (p->cap) hipStreamDestroy(p->cap);
     delete p;
+}
+
+// Capture descriptor upload + persistent kernel for one buffer and kernel family.
+int plan_capture(taa_batch_plan* p, taa_batch_plan::Buf& b, bool fast, int shape, bool cas, bool s16, int total) {
+    const int grid = fast ? taastar::batch_grid_size_fast(shape, cas, s16, p->ctx->device)
+                          : taastar::batch_grid_size(shape, cas, s16, p->arith, p->ctx->device);
+    hipGraph_t g = nullptr;
+    hipError_t e;
+    if ((e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
+    hipMemcpyAsync(b.dev, b.host, staging_bytes(), hipMemcpyHostToDevice, p->cap);
+    const auto* d_ents = static_cast<const taastar::BatchEntry*>(b.dev);
+    if (fast) taastar::launch_batch_fast(d_ents, p->n, total, shape, cas, s16, grid, p->cap);
+    else taastar::launch_batch(d_ents, p->n, total, shape, cas, s16, p->arith, grid, p->cap);
+    if ((e = hipStreamEndCapture(p->cap, &g)) != hipSuccess) return hip_fail(e, "end capture");
+    e = hipGraphInstantiate(&b.exec[fast ? 1 : 0], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+    b.shape[fast ? 1 : 0] = shape;
+    b.total[fast ? 1 : 0] = total;
+    return TAA_OK;
 }
 
 }  // namespace
@@ -552,6 +585,10 @@ int32_t taa_batch_plan_create(taa_ctx_t* c, int32_t n, const taa_batch_item_t* i
     if (!c || !items || !out) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
     *out = nullptr;
     if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith");
+    if (n <= 0 || n > taa_ctx::kCap) return fail(TAA_ERR_CAPACITY, "batch of %d items (capacity %d)", n, taa_ctx::kCap);
+    for (int i = 0; i < n; ++i)
+        if (!items[i].session || !items[i].uniforms || !items[i].result_screen)
+            return fail(TAA_ERR_INVALID_ARG, "item %d: NULL field", i);
     taa_batch_plan* p = new (std::nothrow) taa_batch_plan();
     if (!p) return fail(TAA_ERR_OUT_OF_MEMORY, "host allocation");
     p->ctx = c;
@@ -559,47 +596,16 @@ int32_t taa_batch_plan_create(taa_ctx_t* c, int32_t n, const taa_batch_item_t* i
     p->arith = arith;
     p->items.assign(items, items + n);
     hipSetDevice(c->device);
-    // validate once and fix the launch geometry (it does not change between launches)
-    int shape = 0, total = 0;
-    bool cas = false, s16 = false;
-    std::vector<char> scratch(staging_bytes());
-    std::vector<char> dscratch(staging_bytes());
-    int rc = build_batch(c, n, items, arith, scratch.data(), dscratch.data(), shape, cas, s16, total);
-    if (rc) { delete p; return rc; }
-    const int grid = taastar::batch_grid_size(shape, cas, s16, arith, c->device);
-    hipStream_t cap = nullptr;
-    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) { delete p; return fail(TAA_ERR_HIP, "stream"); }
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking)) != hipSuccess) { plan_free(p); return hip_fail(e, "stream"); }
     for (auto& b : p->buf) {
-        hipError_t e;
         if ((e = hipHostMalloc(&b.host, staging_bytes(), hipHostMallocDefault)) != hipSuccess ||
             (e = hipMalloc(&b.dev, staging_bytes())) != hipSuccess ||
             (e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming)) != hipSuccess) {
-            hipStreamDestroy(cap);
             plan_free(p);
             return fail(TAA_ERR_OUT_OF_MEMORY, "plan buffers: %s", hipGetErrorString(e));
         }
-        hipGraph_t g = nullptr;
-        if ((e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
-            hipStreamDestroy(cap);
-            plan_free(p);
-            return hip_fail(e, "begin capture");
-        }
-        hipMemcpyAsync(b.dev, b.host, staging_bytes(), hipMemcpyHostToDevice, cap);
-        taastar::launch_batch(static_cast<const taastar::BatchEntry*>(b.dev), n, total, shape, cas, s16, arith, grid, cap);
-        if ((e = hipStreamEndCapture(cap, &g)) != hipSuccess) {
-            hipStreamDestroy(cap);
-            plan_free(p);
-            return hip_fail(e, "end capture");
-        }
-        e = hipGraphInstantiate(&b.exec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            hipStreamDestroy(cap);
-            plan_free(p);
-            return hip_fail(e, "graph instantiate");
-        }
     }
-    hipStreamDestroy(cap);
     *out = p;
     return ok();
 }
@@ -612,11 +618,20 @@ int32_t taa_batch_plan_launch(taa_batch_plan_t* p, void* stream) {
         if (e != hipSuccess) return hip_fail(e, "plan buffer wait");
     }
     int shape = 0, total = 0;
-    bool cas = false, s16 = false;
-    int rc = build_batch(p->ctx, p->n, p->items.data(), p->arith, b.host, b.dev, shape, cas, s16, total);
+    bool cas = false, s16 = false, fast = false;
+    int rc = build_batch(p->ctx, p->n, p->items.data(), p->arith, b.host, b.dev, shape, cas, s16, total, fast);
     if (rc) return rc;
+    const int fam = fast ? 1 : 0;
+    if (b.exec[fam] && (b.shape[fam] != shape || b.total[fam] != total)) {  // tile table changed: recapture
+        hipGraphExecDestroy(b.exec[fam]);
+        b.exec[fam] = nullptr;
+    }
+    if (!b.exec[fam]) {
+        rc = plan_capture(p, b, fast, shape, cas, s16, total);
+        if (rc) return rc;
+    }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipError_t e = hipGraphLaunch(b.exec, st);
+    hipError_t e = hipGraphLaunch(b.exec[fam], st);
     if (e != hipSuccess) return hip_fail(e, "graph launch");
     if ((e = hipEventRecord(b.done, st)) != hipSuccess) return hip_fail(e, "event record");
     b.used = true;
@@ -687,7 +702,18 @@ int32_t taa_autotune(taa_ctx_t* c, int32_t w, int32_t h, int32_t fmt, int32_t ar
         }
         hd[i] = ((x / 37 + y / 23) & 1) ? 0.4f : 0.7f;
         float v2[2] = {(float)((x % 13) - 6) / (float)w, (float)((y % 11) - 5) / (float)h};
-        std::memcpy(&hv[i * 8], v2, 8);
+        if (fmt == TAA_FMT_RGBA16F) {  // RG16F: small normal values, truncated to fp16
+            for (int k = 0; k < 2; ++k) {
+                uint32_t u;
+                std::memcpy(&u, &v2[k], 4);
+                const int e = (int)((u >> 23) & 0xff) - 127;
+                uint16_t hb = (e < -14) ? (uint16_t)((u >> 16) & 0x8000u)
+                                        : (uint16_t)(((u >> 16) & 0x8000u) | ((uint32_t)(e + 15) << 10) | ((u >> 13) & 0x3ff));
+                std::memcpy(&hv[i * 4 + 2 * k], &hb, 2);
+            }
+        } else {
+            std::memcpy(&hv[i * 8], v2, 8);
+        }
     }
     void *dc = nullptr, *dh = nullptr, *dho = nullptr, *ds = nullptr, *dv = nullptr;
     float *dd = nullptr, *dhd = nullptr;
@@ -722,7 +748,7 @@ int32_t taa_autotune(taa_ctx_t* c, int32_t w, int32_t h, int32_t fmt, int32_t ar
     std::memset(&img, 0, sizeof(img));
     img.color = dc; img.depth = dd; img.history = dh; img.history_depth = dhd; img.result_screen = ds;
     img.velocity = dv; img.result_history = dho; img.width = w; img.height = h; img.pitch = w;
-    img.color_format = fmt; img.velocity_format = TAA_VEL_RG32F;
+    img.color_format = fmt; img.velocity_format = fmt == TAA_FMT_RGBA16F ? TAA_VEL_RG16F : TAA_VEL_RG32F;
     int rc = make_desc(&img, cas ? &K : nullptr, fd);
     if (rc) { release(); return rc; }
     hipEvent_t e0, e1;
@@ -734,9 +760,14 @@ int32_t taa_autotune(taa_ctx_t* c, int32_t w, int32_t h, int32_t fmt, int32_t ar
         const taastar::TileShapeDef sd = shape_def(s, cas != 0);
         fd.tiles_x = taastar::tiles_for(w, sd.tw);
         fd.tiles_y = taastar::tiles_for(h, sd.th);
-        for (int i = 0; i < 3; ++i) taastar::launch_resolve(fd, U, s, arith, st);
+        const bool fast = arith == TAA_ARITH_FP16 && taastar::fast_path_supported(fd, U, 0);
+        auto go = [&]() {
+            if (fast) taastar::launch_resolve_fast(fd, U, s, st);
+            else taastar::launch_resolve(fd, U, s, arith, st);
+        };
+        for (int i = 0; i < 3; ++i) go();
         hipEventRecord(e0, st);
-        for (int i = 0; i < iters; ++i) taastar::launch_resolve(fd, U, s, arith, st);
+        for (int i = 0; i < iters; ++i) go();
         hipEventRecord(e1, st);
         hipEventSynchronize(e1);
         float ms = 0.f;

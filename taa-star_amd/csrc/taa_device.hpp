@@ -1,0 +1,258 @@
+// Device helpers shared by the gfx950 resolve kernels: storage formats, FP32 operations in
+// taa.comp order, the CAS filter and the tile-to-XCD mapping.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "taa_internal.hpp"
+
+namespace taastar {
+namespace {
+
+// ------------------------------------------------------------------------------------------
+// storage
+
+__device__ __forceinline__ float h2f(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
+// fp32 -> fp16 storage conversion: round to nearest even, saturating to +-65504 (NaN stays NaN)
+// so that an overflow never becomes an Inf that the history filter would spread as NaN.
+__device__ __forceinline__ float sat_f16_range(float f) { return f == f ? fminf(fmaxf(f, -65504.0f), 65504.0f) : f; }
+__device__ __forceinline__ uint32_t f2h(float f) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)sat_f16_range(f)); }
+__device__ __forceinline__ float round_f16(float f) { return (float)(_Float16)sat_f16_range(f); }
+
+template <bool S16>
+__device__ __forceinline__ float4 ld_rgba(const void* p, size_t i) {
+    if constexpr (S16) {
+        uint2 v = static_cast<const uint2*>(p)[i];
+        return make_float4(h2f(v.x & 0xffffu), h2f(v.x >> 16), h2f(v.y & 0xffffu), h2f(v.y >> 16));
+    } else {
+        return static_cast<const float4*>(p)[i];
+    }
+}
+template <bool S16>
+__device__ __forceinline__ void st_rgba(void* p, size_t i, float4 v) {
+    if constexpr (S16) {
+        uint2 o;
+        o.x = f2h(v.x) | (f2h(v.y) << 16);
+        o.y = f2h(v.z) | (f2h(v.w) << 16);
+        static_cast<uint2*>(p)[i] = o;
+    } else {
+        static_cast<float4*>(p)[i] = v;
+    }
+}
+__device__ __forceinline__ void st_rgba16(void* p, size_t i, float4 v) { st_rgba<true>(p, i, v); }
+
+// Velocity texel; RG formats read back as (r, g, 0, 1).
+__device__ __forceinline__ float4 ld_vel(const void* p, int fmt, size_t i) {
+    switch (fmt) {
+        case TAA_VEL_RG32F: { float2 v = static_cast<const float2*>(p)[i]; return make_float4(v.x, v.y, 0.f, 1.f); }
+        case TAA_VEL_RG16F: { uint32_t v = static_cast<const uint32_t*>(p)[i]; return make_float4(h2f(v & 0xffffu), h2f(v >> 16), 0.f, 1.f); }
+        case TAA_VEL_RGBA32F: return static_cast<const float4*>(p)[i];
+        default: {
+            uint2 v = static_cast<const uint2*>(p)[i];
+            return make_float4(h2f(v.x & 0xffffu), h2f(v.x >> 16), h2f(v.y & 0xffffu), h2f(v.y >> 16));
+        }
+    }
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// int(x): NaN -> 0, saturating to the int range, truncation toward zero -- exactly what
+// V_CVT_I32_F32 does, so emit that instruction instead of letting the compiler treat
+// out-of-range conversions as undefined.
+__device__ __forceinline__ int to_int(float v) {
+    int r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP32 arithmetic in taa.comp order
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, f3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 vmin(f3 a, f3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+__device__ __forceinline__ f3 vmax(f3 a, f3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+__device__ __forceinline__ float gmix(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+__device__ __forceinline__ float gclamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+
+// taa.comp:158-174
+__device__ __forceinline__ f3 rgb_to_ycocg(f3 c) {
+    return {.25f * c.x + .5f * c.y + .25f * c.z, .5f * c.x - .5f * c.z, -.25f * c.x + .5f * c.y - .25f * c.z};
+}
+__device__ __forceinline__ f3 ycocg_to_rgb(f3 c) {
+    float tmp = c.x - c.z;
+    return {tmp + c.y, c.x + c.z, tmp - c.y};
+}
+// taa.comp:182-197
+__device__ __forceinline__ f3 tonemap_rgb(const taa_params_t& P, f3 hdr) {
+    if (P.mToneMapLumaKaris) {
+        float luma = fmaxf(fmaxf(hdr.x, hdr.y), hdr.z);
+        return hdr / (1.0f + luma);
+    }
+    return hdr;
+}
+__device__ __forceinline__ f3 un_tonemap_rgb(const taa_params_t& P, f3 ldr) {
+    if (P.mToneMapLumaKaris) {
+        float luma = fmaxf(fmaxf(ldr.x, ldr.y), ldr.z);
+        return ldr / (1.0f - luma);
+    }
+    return ldr;
+}
+__device__ __forceinline__ f3 stage_convert(const taa_params_t& P, f3 c) {
+    c = tonemap_rgb(P, c);
+    return P.mUseYCoCg ? rgb_to_ycocg(c) : c;
+}
+__device__ __forceinline__ float luminance(const taa_params_t& P, f3 c) { return P.mUseYCoCg ? c.x : rgb_to_ycocg(c).x; }
+__device__ __forceinline__ float linearize_depth(float d, float n, float f) { return n * f / (f + d * (n - f)); }
+
+// Bilinear (clamp to edge, exact weights) over an RGBA image -- the sampler of taa.hpp:274 at a
+// position that is not a texel centre.
+template <bool S16>
+__device__ float4 bilinear_rgba(const void* img, int W, int H, int pitch, float u, float v) {
+    float px = u * (float)W - 0.5f, py = v * (float)H - 0.5f;
+    float fx = floorf(px), fy = floorf(py);
+    float tx = px - fx, ty = py - fy;
+    int x0 = clampi(to_int(fx), -1, W), y0 = clampi(to_int(fy), -1, H);
+    int xa = clampi(x0, 0, W - 1), xb = clampi(x0 + 1, 0, W - 1);
+    int ya = clampi(y0, 0, H - 1), yb = clampi(y0 + 1, 0, H - 1);
+    float4 a = ld_rgba<S16>(img, (size_t)ya * pitch + xa), b = ld_rgba<S16>(img, (size_t)ya * pitch + xb);
+    float4 c = ld_rgba<S16>(img, (size_t)yb * pitch + xa), d = ld_rgba<S16>(img, (size_t)yb * pitch + xb);
+    float ux = 1.0f - tx, uy = 1.0f - ty;
+    float4 top = make_float4(a.x * ux + b.x * tx, a.y * ux + b.y * tx, a.z * ux + b.z * tx, a.w * ux + b.w * tx);
+    float4 bot = make_float4(c.x * ux + d.x * tx, c.y * ux + d.y * tx, c.z * ux + d.z * tx, c.w * ux + d.w * tx);
+    return make_float4(top.x * uy + bot.x * ty, top.y * uy + bot.y * ty, top.z * uy + bot.z * ty, top.w * uy + bot.w * ty);
+}
+
+// Catmull-Rom history fetch (taa.comp:441-514), 16-tap texelFetch form with the edge clamp
+// at size-1; identical to the 9-tap bilinear form up to rounding.
+template <bool S16>
+__device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
+    const int W = fd.W, H = fd.H;
+    float iTx = hu * (float)W, iTy = hv * (float)H;
+    float fbx = floorf(iTx - 0.5f), fby = floorf(iTy - 0.5f);
+    float tcx = fbx + 0.5f, tcy = fby + 0.5f;
+    float fx = iTx - tcx, fy = iTy - tcy;
+    float f2x = fx * fx, f2y = fy * fy, f3x = f2x * fx, f3y = f2y * fy;
+    float wx[4] = {-0.5f * f3x + f2x - 0.5f * fx, 1.5f * f3x - 2.5f * f2x + 1.0f,
+                   -1.5f * f3x + 2.0f * f2x + 0.5f * fx, 0.5f * f3x - 0.5f * f2x};
+    float wy[4] = {-0.5f * f3y + f2y - 0.5f * fy, 1.5f * f3y - 2.5f * f2y + 1.0f,
+                   -1.5f * f3y + 2.0f * f2y + 0.5f * fy, 0.5f * f3y - 0.5f * f2y};
+    int kx = clampi(to_int(fbx), -2, W + 1), ky = clampi(to_int(fby), -2, H + 1);
+    int xs[4], ys[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        xs[i] = clampi(kx - 1 + i, 0, W - 1);
+        ys[i] = clampi(ky - 1 + i, 0, H - 1) * fd.pitch;
+    }
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float4 t = ld_rgba<S16>(fd.history, (size_t)ys[j] + xs[i]);
+            float a = wx[i], b = wy[j];
+            float4 s = make_float4(t.x * a * b, t.y * a * b, t.z * a * b, t.w * a * b);
+            if (i == 0 && j == 0) acc = s;
+            else acc = make_float4(acc.x + s.x, acc.y + s.y, acc.z + s.z, acc.w + s.w);
+        }
+    return acc;
+}
+
+// B-spline history fetch (taa.comp:517-543): four bilinear taps.
+template <bool S16>
+__device__ float4 history_bspline(const FrameDesc& fd, float hu, float hv) {
+    const int W = fd.W, H = fd.H;
+    float sx = (float)W, sy = (float)H;
+    float ix = 1.0f / sx, iy = 1.0f / sy;
+    float iTx = hu * sx, iTy = hv * sy;
+    float tcx = floorf(iTx - 0.5f) + 0.5f, tcy = floorf(iTy - 0.5f) + 0.5f;
+    float fx = iTx - tcx, fy = iTy - tcy;
+    float f2x = fx * fx, f2y = fy * fy, f3x = f2x * fx, f3y = f2y * fy;
+    float w0x = f2x - 0.5f * (f3x + fx), w0y = f2y - 0.5f * (f3y + fy);
+    float w1x = 1.5f * f3x - 2.5f * f2x + 1.0f, w1y = 1.5f * f3y - 2.5f * f2y + 1.0f;
+    float w3x = 0.5f * (f3x - f2x), w3y = 0.5f * (f3y - f2y);
+    float w2x = 1.0f - w0x - w1x - w3x, w2y = 1.0f - w0y - w1y - w3y;
+    float s0x = w0x + w1x, s0y = w0y + w1y, s1x = w2x + w3x, s1y = w2y + w3y;
+    float f0x = w1x / (w0x + w1x), f0y = w1y / (w0y + w1y);
+    float f1x = w3x / (w2x + w3x), f1y = w3y / (w2y + w3y);
+    float t0x = (tcx - 1.0f + f0x) * ix, t0y = (tcy - 1.0f + f0y) * iy;
+    float t1x = (tcx + 1.0f + f1x) * ix, t1y = (tcy + 1.0f + f1y) * iy;
+    float4 a = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, t0x, t0y);
+    float4 b = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, t1x, t0y);
+    float4 c = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, t0x, t1y);
+    float4 d = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, t1x, t1y);
+    float4 r0 = make_float4(a.x * s0x + b.x * s1x, a.y * s0x + b.y * s1x, a.z * s0x + b.z * s1x, a.w * s0x + b.w * s1x);
+    float4 r1 = make_float4(c.x * s0x + d.x * s1x, c.y * s0x + d.y * s1x, c.z * s0x + d.z * s1x, c.w * s0x + d.w * s1x);
+    return make_float4(r0.x * s0y + r1.x * s1y, r0.y * s0y + r1.y * s1y, r0.z * s0y + r1.z * s1y, r0.w * s0y + r1.w * s1y);
+}
+
+// taa.comp:323-345 with p = (0,0,0,1)
+__device__ __forceinline__ f3 clipAabb(f3 mn, f3 mx, f3 q) {
+    const float eps = 1e-7f;
+    f3 pClip = 0.5f * (mx + mn);
+    f3 eClip = 0.5f * (mx - mn) + f3{eps, eps, eps};
+    f3 vClip = q - pClip;
+    f3 vUnit = vClip / eClip;
+    float ma = fmaxf(fabsf(vUnit.x), fmaxf(fabsf(vUnit.y), fabsf(vUnit.z)));
+    if (ma > 1.0f) return pClip + vClip / ma;
+    return q;
+}
+// taa.comp:348-369
+__device__ __forceinline__ f3 clipAabbSlow(f3 mn, f3 mx, f3 p, f3 q) {
+    f3 r = q - p, rmax = mx - p, rmin = mn - p;
+    const float eps = 1e-7f;
+    if (r.x > rmax.x + eps) r = r * (rmax.x / r.x);
+    if (r.y > rmax.y + eps) r = r * (rmax.y / r.y);
+    if (r.z > rmax.z + eps) r = r * (rmax.z / r.z);
+    if (r.x < rmin.x - eps) r = r * (rmin.x / r.x);
+    if (r.y < rmin.y - eps) r = r * (rmin.y / r.y);
+    if (r.z < rmin.z - eps) r = r * (rmin.z / r.z);
+    return p + r;
+}
+
+// ------------------------------------------------------------------------------------------
+// CAS (ffx_cas.h:424-537, sharpen only, green weight for all channels)
+
+__device__ __forceinline__ float prx_lo_sqrt(float a) { return __uint_as_float((__float_as_uint(a) >> 1u) + 0x1fbc4639u); }
+__device__ __forceinline__ float prx_lo_rcp(float a) { return __uint_as_float(0x7ef07ebbu - __float_as_uint(a)); }
+__device__ __forceinline__ float prx_med_rcp(float a) {
+    float b = __uint_as_float(0x7ef19fffu - __float_as_uint(a));
+    return b * (-b * a + 2.0f);
+}
+__device__ __forceinline__ float sat(float a) { return fminf(fmaxf(a, 0.0f), 1.0f); }
+__device__ __forceinline__ float min3(float x, float y, float z) { return fminf(x, fminf(y, z)); }
+__device__ __forceinline__ float max3(float x, float y, float z) { return fmaxf(x, fmaxf(y, z)); }
+
+__device__ __forceinline__ f3 cas_filter(f3 b, f3 d, f3 e, f3 f, f3 h, float peak) {
+    float mxG = max3(max3(d.y, e.y, f.y), b.y, h.y);
+    float mnG = min3(min3(d.y, e.y, f.y), b.y, h.y);
+    float rcpMG = prx_lo_rcp(mxG);
+    float ampG = sat(fminf(mnG, 1.0f - mxG) * rcpMG);
+    ampG = prx_lo_sqrt(ampG);
+    float wG = ampG * peak;
+    float rcpWeight = prx_med_rcp(1.0f + 4.0f * wG);
+    f3 o;
+    o.x = sat((b.x * wG + d.x * wG + f.x * wG + h.x * wG + e.x) * rcpWeight);
+    o.y = sat((b.y * wG + d.y * wG + f.y * wG + h.y * wG + e.y) * rcpWeight);
+    o.z = sat((b.z * wG + d.z * wG + f.z * wG + h.z * wG + e.z) * rcpWeight);
+    return o;
+}
+
+
+// Workgroup -> tile so that XCD k (blockIdx % 8) gets the contiguous k-th eighth of the tiles.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n >> 3, r = n & 7;
+    const int xcd = b & 7, slot = b >> 3;
+    return xcd * q + min(xcd, r) + slot;
+}
+
+}  // namespace
+}  // namespace taastar

@@ -1,0 +1,507 @@
+// gfx950 throughput kernel of the temporal resolve (TAA_ARITH_FP16): the taa.comp configuration
+// of variance clipping + Catmull-Rom history + simple or closest-depth velocity, with optional
+// clamp/clip, reject-outside, depth rejection (raw or linear), Lottes luma weighting and fused CAS.
+//
+// Arithmetic: texels stay fp16 (storage format) and enter the Catmull-Rom sums through mixed
+// fp16 x fp32 FMAs, variance moments, rectification and the blend run in fp32 with contraction
+// and the hardware reciprocal / square root. Everything that selects a texel -- the closest-depth
+// tap, historyUv = uv - v, the Catmull-Rom base texel floor(uv*size - 0.5), the depth-test texel,
+// the out-of-screen and depth-rejection decisions -- is computed exactly as in the FP32 path
+// (IEEE division for uv and for the linear depths), so indices and rejections are bit-exact.
+//
+// Per workgroup (256 threads): stage the tile + apron once (YCoCg fp32, depth, velocity bits, and
+// the tile's uv columns/rows), resolve the region, sharpen from LDS when CAS is fused.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "taa_device.hpp"
+#include "taa_internal.hpp"
+
+namespace taastar {
+namespace {
+
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+template <bool S16> struct VelSlot;
+template <> struct VelSlot<true> { typedef uint32_t T; };  // RG16F bits
+template <> struct VelSlot<false> { typedef float2 T; };   // RG32F
+
+template <int TW_, int TH_, bool CAS_, bool S16_>
+struct FGeo {
+    static constexpr int TW = TW_, TH = TH_;
+    static constexpr bool CAS = CAS_, S16 = S16_;
+    static constexpr int NT = 256;
+    static constexpr int A = CAS ? 2 : 1;
+    static constexpr int SW = TW + 2 * A, SH = TH + 2 * A, SP = SW + 1;
+    static constexpr int RA = CAS ? 1 : 0;
+    static constexpr int RW = TW + 2 * RA, RH = TH + 2 * RA, RP = RW + 1;
+    typedef typename VelSlot<S16>::T VelT;
+    // resolved screen texel kept for the fused CAS: the value the stored image would hold
+    typedef typename std::conditional<S16, uint2, float4>::type ResT;
+    struct Smem {
+        float4 cd[SH * SP];  // Y, Co, Cg (RGB without YCoCg), depth: one 16-byte LDS read per tap
+        VelT v[SH * SP];
+        float ux[SW], uy[SH];
+        ResT r[CAS ? RH * RP : 1];
+    };
+};
+
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+template <bool S16>
+__device__ __forceinline__ float2 vel_xy(typename VelSlot<S16>::T v) {
+    if constexpr (S16) return make_float2(h2f(v & 0xffffu), h2f(v >> 16));
+    else return v;
+}
+
+// Catmull-Rom weights (taa.comp:464-467) for one axis, fp32
+struct W4 { float w[4]; };
+__device__ __forceinline__ W4 cr_weights(float f) {
+    const float f2 = f * f;
+    W4 r;
+    r.w[0] = f * __builtin_fmaf(f, __builtin_fmaf(f, -0.5f, 1.0f), -0.5f);
+    r.w[1] = __builtin_fmaf(f2, __builtin_fmaf(f, 1.5f, -2.5f), 1.0f);
+    r.w[2] = f * __builtin_fmaf(f, __builtin_fmaf(f, -1.5f, 2.0f), 0.5f);
+    r.w[3] = f2 * __builtin_fmaf(f, 0.5f, -0.5f);
+    return r;
+}
+
+// one texel of the history as three fp32 channels (fp16 storage feeds mixed FMAs)
+template <bool S16>
+struct Texel {
+    float r, g, b;
+};
+template <bool S16>
+__device__ __forceinline__ Texel<S16> load_texel(const void* base, size_t i) {
+    if constexpr (S16) {
+        uint2 v = static_cast<const uint2*>(base)[i];
+        h2 rg = __builtin_bit_cast(h2, v.x);
+        h2 ba = __builtin_bit_cast(h2, v.y);
+        return {(float)rg.x, (float)rg.y, (float)ba.x};
+    } else {
+        float4 v = static_cast<const float4*>(base)[i];
+        return {v.x, v.y, v.z};
+    }
+}
+
+// Catmull-Rom 4x4 history filter (separable, fp32 accumulation), rgb only.
+template <bool S16>
+__device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, int pitch, int kx, int ky, float fx, float fy) {
+    const W4 wx = cr_weights(fx), wy = cr_weights(fy);
+    f3 acc = {0.f, 0.f, 0.f};
+    const bool interior = kx >= 1 && kx + 2 <= W - 1 && ky >= 1 && ky + 2 <= H - 1;
+    if (interior) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            // 32-bit product, widened before the column offsets so the four texel loads of a row
+            // fold into one address with immediate offsets
+            const size_t rb = (size_t)((uint32_t)(ky - 1 + j) * (uint32_t)pitch + (uint32_t)(kx - 1));
+            f3 row = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                Texel<S16> t = load_texel<S16>(hist, rb + i);
+                row.x = __builtin_fmaf(t.r, wx.w[i], row.x);
+                row.y = __builtin_fmaf(t.g, wx.w[i], row.y);
+                row.z = __builtin_fmaf(t.b, wx.w[i], row.z);
+            }
+            acc.x = __builtin_fmaf(row.x, wy.w[j], acc.x);
+            acc.y = __builtin_fmaf(row.y, wy.w[j], acc.y);
+            acc.z = __builtin_fmaf(row.z, wy.w[j], acc.z);
+        }
+    } else {
+        const int kxc = clampi(kx, -2, W + 1), kyc = clampi(ky, -2, H + 1);
+        int xs[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xs[i] = clampi(kxc - 1 + i, 0, W - 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t rb = (size_t)((uint32_t)clampi(kyc - 1 + j, 0, H - 1) * (uint32_t)pitch);
+            f3 row = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                Texel<S16> t = load_texel<S16>(hist, rb + xs[i]);
+                row.x = __builtin_fmaf(t.r, wx.w[i], row.x);
+                row.y = __builtin_fmaf(t.g, wx.w[i], row.y);
+                row.z = __builtin_fmaf(t.b, wx.w[i], row.z);
+            }
+            acc.x = __builtin_fmaf(row.x, wy.w[j], acc.x);
+            acc.y = __builtin_fmaf(row.y, wy.w[j], acc.y);
+            acc.z = __builtin_fmaf(row.z, wy.w[j], acc.z);
+        }
+    }
+    return acc;
+}
+
+// CAS (ffx_cas.h:424-537) with contraction: the same filter as cas_filter(), the four-neighbour
+// sum factored as (b + d + f + h) * w + e.
+__device__ __forceinline__ f3 cas_filter_fma(f3 b, f3 d, f3 e, f3 f, f3 h, float peak) {
+    const float mxG = fmaxf(fmaxf(d.y, fmaxf(e.y, f.y)), fmaxf(b.y, h.y));
+    const float mnG = fminf(fminf(d.y, fminf(e.y, f.y)), fminf(b.y, h.y));
+    float amp = sat(fminf(mnG, 1.0f - mxG) * prx_lo_rcp(mxG));
+    amp = prx_lo_sqrt(amp);
+    const float w = amp * peak;
+    const float rw = prx_med_rcp(__builtin_fmaf(4.0f, w, 1.0f));
+    f3 o;
+    o.x = sat(__builtin_fmaf((b.x + d.x) + (f.x + h.x), w, e.x) * rw);
+    o.y = sat(__builtin_fmaf((b.y + d.y) + (f.y + h.y), w, e.y) * rw);
+    o.z = sat(__builtin_fmaf((b.z + d.z) + (f.z + h.z), w, e.z) * rw);
+    return o;
+}
+
+template <class G, bool INSP>
+__device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm) {
+    constexpr bool S16 = G::S16;
+    const taa_params_t& P = U.param[fd.param_idx];
+    const int W = fd.W, H = fd.H;
+    const uint32_t pitch = (uint32_t)fd.pitch;
+    const int ty = tile / fd.tiles_x, tx = tile - ty * fd.tiles_x;
+    const int x0 = fd.x_begin + tx * G::TW, y0 = ty * G::TH;
+    const int gx0 = x0 - G::A, gy0 = y0 - G::A;
+    const int tid = threadIdx.x;
+
+    const bool ycocg = P.mUseYCoCg != 0;
+    const bool closest_mode = P.mVelocitySampleMode == 2;
+    const int rect = P.mColorClampingOrClipping;
+    const bool lottes = P.mLumaWeightingLottes != 0;
+    const bool reject_out = P.mRejectOutside != 0;
+    const bool depth_cull = P.mDepthCulling != 0;
+    const bool lin_depth = U.pad1 > 0.0f;
+    const bool reset = U.mResetHistory != 0;
+    const float gamma = P.mVarClipGamma;
+    const float fW = (float)W, fH = (float)H;
+    typedef typename G::VelT VelT;
+    const VelT* __restrict__ vel = static_cast<const VelT*>(fd.velocity);
+
+    // 1. stage the region: YCoCg (fp32), depth, velocity; uv of the staged columns and rows
+    {
+        constexpr int DY = G::NT / G::SW, DX = G::NT % G::SW;
+        int sy = tid / G::SW, sx = tid - sy * G::SW;
+        for (int i = tid; i < G::SW * G::SH; i += G::NT) {
+            const int gx = clampi(gx0 + sx, 0, W - 1), gy = clampi(gy0 + sy, 0, H - 1);
+            const uint32_t gi = (uint32_t)gy * pitch + (uint32_t)gx;
+            const float4 c = ld_rgba<S16>(fd.color, gi);
+            float Y = c.x, Co = c.y, Cg = c.z;
+            if (ycocg) {
+                const float t = 0.25f * (c.x + c.z);
+                Y = __builtin_fmaf(0.5f, c.y, t);
+                Cg = __builtin_fmaf(0.5f, c.y, -t);
+                Co = 0.5f * (c.x - c.z);
+            }
+            const int li = sy * G::SP + sx;
+            sm.cd[li] = make_float4(Y, Co, Cg, fd.depth[gi]);
+            sm.v[li] = vel[gi];
+            sx += DX;
+            sy += DY;
+            if (sx >= G::SW) { sx -= G::SW; sy += 1; }
+        }
+    }
+    if (tid < G::SW) sm.ux[tid] = ((float)clampi(gx0 + tid, 0, W - 1) + 0.5f) / fW;              // taa.comp:131
+    else if (tid < G::SW + G::SH) sm.uy[tid - G::SW] = ((float)clampi(gy0 + tid - G::SW, 0, H - 1) + 0.5f) / fH;
+    __syncthreads();
+
+    constexpr float inv9 = 1.0f / 9.0f;
+    constexpr int RDY = G::NT / G::RW, RDX = G::NT % G::RW;
+    int ry = tid / G::RW, rx = tid - ry * G::RW;
+    for (int i = tid; i < G::RW * G::RH; i += G::NT, rx += RDX, ry += RDY) {
+        if (rx >= G::RW) { rx -= G::RW; ry += 1; }
+        const int px = x0 - G::RA + rx, py = y0 - G::RA + ry;
+        const int cx = clampi(px, 0, W - 1), cy = clampi(py, 0, H - 1);
+        const bool owned = rx >= G::RA && rx < G::RA + G::TW && ry >= G::RA && ry < G::RA + G::TH && px < fd.x_end && py < H;
+        const int lx = cx - gx0, ly = cy - gy0, l = ly * G::SP + lx;
+
+        // ---- 3x3 neighbourhood: variance box (fp32 moments) and closest-depth tap
+        // (first minimum in scan order, strict <), from the same nine LDS reads
+        f3 m1, m2;
+        float dC;
+        int vo = -G::SP - 1, closest = 0;
+        {
+            const float4 t = sm.cd[l - G::SP - 1];
+            m1 = {t.x, t.y, t.z};
+            m2 = {t.x * t.x, t.y * t.y, t.z * t.z};
+            dC = t.w;
+        }
+#pragma unroll
+        for (int k = 1; k < 9; ++k) {
+            const int o = (k / 3 - 1) * G::SP + (k % 3 - 1);
+            const float4 t = sm.cd[l + o];
+            m1.x += t.x; m1.y += t.y; m1.z += t.z;
+            m2.x = __builtin_fmaf(t.x, t.x, m2.x);
+            m2.y = __builtin_fmaf(t.y, t.y, m2.y);
+            m2.z = __builtin_fmaf(t.z, t.z, m2.z);
+            if (t.w < dC) {
+                dC = t.w;
+                vo = o;
+                if constexpr (INSP) closest = k;
+            }
+        }
+        const f3 mean = {m1.x * inv9, m1.y * inv9, m1.z * inv9};
+        const f3 sig = {sqrt_fast(fmaxf(__builtin_fmaf(-mean.x, mean.x, m2.x * inv9), 0.f)),
+                        sqrt_fast(fmaxf(__builtin_fmaf(-mean.y, mean.y, m2.y * inv9), 0.f)),
+                        sqrt_fast(fmaxf(__builtin_fmaf(-mean.z, mean.z, m2.z * inv9), 0.f))};
+        const float4 ctr = sm.cd[l];
+        const f3 cur = {ctr.x, ctr.y, ctr.z};
+        const float depth = ctr.w;
+        if (!closest_mode) {  // velocity of the centre tap
+            vo = 0;
+            closest = -1;
+        }
+        const float2 v = vel_xy<S16>(sm.v[l + vo]);
+        const float hu = sm.ux[lx] - v.x, hv = sm.uy[ly] - v.y;  // taa.comp:421
+
+        // ---- Catmull-Rom history (taa.comp:441-514)
+        const float tX = hu * fW - 0.5f, tY = hv * fH - 0.5f;
+        const float fbx = floorf(tX), fby = floorf(tY);
+        const int kx = to_int(fbx), ky = to_int(fby);
+        const f3 hrgb = history_cr_fast<S16>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby);
+        f3 h = hrgb;
+        if (ycocg) {
+            const float t = 0.25f * (hrgb.x + hrgb.z);
+            h = {__builtin_fmaf(0.5f, hrgb.y, t), 0.5f * (hrgb.x - hrgb.z), __builtin_fmaf(0.5f, hrgb.y, -t)};
+        }
+
+        // ---- rejection (taa.comp:787-823), decisions as in the FP32 path
+        bool rejected = false;
+        int flags = TAA_FLAG_USED_VELOCITY, dtx = 0, dty = 0;
+        if (reject_out && (hu < 0.0f || hv < 0.0f || hu >= 1.0f || hv >= 1.0f)) {
+            rejected = true;
+            flags |= TAA_FLAG_OUTSIDE;
+        }
+        if (depth_cull) {
+            dtx = to_int(hu * fW);
+            dty = to_int(hv * fH);
+            const float hd = fd.hdepth[(uint32_t)clampi(dty, 0, H - 1) * pitch + (uint32_t)clampi(dtx, 0, W - 1)];
+            bool rej;
+            if (lin_depth) {
+                const float n = U.mCamNearPlane, f = U.mCamFarPlane;
+                rej = fabsf(linearize_depth(hd, n, f) - linearize_depth(depth, n, f)) > U.pad1;
+            } else {
+                rej = fabsf(hd - depth) > 0.1f * (1.0f - hd);
+            }
+            if (rej) {
+                rejected = true;
+                flags |= TAA_FLAG_DEPTH_REJECT;
+            }
+        }
+
+        // ---- rectification: clamp to / clip towards the variance box (centre = mean)
+        const f3 mn = {__builtin_fmaf(-gamma, sig.x, mean.x), __builtin_fmaf(-gamma, sig.y, mean.y), __builtin_fmaf(-gamma, sig.z, mean.z)};
+        const f3 mx = {__builtin_fmaf(gamma, sig.x, mean.x), __builtin_fmaf(gamma, sig.y, mean.y), __builtin_fmaf(gamma, sig.z, mean.z)};
+        if (rect == 2) {
+            const f3 vc = h - mean;
+            const float ax = fabsf(vc.x) * rcp_fast(__builtin_fmaf(gamma, sig.x, 1e-7f));
+            const float ay = fabsf(vc.y) * rcp_fast(__builtin_fmaf(gamma, sig.y, 1e-7f));
+            const float az = fabsf(vc.z) * rcp_fast(__builtin_fmaf(gamma, sig.z, 1e-7f));
+            const float ma = fmaxf(ax, fmaxf(ay, az));
+            if (ma > 1.0f) {
+                const float s = rcp_fast(ma);
+                h = {__builtin_fmaf(vc.x, s, mean.x), __builtin_fmaf(vc.y, s, mean.y), __builtin_fmaf(vc.z, s, mean.z)};
+            }
+        } else if (rect == 1) {
+            h = vmin(vmax(h, mn), mx);
+        }
+
+        // ---- blend (taa.comp:850-900)
+        float alpha = P.mAlpha;
+        if (rejected) {
+            alpha = P.mRejectionAlpha;
+        } else if (lottes) {
+            const float lc = ycocg ? cur.x : 0.25f * (cur.x + cur.z) + 0.5f * cur.y;
+            const float lh = ycocg ? h.x : 0.25f * (h.x + h.z) + 0.5f * h.y;
+            const float diff = fabsf(lc - lh) * rcp_fast(fmaxf(fmaxf(lc, lh), 0.2f));
+            const float w = 1.0f - diff;
+            alpha = __builtin_fmaf(w * w, P.mMinAlpha - P.mMaxAlpha, P.mMaxAlpha);
+        }
+        if (reset) alpha = 1.0f;
+        f3 aa = {__builtin_fmaf(alpha, cur.x - h.x, h.x), __builtin_fmaf(alpha, cur.y - h.y, h.y), __builtin_fmaf(alpha, cur.z - h.z, h.z)};
+        if (reset) aa = cur;  // exactly the current colour, whatever the history holds
+        if (ycocg) {
+            const float t = aa.x - aa.z;
+            aa = {t + aa.y, aa.x + aa.z, t - aa.y};
+        }
+
+        const uint32_t gi = (uint32_t)cy * pitch + (uint32_t)cx;
+        if (owned) {
+            st_rgba<S16>(fd.history_out, gi, make_float4(aa.x, aa.y, aa.z, 0.0f));
+            if constexpr (!G::CAS) st_rgba<S16>(fd.screen, gi, make_float4(aa.x, aa.y, aa.z, 1.0f));
+            if constexpr (INSP) {
+                if (rejected) flags |= TAA_FLAG_REJECTED;
+                int32_t* w = static_cast<int32_t*>(fd.inspect) + (size_t)gi * TAA_INSPECT_WORDS;
+                w[TAA_INSPECT_HIST_TC_X] = kx;
+                w[TAA_INSPECT_HIST_TC_Y] = ky;
+                w[TAA_INSPECT_CLOSEST_TAP] = closest;
+                w[TAA_INSPECT_FLAGS] = flags;
+                w[TAA_INSPECT_COLMIN + 0] = __float_as_int(mn.x);
+                w[TAA_INSPECT_COLMIN + 1] = __float_as_int(mn.y);
+                w[TAA_INSPECT_COLMIN + 2] = __float_as_int(mn.z);
+                w[TAA_INSPECT_COLMAX + 0] = __float_as_int(mx.x);
+                w[TAA_INSPECT_COLMAX + 1] = __float_as_int(mx.y);
+                w[TAA_INSPECT_COLMAX + 2] = __float_as_int(mx.z);
+                w[TAA_INSPECT_BLEND] = __float_as_int(alpha);
+                w[TAA_INSPECT_DEPTH_TC_X] = dtx;
+                w[TAA_INSPECT_DEPTH_TC_Y] = dty;
+                w[TAA_INSPECT_HIST_UV_X] = __float_as_int(hu);
+                w[TAA_INSPECT_HIST_UV_Y] = __float_as_int(hv);
+                w[15] = 0;
+            }
+        }
+        if constexpr (G::CAS) {
+            const int li = ry * G::RP + rx;
+            if constexpr (S16) sm.r[li] = make_uint2(f2h(aa.x) | (f2h(aa.y) << 16), f2h(aa.z));
+            else sm.r[li] = make_float4(aa.x, aa.y, aa.z, 0.0f);
+        }
+    }
+
+    if constexpr (G::CAS) {
+        __syncthreads();
+        const float peak = __uint_as_float(fd.cas_peak_bits);
+        constexpr int CDY = G::NT / G::TW, CDX = G::NT % G::TW;
+        int oy = tid / G::TW, ox = tid - oy * G::TW;
+        for (int i = tid; i < G::TW * G::TH; i += G::NT, ox += CDX, oy += CDY) {
+            if (ox >= G::TW) { ox -= G::TW; oy += 1; }
+            const int px = x0 + ox, py = y0 + oy;
+            if (px >= fd.x_end || py >= H) continue;
+            const int li = (oy + 1) * G::RP + (ox + 1);
+            auto at = [&](int k) {
+                if constexpr (S16) {
+                    const uint2 q = sm.r[k];
+                    return f3{h2f(q.x & 0xffffu), h2f(q.x >> 16), h2f(q.y & 0xffffu)};
+                } else {
+                    const float4 q = sm.r[k];
+                    return f3{q.x, q.y, q.z};
+                }
+            };
+            const f3 o = cas_filter_fma(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak);
+            st_rgba<S16>(fd.screen, (uint32_t)py * pitch + (uint32_t)px, make_float4(o.x, o.y, o.z, 1.0f));
+        }
+    }
+}
+
+struct FastArgs {
+    FrameDesc fd;
+    taa_uniforms_t U;
+};
+
+template <int TW, int TH, bool CAS, bool S16, bool INSP>
+__global__ __launch_bounds__(256) void k_resolve_fast(const FastArgs a) {
+    using G = FGeo<TW, TH, CAS, S16>;
+    __shared__ typename G::Smem sm;
+    const int n = a.fd.tiles_x * a.fd.tiles_y;
+    fast_tile<G, INSP>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
+}
+
+template <int TW, int TH, bool CAS, bool S16>
+__global__ __launch_bounds__(256) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
+    using G = FGeo<TW, TH, CAS, S16>;
+    __shared__ typename G::Smem sm;
+    const int per = (total_tiles + 7) >> 3;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
+    for (int j = slot; j < per; j += slots) {
+        const int t = xcd * per + j;
+        if (t >= total_tiles) break;
+        int lo = 0, hi = n_entries - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (entries[mid].tile_begin <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchEntry& e = entries[lo];
+        fast_tile<G, false>(e.fd, *e.U, t - e.tile_begin, sm);
+        __syncthreads();
+    }
+}
+
+template <bool CAS, bool S16, bool INSP>
+hipError_t launch_fast_t(const FrameDesc& fd, const taa_uniforms_t& U, int shape, hipStream_t st) {
+    FastArgs a;
+    a.fd = fd;
+    a.U = U;
+    const int n = fd.tiles_x * fd.tiles_y;
+    constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
+    switch (shape) {
+        case 0: hipLaunchKernelGGL((k_resolve_fast<S[0].tw, S[0].th, CAS, S16, INSP>), dim3(n), dim3(256), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_resolve_fast<S[1].tw, S[1].th, CAS, S16, INSP>), dim3(n), dim3(256), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_resolve_fast<S[2].tw, S[2].th, CAS, S16, INSP>), dim3(n), dim3(256), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_resolve_fast<S[3].tw, S[3].th, CAS, S16, INSP>), dim3(n), dim3(256), 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+template <bool CAS, bool S16>
+hipError_t launch_fast_batch_t(const BatchEntry* e, int n, int total, int shape, int grid, hipStream_t st) {
+    constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
+    switch (shape) {
+        case 0: hipLaunchKernelGGL((k_resolve_fast_batch<S[0].tw, S[0].th, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 1: hipLaunchKernelGGL((k_resolve_fast_batch<S[1].tw, S[1].th, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 2: hipLaunchKernelGGL((k_resolve_fast_batch<S[2].tw, S[2].th, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        default: hipLaunchKernelGGL((k_resolve_fast_batch<S[3].tw, S[3].th, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+    }
+    return hipGetLastError();
+}
+
+template <bool CAS, bool S16>
+const void* fast_batch_ptr(int shape) {
+    constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
+    switch (shape) {
+        case 0: return (const void*)&k_resolve_fast_batch<S[0].tw, S[0].th, CAS, S16>;
+        case 1: return (const void*)&k_resolve_fast_batch<S[1].tw, S[1].th, CAS, S16>;
+        case 2: return (const void*)&k_resolve_fast_batch<S[2].tw, S[2].th, CAS, S16>;
+        default: return (const void*)&k_resolve_fast_batch<S[3].tw, S[3].th, CAS, S16>;
+    }
+}
+
+}  // namespace
+
+bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx) {
+    const taa_params_t& P = U.param[param_idx];
+    if (fd.vel_fmt != (fd.color_fmt == TAA_FMT_RGBA16F ? TAA_VEL_RG16F : TAA_VEL_RG32F)) return false;
+    if (fd.debug) return false;
+    if (U.mBypassHistoryUpdate || P.mPassThrough || P.mToneMapLumaKaris || P.mUnjitterNeighbourhood ||
+        P.mUnjitterCurrentSample)
+        return false;
+    if (!P.mVarianceClipping || P.mShrinkChromaAxis) return false;
+    if (P.mColorClampingOrClipping < 0 || P.mColorClampingOrClipping > 2) return false;
+    if (P.mInterpolationMode != 2) return false;
+    if (P.mUseVelocityVectors == 0) return false;  // matrix reprojection
+    if (P.mVelocitySampleMode != 0 && P.mVelocitySampleMode != 2) return false;
+    if (P.mDynamicAntiGhosting || P.mVelBasedAlpha || P.mReduceBlendNearClamp || P.mAddNoise) return false;
+    return true;
+}
+
+int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape, void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F, cas = fd.cas_on != 0, insp = fd.inspect != nullptr;
+    hipError_t e;
+    if (insp) {
+        if (cas) e = s16 ? launch_fast_t<true, true, true>(fd, U, shape, st) : launch_fast_t<true, false, true>(fd, U, shape, st);
+        else e = s16 ? launch_fast_t<false, true, true>(fd, U, shape, st) : launch_fast_t<false, false, true>(fd, U, shape, st);
+    } else {
+        if (cas) e = s16 ? launch_fast_t<true, true, false>(fd, U, shape, st) : launch_fast_t<true, false, false>(fd, U, shape, st);
+        else e = s16 ? launch_fast_t<false, true, false>(fd, U, shape, st) : launch_fast_t<false, false, false>(fd, U, shape, st);
+    }
+    return (int)e;
+}
+
+int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, int grid,
+                      void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    if (cas) e = s16 ? launch_fast_batch_t<true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
+                     : launch_fast_batch_t<true, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+    else e = s16 ? launch_fast_batch_t<false, true>(d_entries, n_entries, total_tiles, shape, grid, st)
+                 : launch_fast_batch_t<false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+    return (int)e;
+}
+
+int batch_grid_size_fast(int shape, bool cas, bool s16, int device) {
+    const void* k = cas ? (s16 ? fast_batch_ptr<true, true>(shape) : fast_batch_ptr<true, false>(shape))
+                        : (s16 ? fast_batch_ptr<false, true>(shape) : fast_batch_ptr<false, false>(shape));
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
+    return ((per_cu * cus) + 7) & ~7;
+}
+
+}  // namespace taastar

@@ -61,4 +61,13 @@ int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, in
 // workgroups of one batched launch that keep every CU busy
 int batch_grid_size(int shape, bool cas, bool s16, int arith, int device);
 
+// Throughput kernel (taa_fast.hip): variance clip + Catmull-Rom + simple/closest-depth velocity
+// with RG velocity, fp16-fed mixed arithmetic. fast_path_supported() says whether a parameter
+// block is one it implements; everything else runs the FP32 kernel.
+bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx);
+int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape, void* stream);
+int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16,
+                      int grid, void* stream);
+int batch_grid_size_fast(int shape, bool cas, bool s16, int device);
+
 }  // namespace taastar

@@ -105,6 +105,44 @@ class SceneConfig(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64), ("color_format", C.c_int32),
+                ("arith", C.c_int32), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
+                ("cas_sharpness", C.c_float)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("session_id", C.c_int32), ("pad", C.c_int32), ("tag", C.c_uint64), ("t_submit_us", C.c_double),
+                ("t_done_us", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("batches", C.c_uint64), ("frames", C.c_uint64), ("pixels", C.c_uint64),
+                ("sessions_open", C.c_uint64), ("admissions_refused", C.c_uint64), ("hbm_used", C.c_uint64),
+                ("hbm_budget", C.c_uint64), ("busy_us", C.c_double)]
+
+
+class ServeSimConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("seed", C.c_uint64), ("mode", C.c_int32), ("sessions", C.c_int32),
+                ("n_classes", C.c_int32), ("class_w", C.c_int32 * 8), ("class_h", C.c_int32 * 8),
+                ("class_p", C.c_float * 8), ("arrival_rate", C.c_double), ("churn_per_s", C.c_double),
+                ("duration_s", C.c_double), ("hbm_budget_bytes", C.c_uint64), ("color_format", C.c_int32),
+                ("arith", C.c_int32), ("cas_sharpness", C.c_float), ("max_batch", C.c_int32),
+                ("max_inflight", C.c_int32), ("warmup_frames", C.c_int32)]
+
+
+class ServeReport(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("pixels", C.c_uint64), ("batches", C.c_uint64), ("admissions", C.c_uint64),
+                ("admissions_refused", C.c_uint64), ("sessions_end", C.c_int32), ("sessions_peak", C.c_int32),
+                ("wall_s", C.c_double), ("gpix_per_s", C.c_double), ("offered_fps", C.c_double),
+                ("achieved_fps", C.c_double), ("lat_p50_us", C.c_double), ("lat_p90_us", C.c_double),
+                ("lat_p99_us", C.c_double), ("lat_max_us", C.c_double), ("lat_mean_us", C.c_double),
+                ("mean_batch", C.c_double), ("gpu_busy_frac", C.c_double), ("hbm_peak_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 assert C.sizeof(Params) == 176 and Params.mDebugMask.offset == 144
 assert C.sizeof(Uniforms) == 544 and Uniforms.param.offset == 128 and Uniforms.mJitterNdc.offset == 480
 assert Uniforms.mResetHistory.offset == 528 and Uniforms.pad1.offset == 540
@@ -145,6 +183,18 @@ PROTOTYPES = {
     "taa_scene_create": (I32, [P(SceneConfig), P(VP)]),
     "taa_scene_destroy": (I32, [VP]),
     "taa_scene_render": (I32, [VP, I32, P(F32), P(F32), P(F32), P(F32), I32]),
+    "taa_server_create": (I32, [P(ServerConfig), P(VP)]),
+    "taa_server_destroy": (I32, [VP]),
+    "taa_server_open": (I32, [VP, I32, I32, P(I32)]),
+    "taa_server_close": (I32, [VP, I32]),
+    "taa_server_io": (I32, [VP, I32, P(SessionIO), P(VP)]),
+    "taa_server_submit": (I32, [VP, I32, P(Uniforms), U64, C.c_double]),
+    "taa_server_pump": (I32, [VP]),
+    "taa_server_poll": (I32, [VP, P(Completion), I32, P(I32)]),
+    "taa_server_drain": (I32, [VP]),
+    "taa_server_stats": (I32, [VP, P(ServerStats)]),
+    "taa_server_now_us": (C.c_double, []),
+    "taa_serve_simulate": (I32, [P(ServeSimConfig), P(ServeReport)]),
 }
 
 _lib = None

@@ -9,8 +9,15 @@ REL_TOL = 1e-5      # FP32 path vs FP32 transcription, relative per channel
 ABS_FLOOR = 1e-3    # ... of max(|ref|, ABS_FLOOR): YCoCg / Catmull-Rom intermediates cross zero
 
 
+def to_f16(a):
+    """fp32 -> fp16 as the kernels store it: nearest even, saturating to +-65504, NaN kept."""
+    a = np.asarray(a, np.float32)
+    with np.errstate(invalid="ignore"):
+        return np.where(np.isnan(a), a, np.clip(a, -65504.0, 65504.0)).astype(np.float16)
+
+
 def round_storage(a, fmt):
-    return a.astype(np.float16).astype(np.float32) if fmt == FMT_RGBA16F else a.astype(np.float32)
+    return to_f16(a).astype(np.float32) if fmt == FMT_RGBA16F else a.astype(np.float32)
 
 
 def rel_err(got, ref, floor=ABS_FLOOR):
@@ -74,7 +81,7 @@ class Sequence:
             o_scr = round_storage(o["screen"], self.fmt)
             if self.cas is not None:
                 o_scr = round_storage(oracle.cas(o_scr, self.cas.const1[0]), self.fmt)
-            o_dbg = o["debug"].astype(np.float16).astype(np.float32)  # uDebug is RGBA16F
+            o_dbg = to_f16(o["debug"]).astype(np.float32)  # uDebug is RGBA16F
             out.update(oracle_history=o_hist, oracle_screen=o_scr, oracle_inspect=o["inspect"], oracle_debug=o_dbg)
             self.hist_o = o_hist
         self.prev_depth = dep

@@ -1,0 +1,136 @@
+"""The FP16-arithmetic path (TAA_ARITH_FP16) on FP16 storage against the FP64 oracle.
+
+Bar: reprojected texel indices, closest-depth taps, depth-test texels and rejection flags
+bit-exact against the FP32 transcription (they are computed in fp32 on both sides); the
+sharpened screen (display-referred, CAS output in [0,1], peak 1.0) >= 55 dB PSNR against the FP64
+oracle on every frame, over the pixels outside the reference's own divergence footprint.
+
+Divergence footprint: taa.comp's Lottes weight w = 1 - |lc - lh| / max(lc, lh, 0.2) is not clamped
+(taa.comp:865-869); once Catmull-Rom undershoot makes a luma negative, w*w > 1 makes alpha negative
+and the blend extrapolates. On the sigma = 2 HDR scene a few pixels then grow without bound in the
+FP64 oracle itself (1e9 and more) and their outcome is chaotic: the FP32 transcription with the
+same FP16 storage also differs there by up to 1.0 after CAS saturation. Those pixels are found from
+the FP64 history alone (|h| > 4x the frame's input maximum in the last 8 frames, dilated by the
+Catmull-Rom support) and excluded from the PSNR bar; their count is reported and bounded, the
+full-frame PSNR is reported and must stay within 3 dB of the FP32-arithmetic / FP16-storage one.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from gpu_helpers import INDEX_WORDS, psnr, to_f16  # noqa: E402
+from taastar import FLAG_RECTIFIED  # noqa: E402
+from taastar import ARITH_FP16, INSPECT  # noqa: E402
+from taastar.workloads import (cas_for, decode, encode_color, encode_velocity, scene_for, uniforms_for,  # noqa: E402
+                               velocity_as_read, workload)
+from scipy.ndimage import binary_dilation  # noqa: E402
+
+PSNR_MIN_DB = 55.0
+
+
+@pytest.fixture(autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def run_fp16_sequence(oracle, name, W, H, frames, report):
+    from taastar.device import FrameBuffers
+    w = workload(name)
+    sc = scene_for(w, width=W, height=H)
+    k = cas_for(w, W, H)
+    bufs = FrameBuffers(W, H, w["fmt"], w["vel"], inspect=True)
+    h64 = np.zeros((H, W, 4))
+    h32 = np.zeros((H, W, 4), np.float32)
+    prev = np.zeros((H, W), np.float32)
+    recent = []
+    worst = 1e9
+    for f in range(frames):
+        col, dep, vel4, jit = sc.render(f)
+        col_s, vel_s = encode_color(col, w["fmt"]), encode_velocity(vel4, w["vel"])
+        bufs.upload(col_s, dep, vel_s)
+        U = uniforms_for(w, f, jit)
+        bufs.resolve(U, k, ARITH_FP16)
+        torch.cuda.synchronize()
+        g_scr = decode(bufs.screen.cpu().numpy())
+        g_his = decode(bufs.hist_in.cpu().numpy())
+        g_ins = bufs.inspect.cpu().numpy()
+        args = (decode(col_s), dep, velocity_as_read(vel_s))
+        o64 = oracle.resolve(U, *args, h64, prev, dtype=np.float64, debug=False, inspect=False)
+        s64 = oracle.cas(o64["screen"], k.const1[0], dtype=np.float64)
+        # selections are made in fp32 arithmetic: check them against the FP32 transcription
+        o32 = oracle.resolve(U, *args, h32, prev, dtype=np.float32, debug=False)
+        for wd in INDEX_WORDS:
+            g, o = g_ins[..., wd], o32["inspect"][..., wd]
+            if wd == INSPECT["FLAGS"]:  # "rectified" is a threshold on arithmetic, not a selection
+                g, o = g & ~FLAG_RECTIFIED, o & ~FLAG_RECTIFIED
+            bad = np.count_nonzero(g != o)
+            assert bad == 0, f"frame {f}: inspect word {wd} differs at {bad} texels"
+        # divergence footprint of the FP64 oracle (see module docstring)
+        cmax = float(decode(col_s)[..., :3].max())
+        recent.append(np.abs(o64["history"][..., :3]).max(-1) > 4.0 * cmax)
+        del recent[:-8]
+        unstable = binary_dilation(np.logical_or.reduce(recent), iterations=3)
+        stable = ~unstable
+        s32 = oracle.cas(to_f16(o32["screen"]).astype(np.float32), k.const1[0])
+        p_all = psnr(g_scr[..., :3], s64[..., :3], 1.0)
+        p_base = psnr(s32[..., :3], s64[..., :3], 1.0)
+        p_scr = psnr(g_scr[stable][:, :3], s64[stable][:, :3], 1.0)
+        err = float(np.abs(g_scr[stable][:, :3] - s64[stable][:, :3]).max())
+        report.append(dict(frame=f, screen_psnr_db=round(p_scr, 2), screen_max_abs=err, full_frame_psnr_db=round(p_all, 2),
+                           fp32_fp16storage_psnr_db=round(p_base, 2), masked_pixels=int(unstable.sum())))
+        worst = min(worst, p_scr)
+        assert p_scr >= PSNR_MIN_DB, f"frame {f}: screen PSNR {p_scr:.2f} dB < {PSNR_MIN_DB}: {report[-1]}"
+        assert p_all >= min(p_base - 3.0, 60.0), f"frame {f}: full-frame {p_all:.2f} dB vs FP32 path {p_base:.2f} dB"
+        assert unstable.sum() <= 1e-3 * W * H, report[-1]
+        h64 = o64["history"]
+        h32 = to_f16(o32["history"]).astype(np.float32)
+        prev = dep
+    print(f"{name} {W}x{H} x{frames}: worst screen PSNR {worst:.2f} dB", report, flush=True)
+    return worst
+
+
+def test_fp16_path_960x540_16_frames(oracle):
+    print("case fp16 c3-scene 960x540 x16", flush=True)
+    rep = []
+    run_fp16_sequence(oracle, "c3", 960, 540, 16, rep)
+
+
+def test_fp16_path_full_4k(oracle):
+    print("case fp16 c3 3840x2160 x6", flush=True)
+    rep = []
+    run_fp16_sequence(oracle, "c3", 3840, 2160, 6, rep)
+
+
+def test_fp16_path_fp32_storage_indices(oracle):
+    """ARITH_FP16 on FP32 storage (c2 workload): same exactness of the selection logic."""
+    rep = []
+    from taastar.device import FrameBuffers
+    w = workload("c2")
+    W, H = 320, 180
+    sc = scene_for(w, width=W, height=H)
+    k = cas_for(w, W, H)
+    bufs = FrameBuffers(W, H, w["fmt"], w["vel"], inspect=True)
+    h32 = np.zeros((H, W, 4), np.float32)
+    prev = np.zeros((H, W), np.float32)
+    for f in range(6):
+        col, dep, vel4, jit = sc.render(f)
+        col_s, vel_s = encode_color(col, w["fmt"]), encode_velocity(vel4, w["vel"])
+        bufs.upload(col_s, dep, vel_s)
+        U = uniforms_for(w, f, jit)
+        bufs.resolve(U, k, ARITH_FP16)
+        torch.cuda.synchronize()
+        g_ins = bufs.inspect.cpu().numpy()
+        o = oracle.resolve(U, decode(col_s), dep, velocity_as_read(vel_s), h32, prev, dtype=np.float32, debug=False)
+        for wd in INDEX_WORDS:
+            g, oo = g_ins[..., wd], o["inspect"][..., wd]
+            if wd == INSPECT["FLAGS"]:
+                g, oo = g & ~FLAG_RECTIFIED, oo & ~FLAG_RECTIFIED
+            assert np.count_nonzero(g != oo) == 0, (f, wd)
+        g_scr = decode(bufs.screen.cpu().numpy())
+        s = oracle.cas(o["screen"], k.const1[0])
+        assert psnr(g_scr[..., :3], s[..., :3]) > 60.0
+        h32, prev = o["history"], dep

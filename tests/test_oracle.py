@@ -1,0 +1,517 @@
+"""Pins the oracle (the FP64 / FP32 transcription of taa.comp + CasFilter) against checks that do
+not restate its formulas: closed forms, numpy brute force, textbook kernels, the reference's own
+compiled CasSetup, and algebraic invariants. Runs on the CPU."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import taastar
+from taastar import INSPECT
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+f32 = np.float32
+
+
+# ---------------------------------------------------------------------------------------------
+# helpers
+
+def params(**kw):
+    p = taastar.default_params()
+    # a neutral base: RGB, no box, no rectification, no rejection, alpha from kw
+    p.mUseYCoCg = 0
+    p.mVarianceClipping = 0
+    p.mColorClampingOrClipping = 0
+    p.mInterpolationMode = 2
+    p.mUseVelocityVectors = 2
+    p.mVelocitySampleMode = 0
+    p.mAlpha = 0.0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def uniforms(p, reset=0, near=0.1, far=100.0, pad1=0.0):
+    U = taastar.Uniforms()
+    U.param[0] = p
+    U.param[1] = p
+    U.mResetHistory = reset
+    U.mCamNearPlane, U.mCamFarPlane, U.pad1 = near, far, pad1
+    for i in (0, 5, 10, 15):
+        U.mHistoryViewProjMatrix[i] = 1.0
+        U.mInverseViewProjMatrix[i] = 1.0
+    return U
+
+
+def frame(H, W, rng, vel=(0.0, 0.0), hist=None, depth=None, color=None):
+    col = color if color is not None else np.concatenate(
+        [rng.uniform(0.05, 2.0, (H, W, 3)), np.ones((H, W, 1))], -1).astype(f32)
+    dep = depth if depth is not None else rng.uniform(0.1, 1.0, (H, W)).astype(f32)
+    v = np.zeros((H, W, 4), f32)
+    v[..., 0], v[..., 1], v[..., 3] = vel[0], vel[1], 1.0
+    h = hist if hist is not None else np.concatenate([rng.uniform(0.05, 2.0, (H, W, 3)), np.zeros((H, W, 1))], -1)
+    return col, dep, v, h
+
+
+def run(oracle, U, col, dep, vel, hist, hdep=None, dtype=np.float64, cr16=True):
+    hdep = dep if hdep is None else hdep
+    return oracle.resolve(U, col, dep, vel, np.asarray(hist, dtype), hdep, dtype=dtype, cr16=cr16, threads=4)
+
+
+def box(o):
+    ins = o["inspect"]
+    mn = ins[..., INSPECT["COLMIN"]:INSPECT["COLMIN"] + 3].copy().view(f32)
+    mx = ins[..., INSPECT["COLMAX"]:INSPECT["COLMAX"] + 3].copy().view(f32)
+    return mn, mx
+
+
+def windows(a):
+    """3x3 edge-clamped windows of [H, W, ...] in taa.comp's tap order: centre first, then
+    (-1,-1) (0,-1) (1,-1) (-1,0) (1,0) (-1,1) (0,1) (1,1)."""
+    pad = np.pad(a, [(1, 1), (1, 1)] + [(0, 0)] * (a.ndim - 2), mode="edge")
+    H, W = a.shape[:2]
+    offs = [(0, 0), (-1, -1), (0, -1), (1, -1), (-1, 0), (1, 0), (-1, 1), (0, 1), (1, 1)]
+    return np.stack([pad[1 + dy:1 + dy + H, 1 + dx:1 + dx + W] for dx, dy in offs])
+
+
+def ycocg64(c):
+    r, g, b = (c[..., i].astype(np.float64) for i in range(3))
+    return np.stack([0.25 * r + 0.5 * g + 0.25 * b, 0.5 * r - 0.5 * b, -0.25 * r + 0.5 * g - 0.25 * b], -1)
+
+
+# ---------------------------------------------------------------------------------------------
+# jitter, CAS constants
+
+def test_halton_known_answers(oracle):
+    h2 = [1 / 2, 1 / 4, 3 / 4, 1 / 8, 5 / 8, 3 / 8, 7 / 8, 1 / 16]
+    h3 = [1 / 3, 2 / 3, 1 / 9, 4 / 9, 7 / 9, 2 / 9, 5 / 9, 8 / 9]
+    for i in range(8):
+        assert oracle.lib().oracle_halton(i + 1, 2) == f32(h2[i])  # dyadic: exact
+        assert abs(oracle.lib().oracle_halton(i + 1, 3) - h3[i]) <= 2 * np.spacing(f32(h3[i]))
+
+
+def test_jitter_selection_closed_forms(oracle):
+    W, H = 640, 360
+    px, py = f32(2) / f32(W), f32(2) / f32(H)
+    h2 = [1 / 2, 1 / 4, 3 / 4, 1 / 8, 5 / 8, 3 / 8, 7 / 8, 1 / 16]
+    h3 = [1 / 3, 2 / 3, 1 / 9, 4 / 9, 7 / 9, 2 / 9, 5 / 9, 8 / 9]
+    for f in range(20):
+        (x, y), n = oracle.jitter(2, f, W, H)
+        assert n == 8
+        k = f % 8
+        assert abs(x - px * (h2[k] - 0.5)) < 1e-8 and abs(y - py * (h3[k] - 0.5)) < 1e-8
+        (x, y), n = oracle.jitter(0, f, W, H)
+        q = [(-.25, -.25), (.25, -.25), (.25, .25), (-.25, .25)][f % 4]
+        assert n == 4 and (x, y) == (px * f32(q[0]), py * f32(q[1]))
+        (x, y), _ = oracle.jitter(4, f, W, H)
+        assert abs(x - px * (2 * (f % 4) - 3) / 8) < 1e-9 and abs(y - py * (2 * ((f % 16) // 4) - 3) / 8) < 1e-9
+    # slow motion divides the frame id, a fixed index pins it, rotation by 90 degrees swaps axes
+    assert oracle.jitter(2, 13, W, H, slow=4)[0] == oracle.jitter(2, 3, W, H)[0]
+    assert oracle.jitter(2, 13, W, H, fixed=5)[0] == oracle.jitter(2, 5, W, H)[0]
+    (x0, y0), _ = oracle.jitter(2, 1, W, H)
+    (xr, yr), _ = oracle.jitter(2, 1, W, H, rot=90.0)
+    assert abs(xr + y0) < 1e-7 and abs(yr - x0) < 1e-7
+
+
+def test_cas_setup_closed_form(oracle):
+    for s in np.linspace(0, 1, 41):
+        c0, c1 = oracle.cas_setup(float(s), 1920, 1080)
+        peak = np.array(c1[0], np.uint32).view(f32)
+        exact = -1.0 / (8.0 + (5.0 - 8.0) * float(f32(s)))
+        assert abs(peak - exact) <= 2 * np.spacing(f32(abs(exact)))
+        half = np.array(c1[1] & 0xFFFF, np.uint16).view(np.float16)
+        assert abs(float(half) - exact) <= abs(exact) * 2 ** -10  # truncated to fp16
+        assert c1[1] >> 16 == 0 and c0 == [0x3F800000, 0x3F800000, 0, 0] and c1[2] == 0x41000000
+
+
+def test_cas_setup_against_committed_reference_vectors(oracle):
+    """tests/golden/cas_setup.json was written by tests/golden/make_cas_setup.py from the
+    reference's own ffx_cas.h compiled for the CPU."""
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "cas_setup.json")))
+    assert len(g["cases"]) >= 100
+    for case in g["cases"]:
+        c0, c1 = oracle.cas_setup(case["sharpness"], case["in_w"], case["in_h"])
+        assert c0 == case["const0"] and c1 == case["const1"], case
+        k = taastar.cas_setup(case["sharpness"], case["in_w"], case["in_h"])
+        assert list(k.const0) == case["const0"] and list(k.const1) == case["const1"]
+
+
+# ---------------------------------------------------------------------------------------------
+# neighbourhood box (taa.comp:259-319)
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_minmax_box_is_the_3x3_order_statistic(oracle, dtype):
+    rng = np.random.default_rng(1)
+    col, dep, vel, hist = frame(17, 23, rng)
+    o = run(oracle, uniforms(params()), col, dep, vel, hist, dtype=dtype)
+    mn, mx = box(o)
+    w = windows(col[..., :3])
+    assert np.array_equal(mn, w.min(0)) and np.array_equal(mx, w.max(0))
+
+
+def test_minmax_box_in_ycocg(oracle):
+    rng = np.random.default_rng(2)
+    col, dep, vel, hist = frame(11, 19, rng)
+    o = run(oracle, uniforms(params(mUseYCoCg=1)), col, dep, vel, hist)
+    mn, mx = box(o)
+    w = windows(ycocg64(col))
+    assert np.array_equal(mn, w.min(0).astype(f32)) and np.array_equal(mx, w.max(0).astype(f32))
+
+
+@pytest.mark.parametrize("gamma", [1.0, 0.75, 0.0])
+def test_variance_box_is_mean_plus_minus_gamma_sigma(oracle, gamma):
+    rng = np.random.default_rng(3)
+    col, dep, vel, hist = frame(13, 21, rng)
+    p = params(mVarianceClipping=1, mVarClipGamma=gamma)
+    w = windows(col[..., :3].astype(np.float64))
+    mean, std = w.mean(0), w.std(0)  # population sigma over the 9 taps
+    for dtype, tol in [(np.float64, 1e-6), (np.float32, 2e-5)]:
+        mn, mx = box(run(oracle, uniforms(p), col, dep, vel, hist, dtype=dtype))
+        assert np.abs(mn - (mean - gamma * std)).max() < tol and np.abs(mx - (mean + gamma * std)).max() < tol
+    if gamma == 0.0:  # the box collapses to the mean
+        assert np.array_equal(mn, mx)
+
+
+def test_shaped_box_and_chroma_shrink(oracle):
+    rng = np.random.default_rng(4)
+    col, dep, vel, hist = frame(9, 14, rng)
+    w = windows(ycocg64(col))
+    cross = w[[0, 2, 4, 5, 7]]
+    mn = (w.min(0) + cross.min(0)) * 0.5
+    mx = (w.max(0) + cross.max(0)) * 0.5
+    o = run(oracle, uniforms(params(mUseYCoCg=1, mShapedNeighbourhood=1)), col, dep, vel, hist)
+    gmn, gmx = box(o)
+    assert np.abs(gmn - mn).max() < 1e-6 and np.abs(gmx - mx).max() < 1e-6
+    # chroma shrink: same centre, luma extent kept, chroma extents halved
+    o = run(oracle, uniforms(params(mUseYCoCg=1, mShapedNeighbourhood=1, mShrinkChromaAxis=1)), col, dep, vel, hist)
+    smn, smx = box(o)
+    assert np.abs((smn + smx) - (mn + mx)).max() < 1e-6
+    ext = (mx - mn) * np.array([1.0, 0.5, 0.5])
+    assert np.abs((smx - smn) - ext).max() < 1e-6
+
+
+# ---------------------------------------------------------------------------------------------
+# reprojection (taa.comp:371-438)
+
+def test_closest_depth_tap_first_minimum_in_scan_order(oracle):
+    rng = np.random.default_rng(5)
+    H, W = 19, 27
+    dep = rng.integers(0, 3, (H, W)).astype(f32) * f32(0.25) + f32(0.25)  # heavy ties
+    dep[:4, :5] = 0.5  # an all-equal block -> tap 0 (-1,-1)
+    col, _, vel, hist = frame(H, W, rng, depth=dep)
+    o = run(oracle, uniforms(params(mVelocitySampleMode=2)), col, dep, vel, hist)
+    pad = np.pad(dep, 1, mode="edge")
+    scan = np.stack([pad[1 + dy:1 + dy + H, 1 + dx:1 + dx + W] for dy in (-1, 0, 1) for dx in (-1, 0, 1)])
+    expect = np.argmin(scan, axis=0)  # numpy returns the first minimum
+    assert np.array_equal(o["inspect"][..., INSPECT["CLOSEST_TAP"]], expect)
+    assert np.all(o["inspect"][1:3, 1:4, INSPECT["CLOSEST_TAP"]] == 0)
+
+
+def test_history_uv_follows_velocity(oracle):
+    rng = np.random.default_rng(6)
+    H, W = 12, 20
+    vx, vy = f32(0.0137), f32(-0.0291)
+    col, dep, vel, hist = frame(H, W, rng, vel=(vx, vy))
+    for dtype in (np.float64, np.float32):
+        o = run(oracle, uniforms(params()), col, dep, vel, hist, dtype=dtype)
+        hu = o["inspect"][..., INSPECT["HIST_UV_X"]].copy().view(f32)
+        hv = o["inspect"][..., INSPECT["HIST_UV_Y"]].copy().view(f32)
+        x = np.arange(W)[None, :]
+        y = np.arange(H)[:, None]
+        if dtype == np.float64:
+            ex, ey = (x + 0.5) / W - float(vx), (y + 0.5) / H - float(vy)
+            assert np.abs(hu - ex).max() < 1e-7 and np.abs(hv - ey).max() < 1e-7
+        else:  # float32 IEEE: (x + 0.5) / W - v, correctly rounded at each step
+            ex = (x.astype(f32) + f32(0.5)) / f32(W) - vx
+            ey = (y.astype(f32) + f32(0.5)) / f32(H) - vy
+            assert np.array_equal(hu, np.broadcast_to(ex, (H, W))) and np.array_equal(hv, np.broadcast_to(ey, (H, W)))
+        base_x = o["inspect"][..., INSPECT["HIST_TC_X"]]
+        assert np.array_equal(base_x, np.floor(hu.astype(np.float64) * W - 0.5).astype(np.int32)) or dtype == np.float32
+
+
+def test_velocity_sample_modes(oracle):
+    rng = np.random.default_rng(7)
+    H, W = 10, 12
+    col, dep, _, hist = frame(H, W, rng)
+    vel = np.zeros((H, W, 4), f32)
+    vel[..., :2] = rng.uniform(-0.02, 0.02, (H, W, 2))
+    vel[..., 3] = 1.0
+    x = (np.arange(W)[None, :] + 0.5) / W
+    y = (np.arange(H)[:, None] + 0.5) / H
+    hu = lambda o: o["inspect"][..., INSPECT["HIST_UV_X"]].copy().view(f32)
+    # mode 1: the reference assigns the last tap of its loop, (+1,+1), clamped (taa.comp:412)
+    o = run(oracle, uniforms(params(mVelocitySampleMode=1)), col, dep, vel, hist)
+    vpp = np.pad(vel, [(0, 1), (0, 1), (0, 0)], mode="edge")[1:, 1:]
+    assert np.abs(hu(o) - (x - vpp[..., 0])).max() < 1e-7
+    # mode 2: velocity of the closest-depth tap
+    o = run(oracle, uniforms(params(mVelocitySampleMode=2)), col, dep, vel, hist)
+    k = o["inspect"][..., INSPECT["CLOSEST_TAP"]]
+    vpad = np.pad(vel, [(1, 1), (1, 1), (0, 0)], mode="edge")
+    yy, xx = np.mgrid[0:H, 0:W]
+    vsel = vpad[yy + k // 3, xx + k % 3]
+    assert np.abs(hu(o) - (x - vsel[..., 0])).max() < 1e-7
+    # movers only: .w < 0.5 means no velocity -> matrix reprojection with identity = static
+    vel[..., 3] = 0.0
+    o = run(oracle, uniforms(params(mUseVelocityVectors=1)), col, dep, vel, hist)
+    assert np.abs(hu(o) - x).max() < 1e-7
+
+
+def test_matrix_reprojection_translation(oracle):
+    rng = np.random.default_rng(8)
+    H, W = 8, 10
+    col, dep, vel, hist = frame(H, W, rng)
+    U = uniforms(params(mUseVelocityVectors=0))
+    U.mHistoryViewProjMatrix[12] = 0.1   # clip-space translation x (column 3, row 0)
+    U.mHistoryViewProjMatrix[13] = -0.06
+    o = run(oracle, U, col, dep, vel, hist)
+    hu = o["inspect"][..., INSPECT["HIST_UV_X"]].copy().view(f32)
+    hv = o["inspect"][..., INSPECT["HIST_UV_Y"]].copy().view(f32)
+    x = (np.arange(W)[None, :] + 0.5) / W
+    y = (np.arange(H)[:, None] + 0.5) / H
+    assert np.abs(hu - (x + 0.05)).max() < 1e-7 and np.abs(hv - (y - 0.03)).max() < 1e-7
+
+
+# ---------------------------------------------------------------------------------------------
+# history filters (taa.comp:441-549)
+
+def keys_weight(d):
+    d = np.abs(d)
+    return np.where(d <= 1, 1.5 * d ** 3 - 2.5 * d ** 2 + 1, np.where(d < 2, -0.5 * d ** 3 + 2.5 * d ** 2 - 4 * d + 2, 0.0))
+
+
+def filtered_history(oracle, hist, vel_px, mode, dtype=np.float64, cr16=True):
+    """Output history with alpha = 0 and no rectification = the filtered history at uv - v."""
+    H, W = hist.shape[:2]
+    rng = np.random.default_rng(0)
+    col, dep, vel, _ = frame(H, W, rng, vel=(vel_px[0] / W, vel_px[1] / H))
+    o = run(oracle, uniforms(params(mInterpolationMode=mode)), col, dep, vel, hist, dtype=dtype, cr16=cr16)
+    return o["history"], vel
+
+
+def test_catmull_rom_is_the_keys_cubic(oracle):
+    rng = np.random.default_rng(9)
+    H, W = 16, 20
+    hist = np.concatenate([rng.uniform(0, 1, (H, W, 3)), rng.uniform(0, 1, (H, W, 1))], -1)
+    got, vel = filtered_history(oracle, hist, (0.37, -0.81), 2)
+    # textbook separable Keys (a = -1/2) convolution with edge clamping, in numpy
+    for y in range(H):
+        for x in range(W):
+            px = ((x + 0.5) / W - float(vel[y, x, 0])) * W - 0.5  # float(): keep numpy out of float32
+            py = ((y + 0.5) / H - float(vel[y, x, 1])) * H - 0.5
+            bx, by = int(np.floor(px)), int(np.floor(py))
+            acc = np.zeros(3)
+            for j in range(by - 1, by + 3):
+                for i in range(bx - 1, bx + 3):
+                    acc += keys_weight(px - i) * keys_weight(py - j) * hist[min(max(j, 0), H - 1), min(max(i, 0), W - 1), :3]
+            assert np.allclose(got[y, x, :3], acc, rtol=0, atol=1e-12), (x, y)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_history_filters_reproduce_polynomials(oracle, mode):
+    """Catmull-Rom reproduces quadratics, the B-spline and bilinear filters linear ramps."""
+    H, W = 24, 24
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
+    lin = 0.3 + 0.02 * xx - 0.015 * yy
+    quad = 0.2 + 0.01 * xx + 0.003 * xx * xx - 0.002 * yy * yy + 0.001 * xx * yy
+    fields = [lin, quad] if mode == 2 else [lin]
+    for fld in fields:
+        hist = np.stack([fld, 2 * fld, fld + 1, np.zeros_like(fld)], -1)
+        got, vel = filtered_history(oracle, hist, (0.43, 0.71), mode)
+        inner = (slice(4, -4), slice(4, -4))
+        # sample position in texel units: x - v (texel centres at integer coordinates), with the
+        # velocity as stored (float32 UV units)
+        sx, sy = xx - vel[..., 0].astype(np.float64) * W, yy - vel[..., 1].astype(np.float64) * H
+        expect = (0.3 + 0.02 * sx - 0.015 * sy) if fld is lin else \
+            (0.2 + 0.01 * sx + 0.003 * sx * sx - 0.002 * sy * sy + 0.001 * sx * sy)
+        assert np.abs(got[..., 0][inner] - expect[inner]).max() < 1e-9
+
+
+def test_catmull_rom_9tap_equals_16tap(oracle):
+    rng = np.random.default_rng(10)
+    H, W = 21, 17
+    hist = np.concatenate([rng.uniform(0, 3, (H, W, 3)), rng.uniform(0, 1, (H, W, 1))], -1)
+    for v in [(0.3, 0.6), (-2.7, 1.2), (4.4, -3.9), (0.0, 0.0)]:  # includes footprints over the edges
+        a, _ = filtered_history(oracle, hist, v, 2, cr16=True)
+        b, _ = filtered_history(oracle, hist, v, 2, cr16=False)
+        assert np.abs(a - b).max() < 1e-12
+        a32, _ = filtered_history(oracle, hist, v, 2, dtype=np.float32, cr16=True)
+        b32, _ = filtered_history(oracle, hist, v, 2, dtype=np.float32, cr16=False)
+        assert np.abs(a32 - b32).max() < 1e-5
+
+
+def test_integer_shift_is_exact(oracle):
+    """At texel centres (f = 0) Catmull-Rom returns the texel itself, in FP32 too."""
+    rng = np.random.default_rng(11)
+    H, W = 8, 16  # powers of two: uv and the velocity are exact in fp32
+    hist = np.concatenate([rng.uniform(0, 1, (H, W, 3)), np.zeros((H, W, 1))], -1).astype(f32)
+    for dtype in (np.float64, np.float32):
+        got, _ = filtered_history(oracle, hist, (2.0, -1.0), 2, dtype=dtype)
+        src = np.pad(hist, [(1, 1), (2, 2), (0, 0)], mode="edge")
+        shifted = src[2:H + 2, 0:W]  # value at (x - 2, y + 1), clamped
+        assert np.array_equal(got[..., :3].astype(f32), shifted[..., :3])
+
+
+# ---------------------------------------------------------------------------------------------
+# rejection, rectification, blend (taa.comp:787-900)
+
+def test_clip_lands_on_the_box(oracle):
+    rng = np.random.default_rng(12)
+    H, W = 14, 18
+    col, dep, vel, _ = frame(H, W, rng)
+    hist = np.concatenate([rng.uniform(-3, 5, (H, W, 3)), np.zeros((H, W, 1))], -1)
+    o = run(oracle, uniforms(params(mColorClampingOrClipping=2, mInterpolationMode=0)), col, dep, vel, hist)
+    mn, mx = (b.astype(np.float64) for b in box(o))
+    h = o["history"][..., :3]
+    c, e = 0.5 * (mx + mn), 0.5 * (mx - mn) + 1e-7
+    m = np.abs((h - c) / e).max(-1)
+    outside = np.abs((hist[..., :3] - c) / e).max(-1) > 1
+    assert outside.mean() > 0.5
+    assert np.abs(m[outside] - 1.0).max() < 1e-9                      # clipped onto the surface
+    assert np.array_equal(h[~outside], hist[..., :3][~outside])        # inside: untouched
+    # the clipped point lies on the segment from the box centre to the history sample
+    d0, d1 = hist[..., :3] - c, h - c
+    cross = np.cross(d0[outside], d1[outside])
+    assert np.abs(cross).max() < 1e-9
+    o = run(oracle, uniforms(params(mColorClampingOrClipping=1, mInterpolationMode=0)), col, dep, vel, hist)
+    assert np.array_equal(o["history"][..., :3], np.clip(hist[..., :3], mn, mx))
+
+
+def test_blend_closed_forms(oracle):
+    rng = np.random.default_rng(13)
+    H, W = 10, 13
+    col, dep, vel, hist = frame(H, W, rng)
+    # reset: output = current colour, also through the YCoCg round trip
+    for ycocg in (0, 1):
+        o = run(oracle, uniforms(params(mUseYCoCg=ycocg, mAlpha=0.3), reset=1), col, dep, vel, hist)
+        assert np.array_equal(o["screen"][..., :3], col[..., :3].astype(np.float64))
+        assert np.all(o["screen"][..., 3] == 1)
+    # alpha: out = hist*(1-a) + cur*a
+    a = float(f32(0.3))  # mAlpha is a float field
+    o = run(oracle, uniforms(params(mAlpha=a, mInterpolationMode=0)), col, dep, np.zeros_like(vel) + [0, 0, 0, 1], hist)
+    assert np.abs(o["history"][..., :3] - (hist[..., :3] * (1 - a) + col[..., :3].astype(np.float64) * a)).max() < 1e-12
+    # Lottes with history == current: diff = 0 -> alpha = mMinAlpha
+    p = params(mLumaWeightingLottes=1, mUseYCoCg=1, mInterpolationMode=0)
+    o = run(oracle, uniforms(p), col, dep, np.zeros_like(vel) + [0, 0, 0, 1], col.astype(np.float64))
+    blend = o["inspect"][..., INSPECT["BLEND"]].copy().view(f32)
+    assert np.all(blend == f32(p.mMinAlpha))
+    # Lottes with |lc - lh| >= max(lc, lh): alpha = mMaxAlpha
+    o = run(oracle, uniforms(p), col, dep, np.zeros_like(vel) + [0, 0, 0, 1], np.zeros_like(hist))
+    assert np.all(o["inspect"][..., INSPECT["BLEND"]].copy().view(f32) == f32(p.mMaxAlpha))
+
+
+def test_rejections(oracle):
+    rng = np.random.default_rng(14)
+    H, W = 12, 16
+    col, dep, vel, hist = frame(H, W, rng, vel=(0.2, 0.0))  # shift 0.2 of the width: the left part leaves the screen
+    p = params(mRejectOutside=1, mRejectionAlpha=1.0, mAlpha=0.05)
+    o = run(oracle, uniforms(p), col, dep, vel, hist)
+    hu = (np.arange(W) + 0.5) / W - 0.2
+    out = np.broadcast_to(hu < 0, (H, W))
+    flags = o["inspect"][..., INSPECT["FLAGS"]]
+    assert np.array_equal((flags & taastar.FLAG_OUTSIDE) != 0, out)
+    assert np.array_equal(o["history"][out][:, :3], col[out][:, :3].astype(np.float64))  # alpha 1
+    # depth rejection, raw depth (taa.comp:818-822) and the linear-depth extension
+    _, _, vel0, _ = frame(H, W, rng)
+    hdep = rng.uniform(0.1, 1.0, (H, W)).astype(f32)
+    o = run(oracle, uniforms(params(mDepthCulling=1)), col, dep, vel0, hist, hdep=hdep)
+    hd, d = hdep.astype(np.float64), dep.astype(np.float64)
+    rej_raw = np.abs(hd - d) > 0.1 * (1.0 - hd)
+    assert np.array_equal((o["inspect"][..., INSPECT["FLAGS"]] & taastar.FLAG_DEPTH_REJECT) != 0, rej_raw)
+    n, fr = 0.1, 100.0
+    lin = lambda z: n * fr / (fr + z * (n - fr))
+    o = run(oracle, uniforms(params(mDepthCulling=1), near=n, far=fr, pad1=0.05), col, dep, vel0, hist, hdep=hdep)
+    rej_lin = np.abs(lin(hd) - lin(d)) > 0.05
+    assert np.array_equal((o["inspect"][..., INSPECT["FLAGS"]] & taastar.FLAG_DEPTH_REJECT) != 0, rej_lin)
+
+
+def test_passthrough_bypass_and_karis(oracle):
+    rng = np.random.default_rng(15)
+    H, W = 9, 11
+    col, dep, vel, hist = frame(H, W, rng)
+    o = run(oracle, uniforms(params(mPassThrough=1)), col, dep, vel, hist)
+    assert np.array_equal(o["screen"][..., :3], col[..., :3].astype(np.float64))
+    assert np.array_equal(o["history"][..., :3], hist[..., :3]) and np.all(o["history"][..., 3] == 1)
+    U = uniforms(params())
+    U.mBypassHistoryUpdate = 1
+    o = run(oracle, U, col, dep, vel, hist)
+    assert np.array_equal(o["screen"][..., :3], hist[..., :3]) and np.array_equal(o["history"][..., :3], hist[..., :3])
+    # Karis: history holds tone-mapped colour c/(1+max c); the screen un-maps it back
+    o = run(oracle, uniforms(params(mToneMapLumaKaris=1), reset=1), col, dep, vel, hist)
+    c = col[..., :3].astype(np.float64)
+    assert np.abs(o["history"][..., :3] - c / (1 + c.max(-1, keepdims=True))).max() < 1e-15
+    assert np.abs(o["screen"][..., :3] - c).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------------------------
+# CAS (ffx_cas.h:424-537)
+
+def u2f(u):
+    return np.array(u, np.uint32).view(f32)
+
+
+def test_cas_integer_approximations_approximate_the_right_functions():
+    # APrxLoRcpF1 / APrxLoSqrtF1 / APrxMedRcpF1 (ffx_a.h:1455-1457) on fp32 bit patterns
+    x = np.geomspace(1e-3, 1e3, 2001).astype(f32)
+    b = x.view(np.uint32)
+    lo_rcp = (np.uint32(0x7EF07EBB) - b).view(f32)
+    lo_sqrt = ((b >> 1) + np.uint32(0x1FBC4639)).view(f32)
+    m = (np.uint32(0x7EF19FFF) - b).view(f32)
+    med_rcp = m * (-m * x + f32(2))
+    assert np.abs(lo_rcp * x - 1).max() < 0.07
+    assert np.abs(lo_sqrt / np.sqrt(x) - 1).max() < 0.045
+    assert np.abs(med_rcp * x - 1).max() < 4e-3
+
+
+def test_cas_flat_and_range(oracle):
+    c0, c1 = oracle.cas_setup(0.5, 64, 48)
+    img = np.zeros((48, 64, 4), np.float64)
+    for v in (0.02, 0.3, 0.77):
+        img[...] = v
+        out = oracle.cas(img, c1[0], dtype=np.float64)
+        assert np.abs(out[..., :3] - v).max() < 3e-3 * v  # flat input: only the rcp approximation error
+    rng = np.random.default_rng(16)
+    hdr = rng.lognormal(-1, 2, (48, 64, 4))
+    out = oracle.cas(hdr, c1[0], dtype=np.float64)
+    assert out[..., :3].min() >= 0 and out[..., :3].max() <= 1 and np.all(out[..., 3] == 1)
+    # amp = 0 when the local max exceeds 1: sharpness has no effect, output = sat(e)
+    img = np.full((48, 64, 4), 1.5)
+    img[::2, ::2] = 0.4
+    outs = [oracle.cas(img, oracle.cas_setup(s, 64, 48)[1][0], dtype=np.float64) for s in (0.0, 1.0)]
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_cas_sharpens_an_edge(oracle):
+    c0, c1 = oracle.cas_setup(1.0, 32, 8)
+    img = np.full((8, 32, 4), 0.2)
+    img[:, 16:] = 0.6
+    out = oracle.cas(img, c1[0], dtype=np.float64)
+    assert out[4, 15, 0] < 0.2 and out[4, 16, 0] > 0.6  # overshoot on both sides of the step
+    assert abs(out[4, 3, 0] - 0.2) < 1e-3 and abs(out[4, 28, 0] - 0.6) < 2e-3
+    a = oracle.cas(img.astype(f32), c1[0], dtype=np.float32)
+    assert np.abs(a - out).max() < 1e-6
+
+
+def test_fp32_transcription_tracks_fp64(oracle):
+    """The two instantiations on a real scene, 4 frames of the full feature set."""
+    from taastar.workloads import (cas_for, decode, encode_color, encode_velocity, scene_for, uniforms_for,
+                                   velocity_as_read, workload)
+    w = workload("c2")
+    W, H = 128, 72
+    sc = scene_for(w, width=W, height=H)
+    k = cas_for(w, W, H)
+    h32 = np.zeros((H, W, 4), f32)
+    h64 = np.zeros((H, W, 4))
+    prev = np.zeros((H, W), f32)
+    for f in range(4):
+        col, dep, vel4, jit = sc.render(f)
+        U = uniforms_for(w, f, jit)
+        args = (decode(encode_color(col, w["fmt"])), dep, velocity_as_read(encode_velocity(vel4, w["vel"])))
+        o32 = oracle.resolve(U, *args, h32, prev, dtype=np.float32, threads=4)
+        o64 = oracle.resolve(U, *args, h64, prev, dtype=np.float64, threads=4)
+        s32 = oracle.cas(o32["screen"], k.const1[0], dtype=np.float32)
+        s64 = oracle.cas(o64["screen"], k.const1[0], dtype=np.float64)
+        d = np.abs(s32 - s64)
+        assert d.max() < 1e-3 and d.mean() < 1e-6
+        h32, h64, prev = o32["history"], o64["history"], dep
