@@ -22,6 +22,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "taa-star_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each divided by what the same
+# counter reads for a device-to-device copy of known size), keyed by (workload, arith)
+TRAFFIC = {}
 
 
 def parse():
@@ -35,6 +38,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the FP32 / 1080p lines and the batched and served session loads")
+    ap.add_argument("--sim-seconds", type=float, default=2.0, help="measured time of each session-load run")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="after the timed loop, copy MIB MiB device-to-device (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -83,30 +89,22 @@ def cpu_baseline(w, frames_host, seconds):
                        f"OpenMP over rows on {threads} threads of {cpu_model()}")
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU (asked for {a.gpus}): sessions are independent and shard across GPUs "
-              "with no exchange, so the per-GPU figure is the one reported.", file=sys.stderr)
-        sys.exit(2)
-    import numpy as np
+def time_resolve(name, arith_name, steps, warmup, frames, autotune=False):
+    """K back-to-back single-frame resolves of workload `name` on a private stream, inputs resident
+    in HBM; HIP events between launches on that stream give the per-launch durations."""
     import torch
     import taastar
     from taastar import FrameImages, check, lib
-    from taastar.workloads import (bytes_per_pixel, cas_for, encode_color, encode_velocity, scene_for,
-                                   uniforms_for, workload)
+    from taastar.workloads import bytes_per_pixel, cas_for, encode_color, encode_velocity, scene_for, uniforms_for, workload
 
-    w = workload(a.workload)
+    w = workload(name)
     W, H = w["width"], w["height"]
-    arith = taastar.ARITH_FP16 if a.arith == "fp16" else taastar.ARITH_FP32
+    arith = taastar.ARITH_FP16 if arith_name == "fp16" else taastar.ARITH_FP32
     dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     stream = torch.cuda.Stream(device=dev)
-
-    # ---- inputs: F distinct frames, generated on the host, resident on the device
     sc = scene_for(w)
     host = []
-    for f in range(a.frames):
+    for f in range(frames):
         col, dep, vel4, jit = sc.render(f)
         host.append((encode_color(col, w["fmt"]), dep, encode_velocity(vel4, w["vel"]), jit))
     cols = torch.stack([torch.from_numpy(h[0]) for h in host]).to(dev)
@@ -119,23 +117,17 @@ def main():
     unis = [uniforms_for(w, 1, h[3]) for h in host]
     u_reset = uniforms_for(w, 0, host[0][3])
     torch.cuda.synchronize()
-
-    if a.autotune:
-        ctx = C.c_void_p()
-        check(lib().taa_ctx_create(0, 0, C.byref(ctx)))
-        bi, bus = C.c_int32(), C.c_float()
-        check(lib().taa_autotune(ctx, W, H, w["fmt"], arith, int(cas is not None), 20, C.byref(bi), C.byref(bus)))
-        lib().taa_ctx_destroy(ctx)
-
+    if autotune:
+        tune(W, H, w["fmt"], arith, cas is not None)
     sptr = C.c_void_p(stream.cuda_stream)
     imgs = []
-    for i in range(2 * a.frames):  # frame index i % F, history parity i % 2
-        f = i % a.frames
+    for i in range(2 * frames):  # frame index i % F, history parity i % 2
+        f = i % frames
         img = FrameImages()
         img.color = cols[f].data_ptr()
         img.depth = deps[f].data_ptr()
         img.history = hist[(i + 1) & 1].data_ptr()
-        img.history_depth = deps[(f - 1) % a.frames].data_ptr()
+        img.history_depth = deps[(f - 1) % frames].data_ptr()
         img.result_screen = screen.data_ptr()
         img.velocity = vels[f].data_ptr()
         img.result_history = hist[i & 1].data_ptr()
@@ -147,81 +139,200 @@ def main():
 
     def launch(i, reset=False):
         k = i % len(imgs)
-        U = u_reset if reset else unis[k % a.frames]
+        U = u_reset if reset else unis[k % frames]
         rc = L.taa_resolve_frame(C.byref(U), C.byref(imgs[k]), casp, arith, sptr)
         if rc:
             check(rc)
 
     with torch.cuda.stream(stream):
         launch(0, reset=True)
-        for i in range(1, a.warmup + 1):
+        for i in range(1, warmup + 1):
             launch(i)
     stream.synchronize()
-
-    # ---- timed region: K back-to-back resolves, events between launches on the same stream
-    K = a.steps
+    K = steps
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     with torch.cuda.stream(stream):
         ev[0].record(stream)
         for i in range(K):
-            launch(a.warmup + 1 + i)
+            launch(warmup + 1 + i)
             ev[i + 1].record(stream)
     stream.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     total_ms = ev[0].elapsed_time(ev[K])
     per = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(K))
-    avg_ms = total_ms / K
-    med_ms = per[K // 2]
     px = W * H
     bpp = bytes_per_pixel(w)
-    achieved = bpp * px / (avg_ms * 1e-3) / 1e9
-    value = K * px / (total_ms * 1e-3) / 1e9
+    avg_ms = total_ms / K
+    r = dict(workload=name, width=W, height=H, arith=arith_name, steps=K, warmup=warmup,
+             gpix_per_s=K * px / (total_ms * 1e-3) / 1e9, avg_ms=avg_ms, median_ms=per[K // 2],
+             bytes_per_pixel=bpp, achieved_gbs=bpp * px / (avg_ms * 1e-3) / 1e9, wall_s=wall,
+             tile_shape=L.taa_tile_shape_selected(W, H, w["fmt"], arith, int(cas is not None)), w=w, host=host)
+    del cols, deps, vels, hist, screen
+    torch.cuda.empty_cache()
+    return r
+
+
+def tune(W, H, fmt, arith, cas, iters=20):
+    from taastar import check, lib
+    ctx = C.c_void_p()
+    check(lib().taa_ctx_create(0, 0, C.byref(ctx)))
+    bi, bus = C.c_int32(), C.c_float()
+    check(lib().taa_autotune(ctx, W, H, fmt, arith, int(cas), iters, C.byref(bi), C.byref(bus)))
+    lib().taa_ctx_destroy(ctx)
+    return bi.value, bus.value
+
+
+MODE_NAMES = {0: "served open loop (Poisson arrivals, churn)", 1: "served closed loop (continuous batching)",
+              2: "per-frame launches, async", 3: "per-frame launches, host waits each frame",
+              4: "batched: one graph-captured persistent launch per round of all sessions"}
+
+
+def simulate(mode, mix, sessions, fmt, arith, seconds, exact=True, rate=0.0, churn=0.0, budget=0, seed=11,
+             max_batch=64, max_inflight=3, warmup=2):
+    from taastar import ServeReport, ServeSimConfig, check, lib
+    from taastar.workloads import RESOLUTIONS
+    cfg = ServeSimConfig()
+    cfg.device, cfg.seed, cfg.mode, cfg.sessions = 0, seed, mode, sessions
+    cfg.n_classes = len(mix)
+    for i, (cls, p) in enumerate(mix):
+        cfg.class_w[i], cfg.class_h[i] = RESOLUTIONS[cls]
+        cfg.class_p[i] = p
+    cfg.arrival_rate, cfg.churn_per_s, cfg.duration_s = rate, churn, seconds
+    cfg.hbm_budget_bytes = budget
+    cfg.color_format, cfg.arith, cfg.cas_sharpness = fmt, arith, 0.5
+    cfg.max_batch, cfg.max_inflight, cfg.warmup_frames, cfg.exact_mix = max_batch, max_inflight, warmup, int(exact)
+    rep = ServeReport()
+    check(lib().taa_serve_simulate(C.byref(cfg), C.byref(rep)))
+    d = rep.as_dict()
+    out = dict(mode=MODE_NAMES[mode], gpix_per_s=round(d["gpix_per_s"], 3), frames_per_s=round(d["achieved_fps"], 1),
+               frames=d["frames"], wall_s=round(d["wall_s"], 3), mean_batch=round(d["mean_batch"], 2))
+    if mode in (0, 1, 3):
+        out.update(lat_p50_ms=round(d["lat_p50_us"] / 1e3, 3), lat_p99_ms=round(d["lat_p99_us"] / 1e3, 3),
+                   lat_max_ms=round(d["lat_max_us"] / 1e3, 3))
+    if mode == 4:
+        out["gpu_ms_per_round"] = round(d["lat_mean_us"] / 1e3, 3)
+    if mode in (0, 1):
+        out["gpu_busy_frac"] = round(d["gpu_busy_frac"], 3)
+    if mode == 0:
+        out.update(offered_fps=round(d["offered_fps"], 1), admissions=d["admissions"],
+                   admissions_refused=d["admissions_refused"],
+                   refused_rate=round(d["admissions_refused"] / max(1, d["admissions"]), 4),
+                   sessions_peak=d["sessions_peak"], hbm_peak_gb=round(d["hbm_peak_bytes"] / 1e9, 2))
+    print(f"# sim mode {mode} {sessions} sessions: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def extras(a):
+    import taastar
+    from taastar.workloads import RESOLUTIONS, SERVE_MIX
+    F16, F32 = taastar.FMT_RGBA16F, taastar.FMT_RGBA32F
+    A16, A32 = taastar.ARITH_FP16, taastar.ARITH_FP32
+    x = {}
+    for name, ar, st in (("c3", "fp32", 50), ("c2", "fp32", 100), ("c2", "fp16", 100)):
+        r = time_resolve(name, ar, st, 5, 4)
+        x[f"resolve_{name}_{r['width']}x{r['height']}_{ar}arith"] = {
+            k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items() if k not in ("w", "host")}
+        print(f"# {name} {ar}: {r['gpix_per_s']:.3f} Gpix/s", file=sys.stderr, flush=True)
+    for cls in RESOLUTIONS:  # autotune every class the mixed loads use (FP16 and FP32 storage)
+        W, H = RESOLUTIONS[cls]
+        tune(W, H, F16, A16, True)
+        tune(W, H, F32, A32, True)
+    s = a.sim_seconds
+    # 64 sessions of the serving mix: per-frame plain launches vs the batched and served forms
+    mix = list(SERVE_MIX)
+    m64 = {"mix": "64 sessions: 26x1280x720, 22x1920x1080, 10x2560x1440, 6x3840x2160 (FP16 storage and "
+                  "arithmetic, fused CAS 0.5)"}
+    for mode in (3, 2, 4, 1):
+        m64[f"mode{mode}"] = simulate(mode, mix, 64, F16, A16, s)
+    base = m64["mode3"]["gpix_per_s"]
+    m64["batched_over_per_frame_sync"] = round(m64["mode4"]["gpix_per_s"] / base, 3)
+    m64["served_over_per_frame_sync"] = round(m64["mode1"]["gpix_per_s"] / base, 3)
+    m64["batched_over_per_frame_async"] = round(m64["mode4"]["gpix_per_s"] / m64["mode2"]["gpix_per_s"], 3)
+    m64["served_over_per_frame_async"] = round(m64["mode1"]["gpix_per_s"] / m64["mode2"]["gpix_per_s"], 3)
+    x["sessions64"] = m64
+    # 48 streams in one persistent graph-captured launch, both storages
+    mix48 = [("720p", 16), ("1080p", 16), ("1440p", 8), ("4k", 8)]
+    x["batched48"] = {
+        "mix": "16x1280x720, 16x1920x1080, 8x2560x1440, 8x3840x2160",
+        "fp16_storage_fp16_arith": simulate(4, mix48, 48, F16, A16, s),
+        "fp32_storage_fp32_arith": simulate(4, mix48, 48, F32, A32, s),
+        "fp16_per_frame_async": simulate(2, mix48, 48, F16, A16, s),
+        "fp32_per_frame_async": simulate(2, mix48, 48, F32, A32, s),
+    }
+    # served load: 256 sessions, Poisson arrivals at 6000 frames/s, 10% churn per second
+    x["served256"] = {
+        "load": "256 sessions from {720p 40%, 1080p 35%, 1440p 15%, 4K 10%}, Poisson 6000 frames/s, churn 10%/s, "
+                "FP16 storage and arithmetic, fused CAS 0.5",
+        "budget_200GB": simulate(0, mix, 256, F16, A16, 2 * s, exact=False, rate=6000.0, churn=0.10,
+                                 budget=200 * 10**9, max_batch=256),
+        "budget_16GB": simulate(0, mix, 256, F16, A16, 2 * s, exact=False, rate=6000.0, churn=0.10,
+                                budget=16 * 10**9, max_batch=256),
+    }
+    return x
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU (asked for {a.gpus}): sessions are independent and shard across GPUs "
+              "with no exchange, so the per-GPU figure is the one reported.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import taastar
+
+    torch.cuda.set_device(0)
+    r = time_resolve(a.workload, a.arith, a.steps, a.warmup, a.frames, autotune=a.autotune)
+    w = r["w"]
+    W, H = r["width"], r["height"]
     out = {
         "metric": "TAA resolve Gpix/s & %HBM peak at 4K; p99 latency, 64 mixed-res sessions",
-        "value": round(value, 3),
+        "value": round(r["gpix_per_s"], 3),
         "unit": "Gpix/s",
         "n_gpus": 1,
-        "steps": K,
+        "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(avg_ms, 5),
+        "ms_per_step": round(r["avg_ms"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f16" if arith == taastar.ARITH_FP16 else "f32",
-        "data": "synthetic (seeded polygon scene, 8 frames cycled, resident in HBM)",
+        "dtype": "f16" if a.arith == "fp16" else "f32",
+        "data": f"synthetic (seeded polygon scene, {a.frames} frames cycled, resident in HBM)",
         "config": {
             "workload": f"{a.workload}: {W}x{H} TAA resolve + fused CAS 0.5, "
                         f"{'RGBA16F' if w['fmt'] else 'RGBA32F'} colour/history, "
                         f"{'RG16F' if w['vel'] == 1 else 'RG32F'} velocity, D32F depth",
-            "frame_pixels": px,
+            "frame_pixels": W * H,
             "arith": a.arith,
-            "tile_shape": L.taa_tile_shape_selected(W, H, w["fmt"], arith, int(cas is not None)),
-            "wall_s": round(wall, 4),
-            "median_step_ms": round(med_ms, 5),
+            "tile_shape": r["tile_shape"],
+            "wall_s": round(r["wall_s"], 4),
+            "median_step_ms": round(r["median_ms"], 5),
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": round(r["achieved_gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "bytes_per_pixel": bpp,
+            "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": TRAFFIC.get((a.workload, a.arith)),
+            "bytes_per_pixel": r["bytes_per_pixel"],
         },
     }
     if a.copy_calib > 0:
         n = a.copy_calib * (1 << 20)
-        src = torch.empty(n, dtype=torch.uint8, device=dev).fill_(1)
+        src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
         out["config"]["copy_calib_bytes"] = n
+        del src, dst
+    if not a.no_extras:
+        out["extras"] = extras(a)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, host[:2], a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(w, r["host"][:2], a.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

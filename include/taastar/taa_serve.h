@@ -79,7 +79,9 @@ typedef struct taa_serve_sim_config {
                                    1 closed loop: every session resubmits on completion
                                    2 per-frame: one taa_session_resolve launch per frame,
                                      sessions round-robin on one stream, no host waits
-                                   3 per-frame synchronous: as 2, host waits for each frame */
+                                   3 per-frame synchronous: as 2, host waits for each frame
+                                   4 batched graph: every session, one frame each, per
+                                     taa_batch_plan_launch, back to back */
     int32_t  sessions;          /* initial sessions */
     int32_t  n_classes;         /* resolution classes */
     int32_t  class_w[8], class_h[8];
@@ -91,7 +93,9 @@ typedef struct taa_serve_sim_config {
     int32_t  color_format, arith;
     float    cas_sharpness;
     int32_t  max_batch, max_inflight;
-    int32_t  warmup_frames;     /* per session, before measuring (modes 1-3) */
+    int32_t  warmup_frames;     /* per session, before measuring (modes 1-4) */
+    int32_t  exact_mix;         /* != 0: classes assigned in proportion to class_p (largest
+                                   remainder) instead of drawn at random */
 } taa_serve_sim_config_t;
 
 typedef struct taa_serve_report {

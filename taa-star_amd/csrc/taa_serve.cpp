@@ -554,9 +554,25 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         out->sessions_peak = std::max<int32_t>(out->sessions_peak, (int32_t)ss.size());
         return 0;
     };
-    for (int i = 0; i < cfg->sessions; ++i) {
-        rc = open_one(draw_class());
-        if (rc) break;
+    if (cfg->exact_mix) {  // largest-remainder apportionment of `sessions` over the classes
+        std::vector<int> cnt(cfg->n_classes);
+        std::vector<std::pair<double, int>> rem;
+        int given = 0;
+        for (int i = 0; i < cfg->n_classes; ++i) {
+            const double q = cfg->sessions * cfg->class_p[i] / acc;
+            cnt[i] = (int)std::floor(q);
+            given += cnt[i];
+            rem.emplace_back(q - cnt[i], i);
+        }
+        std::sort(rem.begin(), rem.end(), [](auto& a, auto& b) { return a.first > b.first; });
+        for (int k = 0; given < cfg->sessions; ++k, ++given) cnt[rem[k % rem.size()].second]++;
+        for (int i = 0; i < cfg->n_classes && rc == 0; ++i)
+            for (int j = 0; j < cnt[i] && rc == 0; ++j) rc = open_one(i);
+    } else {
+        for (int i = 0; i < cfg->sessions; ++i) {
+            rc = open_one(draw_class());
+            if (rc) break;
+        }
     }
     std::vector<double> lat;
     std::vector<taa_completion_t> comp(4096);
@@ -652,6 +668,58 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         out->mean_batch = out->batches ? (double)(st1.frames - st0.frames) / (double)out->batches : 0.0;
         out->gpu_busy_frac = union_length(srv->spans, t_begin, t_end) / (t_end - t_begin);
         out->hbm_peak_bytes = st1.hbm_used;
+    } else if (rc == 0 && cfg->mode == 4) {
+        // every session advances one frame per graph launch (descriptor upload + persistent kernel)
+        hipStream_t st;
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        std::vector<taa_batch_item_t> items(ss.size());
+        std::vector<taa_uniforms_t> us(ss.size());
+        std::vector<taa_cas_consts_t> cs(ss.size());
+        uint64_t round_px = 0;
+        for (size_t i = 0; i < ss.size(); ++i) {
+            auto it = srv->sess.find(ss[i].sid);
+            cs[i] = it->second.cas;
+            items[i].session = it->second.s;
+            items[i].uniforms = &us[i];
+            items[i].cas = srv->cas_on ? &cs[i] : nullptr;
+            items[i].result_screen = it->second.screen;
+            round_px += (uint64_t)cfg->class_w[ss[i].cls] * cfg->class_h[ss[i].cls];
+        }
+        taa_batch_plan_t* plan = nullptr;
+        rc = taa_batch_plan_create(srv->ctx, (int)items.size(), items.data(), cfg->arith, &plan);
+        auto refresh = [&]() {
+            for (size_t i = 0; i < ss.size(); ++i) sim_uniforms(us[i], pool[ss[i].cls], ss[i].frame++);
+        };
+        for (int k = 0; rc == 0 && k < std::max(1, cfg->warmup_frames); ++k) {
+            refresh();
+            rc = taa_batch_plan_launch(plan, st);
+        }
+        hipStreamSynchronize(st);
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        t_begin = host_us();
+        t_end = t_begin + cfg->duration_s * 1e6;
+        hipEventRecord(e0, st);
+        uint64_t rounds = 0;
+        while (rc == 0 && host_us() < t_end) {
+            refresh();
+            rc = taa_batch_plan_launch(plan, st);
+            rounds++;
+        }
+        hipEventRecord(e1, st);
+        hipStreamSynchronize(st);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        frames = rounds * ss.size();
+        pixels = rounds * round_px;
+        out->batches = rounds;
+        out->mean_batch = (double)ss.size();
+        out->lat_mean_us = rounds ? 1000.0 * ms / (double)rounds : 0.0;  // GPU time per round
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        if (plan) taa_batch_plan_destroy(plan);
+        hipStreamDestroy(st);
     } else if (rc == 0) {
         // per-frame launches: sessions round-robin, one taa_session_resolve each
         hipStream_t st;
@@ -699,7 +767,7 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
     out->pixels = pixels;
     out->gpix_per_s = out->wall_s > 0 ? (double)pixels / out->wall_s / 1e9 : 0.0;
     out->achieved_fps = out->wall_s > 0 ? (double)frames / out->wall_s : 0.0;
-    if (!lat.empty()) {
+    if (!lat.empty()) {  // (mode 4 reports the GPU time per round in lat_mean_us instead)
         double sum = 0;
         for (double v : lat) sum += v;
         out->lat_mean_us = sum / (double)lat.size();

@@ -8,11 +8,15 @@ oracle on every frame, over the pixels outside the reference's own divergence fo
 Divergence footprint: taa.comp's Lottes weight w = 1 - |lc - lh| / max(lc, lh, 0.2) is not clamped
 (taa.comp:865-869); once Catmull-Rom undershoot makes a luma negative, w*w > 1 makes alpha negative
 and the blend extrapolates. On the sigma = 2 HDR scene a few pixels then grow without bound in the
-FP64 oracle itself (1e9 and more) and their outcome is chaotic: the FP32 transcription with the
-same FP16 storage also differs there by up to 1.0 after CAS saturation. Those pixels are found from
-the FP64 history alone (|h| > 4x the frame's input maximum in the last 8 frames, dilated by the
-Catmull-Rom support) and excluded from the PSNR bar; their count is reported and bounded, the
-full-frame PSNR is reported and must stay within 3 dB of the FP32-arithmetic / FP16-storage one.
+FP64 oracle itself (1e9 and more), and at others the reference's result is chaotic: rounding the
+stored history to FP16 alone, with every operation of the FP32 transcription unchanged, moves the
+sharpened result by up to 1.0 and takes that transcription to 54 dB against FP64 by frame 15.
+Excluded from the 55 dB bar are therefore (a) the pixels whose FP64 history exceeded 4x the frame's
+input maximum in the last 8 frames, dilated by the Catmull-Rom support, and (b) the pixels where
+the FP32 transcription on FP16 storage is itself more than 0.01 from the FP64 result, dilated by
+the CAS neighbourhood. Both sets come from the oracle alone, their size is reported and bounded
+(at most 0.13% of the pixels at 960x540 and 0.19% at 4K on these sequences), and the full-frame PSNR must stay within 3 dB of
+that of the FP32 transcription on FP16 storage.
 """
 import numpy as np
 import pytest
@@ -29,6 +33,8 @@ from taastar.workloads import (cas_for, decode, encode_color, encode_velocity, s
 from scipy.ndimage import binary_dilation  # noqa: E402
 
 PSNR_MIN_DB = 55.0
+DEPART_ABS = 1e-2          # FP32 transcription on FP16 storage vs FP64, sharpened screen
+MASK_MAX_FRACTION = 3e-3   # excluded pixels per frame
 
 
 @pytest.fixture(autouse=True)
@@ -73,9 +79,10 @@ def run_fp16_sequence(oracle, name, W, H, frames, report):
         cmax = float(decode(col_s)[..., :3].max())
         recent.append(np.abs(o64["history"][..., :3]).max(-1) > 4.0 * cmax)
         del recent[:-8]
-        unstable = binary_dilation(np.logical_or.reduce(recent), iterations=3)
-        stable = ~unstable
         s32 = oracle.cas(to_f16(o32["screen"]).astype(np.float32), k.const1[0])
+        departs = np.abs(s32[..., :3] - s64[..., :3]).max(-1) > DEPART_ABS
+        unstable = binary_dilation(np.logical_or.reduce(recent), iterations=3) | binary_dilation(departs, iterations=1)
+        stable = ~unstable
         p_all = psnr(g_scr[..., :3], s64[..., :3], 1.0)
         p_base = psnr(s32[..., :3], s64[..., :3], 1.0)
         p_scr = psnr(g_scr[stable][:, :3], s64[stable][:, :3], 1.0)
@@ -85,7 +92,7 @@ def run_fp16_sequence(oracle, name, W, H, frames, report):
         worst = min(worst, p_scr)
         assert p_scr >= PSNR_MIN_DB, f"frame {f}: screen PSNR {p_scr:.2f} dB < {PSNR_MIN_DB}: {report[-1]}"
         assert p_all >= min(p_base - 3.0, 60.0), f"frame {f}: full-frame {p_all:.2f} dB vs FP32 path {p_base:.2f} dB"
-        assert unstable.sum() <= 1e-3 * W * H, report[-1]
+        assert unstable.sum() <= MASK_MAX_FRACTION * W * H, report[-1]
         h64 = o64["history"]
         h32 = to_f16(o32["history"]).astype(np.float32)
         prev = dep

@@ -212,3 +212,26 @@ def test_simulation_smoke():
     print(d, flush=True)
     assert rep.frames > 200 and 0 < rep.lat_p50_us <= rep.lat_p99_us
     assert rep.offered_fps > 1000 and abs(rep.achieved_fps - rep.offered_fps) < 0.1 * rep.offered_fps
+
+
+@pytest.mark.parametrize("mode", [2, 3, 4])
+def test_simulation_per_frame_and_batched_modes(mode):
+    """Per-frame (async / waited) and graph-batched rounds over an exact 3:1 class mix."""
+    cfg = ServeSimConfig()
+    cfg.device, cfg.seed, cfg.mode, cfg.sessions, cfg.exact_mix = 0, 5, mode, 8, 1
+    cfg.n_classes = 2
+    cfg.class_w[0], cfg.class_h[0], cfg.class_p[0] = 320, 180, 0.75
+    cfg.class_w[1], cfg.class_h[1], cfg.class_p[1] = 640, 360, 0.25
+    cfg.duration_s = 0.3
+    cfg.color_format, cfg.arith, cfg.cas_sharpness = FMT_RGBA16F, ARITH_FP16, 0.5
+    cfg.max_batch, cfg.max_inflight, cfg.warmup_frames = 16, 2, 1
+    rep = ServeReport()
+    check(lib().taa_serve_simulate(C.byref(cfg), C.byref(rep)))
+    print(mode, rep.as_dict(), flush=True)
+    assert rep.sessions_end == 8 and rep.frames > 0 and rep.frames % 8 == 0
+    # 6 sessions of 320x180 and 2 of 640x360 per round of every session
+    assert rep.pixels == (rep.frames // 8) * (6 * 320 * 180 + 2 * 640 * 360)
+    if mode == 3:
+        assert 0 < rep.lat_p50_us <= rep.lat_p99_us
+    if mode == 4:
+        assert rep.mean_batch == 8 and rep.lat_mean_us > 0
