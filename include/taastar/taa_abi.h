@@ -11,7 +11,8 @@
  *   - Every entry point returns an int status (TAA_OK or a negative TAA_ERR_*); the message of
  *     the last failure on the calling thread is available from taa_last_error().
  *   - No entry point throws. Only the context/session/plan constructors allocate device memory.
- *   - Images are row-major, tightly packed rows of `pitch` pixels (pitch >= width).
+ *   - Images are row-major, tightly packed rows of `pitch` pixels (pitch >= width). Every
+ *     image must fit in 4 GiB: pitch * height * 16 bytes (64 for the inspect image) < 2^32.
  *
  * The struct layouts below are byte-identical to the reference's host structs (std140 in the
  * shaders); offsets are pinned with static asserts in the C++ translation units.

@@ -82,12 +82,20 @@ int selected_shape(int W, int H, int fmt, int arith, bool cas) {
 taastar::TileShapeDef shape_def(int idx, bool cas) { return cas ? taastar::kShapesCas[idx] : taastar::kShapesPlain[idx]; }
 
 // Fill the kernel-side frame descriptor; validates the images.
+// The kernels address every image with 32-bit byte offsets: the largest image (16 bytes per
+// texel; 64 for the inspect image) must stay below 4 GiB.
+bool fits_offsets(int pitch, int height, int bytes_per_texel) {
+    return (uint64_t)pitch * (uint64_t)height * (uint64_t)bytes_per_texel <= 0xFFFFFFFFull;
+}
+
 int make_desc(const taa_frame_images_t* img, const taa_cas_consts_t* cas, taastar::FrameDesc& fd) {
     if (!img) return fail(TAA_ERR_INVALID_ARG, "images is NULL");
     if (img->width <= 0 || img->height <= 0 || img->width > 32768 || img->height > 32768)
         return fail(TAA_ERR_INVALID_ARG, "bad frame size %dx%d", img->width, img->height);
     const int pitch = img->pitch == 0 ? img->width : img->pitch;
     if (pitch < img->width) return fail(TAA_ERR_INVALID_ARG, "pitch %d < width %d", pitch, img->width);
+    if (!fits_offsets(pitch, img->height, 16) || (img->inspect && !fits_offsets(pitch, img->height, 4 * TAA_INSPECT_WORDS)))
+        return fail(TAA_ERR_INVALID_ARG, "frame of %d x %d texels: an image would exceed 4 GiB", pitch, img->height);
     if (!valid_color_fmt(img->color_format)) return fail(TAA_ERR_INVALID_ARG, "bad color_format %d", img->color_format);
     if (!valid_vel_fmt(img->velocity_format)) return fail(TAA_ERR_INVALID_ARG, "bad velocity_format %d", img->velocity_format);
     if (img->reserved != 0) return fail(TAA_ERR_INVALID_ARG, "reserved must be 0");
@@ -205,7 +213,7 @@ struct taa_session {
 
 namespace {
 
-size_t staging_bytes() { return align256(sizeof(taastar::BatchEntry) * taa_ctx::kCap) + sizeof(taa_uniforms_t) * taa_ctx::kCap; }
+size_t staging_bytes() { return sizeof(taastar::BatchEntry) * taa_ctx::kCap; }
 
 // Frame descriptor for the session's next frame (frame k reads depth[k&1], history[(k+1)&1] and
 // writes history[k&1]; the previous depth is depth[(k+1)&1]).
@@ -282,6 +290,7 @@ int32_t taa_cas_apply(const void* src, void* dst, int32_t width, int32_t height,
     if (!valid_color_fmt(format)) return fail(TAA_ERR_INVALID_ARG, "bad format");
     if (pitch == 0) pitch = width;
     if (pitch < width) return fail(TAA_ERR_INVALID_ARG, "pitch < width");
+    if (!fits_offsets(pitch, height, 16)) return fail(TAA_ERR_INVALID_ARG, "image exceeds 4 GiB");
     int e = taastar::launch_cas(src, dst, width, height, pitch, format, cas->const1[0], stream);
     if (e) return hip_fail((hipError_t)e, "cas launch");
     return ok();
@@ -350,6 +359,7 @@ int32_t taa_session_open(taa_ctx_t* c, int32_t w, int32_t h, int32_t cf, int32_t
     *out = nullptr;
     if (w <= 0 || h <= 0 || w > 32768 || h > 32768) return fail(TAA_ERR_INVALID_ARG, "bad size %dx%d", w, h);
     if (!valid_color_fmt(cf) || !valid_vel_fmt(vf)) return fail(TAA_ERR_INVALID_ARG, "bad format");
+    if (!fits_offsets(w, h, 16)) return fail(TAA_ERR_INVALID_ARG, "session of %dx%d: an image would exceed 4 GiB", w, h);
     const uint64_t bytes = taa_session_bytes(w, h, cf, vf);
     // admission: reserve the bytes against the budget first
     uint64_t cur = c->used.load();
@@ -446,9 +456,8 @@ int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* u, const taa
 static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int arith, void* host, void* dev, int& shape,
                        bool& cas, bool& s16, int& total_tiles, bool& fast) {
     if (n <= 0 || n > taa_ctx::kCap) return fail(TAA_ERR_CAPACITY, "batch of %d items (capacity %d)", n, taa_ctx::kCap);
+    (void)dev;
     auto* ents = static_cast<taastar::BatchEntry*>(host);
-    auto* unis = reinterpret_cast<taa_uniforms_t*>(static_cast<char*>(host) + align256(sizeof(taastar::BatchEntry) * taa_ctx::kCap));
-    auto* dunis = reinterpret_cast<taa_uniforms_t*>(static_cast<char*>(dev) + align256(sizeof(taastar::BatchEntry) * taa_ctx::kCap));
     cas = items[0].cas != nullptr;
     s16 = items[0].session && items[0].session->cf == TAA_FMT_RGBA16F;
     // one tile shape per launch: the one tuned for the largest frame class in the batch
@@ -479,15 +488,14 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
         session_desc(s, items[i].result_screen, items[i].cas, e.fd);
         e.fd.tiles_x = taastar::tiles_for(s->W, sd.tw);
         e.fd.tiles_y = taastar::tiles_for(s->H, sd.th);
-        unis[i] = *items[i].uniforms;
-        if (s->force_reset) unis[i].mResetHistory = 1;
-        e.U = dunis + i;
+        e.U = *items[i].uniforms;
+        if (s->force_reset) e.U.mResetHistory = 1;
         e.tile_begin = tiles;
         tiles += e.fd.tiles_x * e.fd.tiles_y;
     }
     total_tiles = tiles;
     fast = arith == TAA_ARITH_FP16;
-    for (int i = 0; i < n && fast; ++i) fast = taastar::fast_path_supported(ents[i].fd, unis[i], 0);
+    for (int i = 0; i < n && fast; ++i) fast = taastar::fast_path_supported(ents[i].fd, ents[i].U, 0);
     return TAA_OK;
 }
 
@@ -505,7 +513,7 @@ int32_t taa_batch_submit(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items,
     int rc = build_batch(c, n, items, arith, slot.host, slot.dev, shape, cas, s16, total, fast);
     if (rc) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipError_t e = hipMemcpyAsync(slot.dev, slot.host, staging_bytes(), hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemcpyAsync(slot.dev, slot.host, sizeof(taastar::BatchEntry) * (size_t)n, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "descriptor upload");
     const auto* d_ents = static_cast<const taastar::BatchEntry*>(slot.dev);
     int le = fast ? taastar::launch_batch_fast(d_ents, n, total, shape, cas, s16,
@@ -564,7 +572,7 @@ int plan_capture(taa_batch_plan* p, taa_batch_plan::Buf& b, bool fast, int shape
     hipGraph_t g = nullptr;
     hipError_t e;
     if ((e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
-    hipMemcpyAsync(b.dev, b.host, staging_bytes(), hipMemcpyHostToDevice, p->cap);
+    hipMemcpyAsync(b.dev, b.host, sizeof(taastar::BatchEntry) * (size_t)p->n, hipMemcpyHostToDevice, p->cap);
     const auto* d_ents = static_cast<const taastar::BatchEntry*>(b.dev);
     if (fast) taastar::launch_batch_fast(d_ents, p->n, total, shape, cas, s16, grid, p->cap);
     else taastar::launch_batch(d_ents, p->n, total, shape, cas, s16, p->arith, grid, p->cap);

@@ -16,43 +16,78 @@ namespace {
 __device__ __forceinline__ float h2f(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
 // fp32 -> fp16 storage conversion: round to nearest even, saturating to +-65504 (NaN stays NaN)
 // so that an overflow never becomes an Inf that the history filter would spread as NaN.
-__device__ __forceinline__ float sat_f16_range(float f) { return f == f ? fminf(fmaxf(f, -65504.0f), 65504.0f) : f; }
+__device__ __forceinline__ float sat_f16_range(float f) { return f == f ? __builtin_amdgcn_fmed3f(f, -65504.0f, 65504.0f) : f; }
 __device__ __forceinline__ uint32_t f2h(float f) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)sat_f16_range(f)); }
 __device__ __forceinline__ float round_f16(float f) { return (float)(_Float16)sat_f16_range(f); }
+// two channels into one dword (low half = a), same rounding and saturation as f2h
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_f2h(float a, float b) {
+    const f32x2_t v = {sat_f16_range(a), sat_f16_range(b)};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, f16x2_t));
+}
 
+// Global memory through a uniform base pointer and a 32-bit byte offset: every image is smaller
+// than 4 GiB (the host refuses larger ones), so texel offsets are computed in 32 bits and the
+// accesses use the scalar-base form of global loads and stores. `imm` is added to the pointer
+// (not to the 32-bit offset) so that it folds into the instruction's immediate offset.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define TAA_GLOBAL __attribute__((address_space(1)))
+#define TAA_CONSTANT __attribute__((address_space(4)))
+#else
+#define TAA_GLOBAL  // host pass: these functions are parsed but never compiled for the CPU
+#define TAA_CONSTANT
+#endif
+template <class T>
+__device__ __forceinline__ T gload(const void* base, uint32_t off, int imm = 0) {
+    return *(const TAA_GLOBAL T*)((const TAA_GLOBAL char*)base + off + imm);
+}
+template <class T>
+__device__ __forceinline__ void gstore(void* base, uint32_t off, T v, int imm = 0) {
+    *(TAA_GLOBAL T*)((TAA_GLOBAL char*)base + off + imm) = v;
+}
+
+// texel i of an RGBA image (element index, not bytes)
 template <bool S16>
-__device__ __forceinline__ float4 ld_rgba(const void* p, size_t i) {
+__device__ __forceinline__ float4 ld_rgba(const void* p, uint32_t i) {
     if constexpr (S16) {
-        uint2 v = static_cast<const uint2*>(p)[i];
+        uint2 v = gload<uint2>(p, i * 8u);
         return make_float4(h2f(v.x & 0xffffu), h2f(v.x >> 16), h2f(v.y & 0xffffu), h2f(v.y >> 16));
     } else {
-        return static_cast<const float4*>(p)[i];
+        return gload<float4>(p, i * 16u);
     }
 }
 template <bool S16>
-__device__ __forceinline__ void st_rgba(void* p, size_t i, float4 v) {
+__device__ __forceinline__ void st_rgba(void* p, uint32_t i, float4 v) {
     if constexpr (S16) {
-        uint2 o;
-        o.x = f2h(v.x) | (f2h(v.y) << 16);
-        o.y = f2h(v.z) | (f2h(v.w) << 16);
-        static_cast<uint2*>(p)[i] = o;
+        gstore<uint2>(p, i * 8u, make_uint2(pack_f2h(v.x, v.y), pack_f2h(v.z, v.w)));
     } else {
-        static_cast<float4*>(p)[i] = v;
+        gstore<float4>(p, i * 16u, v);
     }
 }
-__device__ __forceinline__ void st_rgba16(void* p, size_t i, float4 v) { st_rgba<true>(p, i, v); }
+__device__ __forceinline__ void st_rgba16(void* p, uint32_t i, float4 v) { st_rgba<true>(p, i, v); }
+__device__ __forceinline__ float ld_f32(const void* p, uint32_t i) { return gload<float>(p, i * 4u); }
 
 // Velocity texel; RG formats read back as (r, g, 0, 1).
-__device__ __forceinline__ float4 ld_vel(const void* p, int fmt, size_t i) {
+__device__ __forceinline__ float4 ld_vel(const void* p, int fmt, uint32_t i) {
     switch (fmt) {
-        case TAA_VEL_RG32F: { float2 v = static_cast<const float2*>(p)[i]; return make_float4(v.x, v.y, 0.f, 1.f); }
-        case TAA_VEL_RG16F: { uint32_t v = static_cast<const uint32_t*>(p)[i]; return make_float4(h2f(v & 0xffffu), h2f(v >> 16), 0.f, 1.f); }
-        case TAA_VEL_RGBA32F: return static_cast<const float4*>(p)[i];
+        case TAA_VEL_RG32F: { float2 v = gload<float2>(p, i * 8u); return make_float4(v.x, v.y, 0.f, 1.f); }
+        case TAA_VEL_RG16F: { uint32_t v = gload<uint32_t>(p, i * 4u); return make_float4(h2f(v & 0xffffu), h2f(v >> 16), 0.f, 1.f); }
+        case TAA_VEL_RGBA32F: return gload<float4>(p, i * 16u);
         default: {
-            uint2 v = static_cast<const uint2*>(p)[i];
+            uint2 v = gload<uint2>(p, i * 8u);
             return make_float4(h2f(v.x & 0xffffu), h2f(v.x >> 16), h2f(v.y & 0xffffu), h2f(v.y >> 16));
         }
     }
+}
+
+// the 16 inspect words of texel i, as four 16-byte stores
+__device__ __forceinline__ void st_inspect(void* p, uint32_t i, const int32_t (&w)[TAA_INSPECT_WORDS]) {
+    const uint32_t o = i * (uint32_t)(TAA_INSPECT_WORDS * 4);
+    gstore<int4>(p, o, make_int4(w[0], w[1], w[2], w[3]));
+    gstore<int4>(p, o, make_int4(w[4], w[5], w[6], w[7]), 16);
+    gstore<int4>(p, o, make_int4(w[8], w[9], w[10], w[11]), 32);
+    gstore<int4>(p, o, make_int4(w[12], w[13], w[14], w[15]), 48);
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
@@ -123,8 +158,8 @@ __device__ float4 bilinear_rgba(const void* img, int W, int H, int pitch, float 
     int x0 = clampi(to_int(fx), -1, W), y0 = clampi(to_int(fy), -1, H);
     int xa = clampi(x0, 0, W - 1), xb = clampi(x0 + 1, 0, W - 1);
     int ya = clampi(y0, 0, H - 1), yb = clampi(y0 + 1, 0, H - 1);
-    float4 a = ld_rgba<S16>(img, (size_t)ya * pitch + xa), b = ld_rgba<S16>(img, (size_t)ya * pitch + xb);
-    float4 c = ld_rgba<S16>(img, (size_t)yb * pitch + xa), d = ld_rgba<S16>(img, (size_t)yb * pitch + xb);
+    float4 a = ld_rgba<S16>(img, (uint32_t)(ya * pitch + xa)), b = ld_rgba<S16>(img, (uint32_t)(ya * pitch + xb));
+    float4 c = ld_rgba<S16>(img, (uint32_t)(yb * pitch + xa)), d = ld_rgba<S16>(img, (uint32_t)(yb * pitch + xb));
     float ux = 1.0f - tx, uy = 1.0f - ty;
     float4 top = make_float4(a.x * ux + b.x * tx, a.y * ux + b.y * tx, a.z * ux + b.z * tx, a.w * ux + b.w * tx);
     float4 bot = make_float4(c.x * ux + d.x * tx, c.y * ux + d.y * tx, c.z * ux + d.z * tx, c.w * ux + d.w * tx);
@@ -157,7 +192,7 @@ __device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
     for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            float4 t = ld_rgba<S16>(fd.history, (size_t)ys[j] + xs[i]);
+            float4 t = ld_rgba<S16>(fd.history, (uint32_t)(ys[j] + xs[i]));
             float a = wx[i], b = wy[j];
             float4 s = make_float4(t.x * a * b, t.y * a * b, t.z * a * b, t.w * a * b);
             if (i == 0 && j == 0) acc = s;
@@ -248,6 +283,20 @@ __device__ __forceinline__ f3 cas_filter(f3 b, f3 d, f3 e, f3 f, f3 h, float pea
 
 
 // Workgroup -> tile so that XCD k (blockIdx % 8) gets the contiguous k-th eighth of the tiles.
+// Entry of a batch descriptor table that covers tile t (the last entry with tile_begin <= t).
+// The table is read-only for the kernel's lifetime: reading it through the constant address
+// space gives scalar loads, and the image pointers in it are known to be global memory.
+__device__ __forceinline__ const BatchEntry& batch_entry(const BatchEntry* entries, int n_entries, int t) {
+    const TAA_CONSTANT BatchEntry* ents = (const TAA_CONSTANT BatchEntry*)entries;
+    int lo = 0, hi = n_entries - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ents[mid].tile_begin <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return *(const BatchEntry*)(ents + lo);
+}
+
 __device__ __forceinline__ int xcd_tile(int b, int n) {
     const int q = n >> 3, r = n & 7;
     const int xcd = b & 7, slot = b >> 3;

@@ -40,11 +40,19 @@ struct FGeo {
     typedef typename VelSlot<S16>::T VelT;
     // resolved screen texel kept for the fused CAS: the value the stored image would hold
     typedef typename std::conditional<S16, uint2, float4>::type ResT;
+    static constexpr int NIT = (RW * RH + NT - 1) / NT;  // resolve iterations per thread
+    // LDS: the staged region (Y, Co, Cg and depth in one 16-byte entry per texel, velocity, the uv
+    // of the staged columns and rows) during the resolve; afterwards the same bytes hold the
+    // resolved region that the fused CAS reads.
     struct Smem {
-        float4 cd[SH * SP];  // Y, Co, Cg (RGB without YCoCg), depth: one 16-byte LDS read per tap
-        VelT v[SH * SP];
-        float ux[SW], uy[SH];
-        ResT r[CAS ? RH * RP : 1];
+        static constexpr int kCd = SH * SP * 16, kVel = SH * SP * (int)sizeof(VelT), kUv = (SW + SH) * 4;
+        static constexpr int kStage = kCd + kVel + kUv, kRes = CAS ? RH * RP * (int)sizeof(ResT) : 0;
+        alignas(16) unsigned char raw[kStage > kRes ? kStage : kRes];
+        __device__ float4* cd() { return reinterpret_cast<float4*>(raw); }
+        __device__ VelT* v() { return reinterpret_cast<VelT*>(raw + kCd); }
+        __device__ float* ux() { return reinterpret_cast<float*>(raw + kCd + kVel); }
+        __device__ float* uy() { return ux() + SW; }
+        __device__ ResT* r() { return reinterpret_cast<ResT*>(raw); }
     };
 };
 
@@ -75,14 +83,14 @@ struct Texel {
     float r, g, b;
 };
 template <bool S16>
-__device__ __forceinline__ Texel<S16> load_texel(const void* base, size_t i) {
+__device__ __forceinline__ Texel<S16> load_texel(const void* base, uint32_t row_bytes, int i) {
     if constexpr (S16) {
-        uint2 v = static_cast<const uint2*>(base)[i];
+        uint2 v = gload<uint2>(base, row_bytes, 8 * i);
         h2 rg = __builtin_bit_cast(h2, v.x);
         h2 ba = __builtin_bit_cast(h2, v.y);
         return {(float)rg.x, (float)rg.y, (float)ba.x};
     } else {
-        float4 v = static_cast<const float4*>(base)[i];
+        float4 v = gload<float4>(base, row_bytes, 16 * i);
         return {v.x, v.y, v.z};
     }
 }
@@ -90,19 +98,19 @@ __device__ __forceinline__ Texel<S16> load_texel(const void* base, size_t i) {
 // Catmull-Rom 4x4 history filter (separable, fp32 accumulation), rgb only.
 template <bool S16>
 __device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, int pitch, int kx, int ky, float fx, float fy) {
+    constexpr uint32_t TB = S16 ? 8u : 16u;  // bytes per texel
     const W4 wx = cr_weights(fx), wy = cr_weights(fy);
     f3 acc = {0.f, 0.f, 0.f};
     const bool interior = kx >= 1 && kx + 2 <= W - 1 && ky >= 1 && ky + 2 <= H - 1;
     if (interior) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            // 32-bit product, widened before the column offsets so the four texel loads of a row
-            // fold into one address with immediate offsets
-            const size_t rb = (size_t)((uint32_t)(ky - 1 + j) * (uint32_t)pitch + (uint32_t)(kx - 1));
+            // byte offset of the row's first texel; the four texels follow as immediate offsets
+            const uint32_t rb = ((uint32_t)(ky - 1 + j) * (uint32_t)pitch + (uint32_t)(kx - 1)) * TB;
             f3 row = {0.f, 0.f, 0.f};
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                Texel<S16> t = load_texel<S16>(hist, rb + i);
+                Texel<S16> t = load_texel<S16>(hist, rb, i);
                 row.x = __builtin_fmaf(t.r, wx.w[i], row.x);
                 row.y = __builtin_fmaf(t.g, wx.w[i], row.y);
                 row.z = __builtin_fmaf(t.b, wx.w[i], row.z);
@@ -118,11 +126,11 @@ __device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, in
         for (int i = 0; i < 4; ++i) xs[i] = clampi(kxc - 1 + i, 0, W - 1);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const size_t rb = (size_t)((uint32_t)clampi(kyc - 1 + j, 0, H - 1) * (uint32_t)pitch);
+            const uint32_t rb = (uint32_t)clampi(kyc - 1 + j, 0, H - 1) * (uint32_t)pitch;
             f3 row = {0.f, 0.f, 0.f};
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                Texel<S16> t = load_texel<S16>(hist, rb + xs[i]);
+                Texel<S16> t = load_texel<S16>(hist, (rb + (uint32_t)xs[i]) * TB, 0);
                 row.x = __builtin_fmaf(t.r, wx.w[i], row.x);
                 row.y = __builtin_fmaf(t.g, wx.w[i], row.y);
                 row.z = __builtin_fmaf(t.b, wx.w[i], row.z);
@@ -151,6 +159,41 @@ __device__ __forceinline__ f3 cas_filter_fma(f3 b, f3 d, f3 e, f3 f, f3 h, float
     return o;
 }
 
+// CAS on fp16 texels (RGBA16F storage): the green minimum and maximum are exact selections on
+// (r,g) pairs in packed-half arithmetic; the amplitude, the weight and the weighted cross sum
+// are fp32 with the fp16 texels entering through mixed-precision FMAs, so HDR values cannot
+// overflow or cancel in half precision (ffx_cas.h:424-537, output alpha 1).
+__device__ __forceinline__ h2 hmax(h2 a, h2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ h2 hmin(h2 a, h2 b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ h2 as_h2(uint32_t v) { return __builtin_bit_cast(h2, v); }
+__device__ __forceinline__ float cas_channel(_Float16 b, _Float16 d, _Float16 e, _Float16 f, _Float16 h, float w, float rw) {
+    float acc = __builtin_fmaf((float)b, w, (float)e);
+    acc = __builtin_fmaf((float)d, w, acc);
+    acc = __builtin_fmaf((float)f, w, acc);
+    acc = __builtin_fmaf((float)h, w, acc);
+    return sat(acc * rw);
+}
+// two values already in [0, 1] into one dword of fp16 (round to nearest even)
+__device__ __forceinline__ uint32_t pack_unit_f2h(float a, float b) {
+    const f32x2_t v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, f16x2_t));
+}
+__device__ __forceinline__ uint2 cas_filter_h2(uint2 b, uint2 d, uint2 e, uint2 f, uint2 h, float peak) {
+    const h2 brg = as_h2(b.x), drg = as_h2(d.x), erg = as_h2(e.x), frg = as_h2(f.x), hrg = as_h2(h.x);
+    const h2 mx = hmax(hmax(drg, hmax(erg, frg)), hmax(brg, hrg));
+    const h2 mn = hmin(hmin(drg, hmin(erg, frg)), hmin(brg, hrg));
+    const float mxG = (float)mx.y, mnG = (float)mn.y;
+    float amp = sat(fminf(mnG, 1.0f - mxG) * prx_lo_rcp(mxG));
+    amp = prx_lo_sqrt(amp);
+    const float w = amp * peak;
+    const float rw = prx_med_rcp(__builtin_fmaf(4.0f, w, 1.0f));
+    const h2 bba = as_h2(b.y), dba = as_h2(d.y), eba = as_h2(e.y), fba = as_h2(f.y), hba = as_h2(h.y);
+    const float r = cas_channel(brg.x, drg.x, erg.x, frg.x, hrg.x, w, rw);
+    const float g = cas_channel(brg.y, drg.y, erg.y, frg.y, hrg.y, w, rw);
+    const float bl = cas_channel(bba.x, dba.x, eba.x, fba.x, hba.x, w, rw);
+    return make_uint2(pack_unit_f2h(r, g), pack_unit_f2h(bl, 1.0f));
+}
+
 template <class G, bool INSP>
 __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm) {
     constexpr bool S16 = G::S16;
@@ -173,7 +216,11 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
     const float gamma = P.mVarClipGamma;
     const float fW = (float)W, fH = (float)H;
     typedef typename G::VelT VelT;
-    const VelT* __restrict__ vel = static_cast<const VelT*>(fd.velocity);
+
+    float4* const scd = sm.cd();
+    VelT* const sv = sm.v();
+    float* const sux = sm.ux();
+    float* const suy = sm.uy();
 
     // 1. stage the region: YCoCg (fp32), depth, velocity; uv of the staged columns and rows
     {
@@ -191,22 +238,29 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
                 Co = 0.5f * (c.x - c.z);
             }
             const int li = sy * G::SP + sx;
-            sm.cd[li] = make_float4(Y, Co, Cg, fd.depth[gi]);
-            sm.v[li] = vel[gi];
+            scd[li] = make_float4(Y, Co, Cg, ld_f32(fd.depth, gi));
+            sv[li] = gload<VelT>(fd.velocity, gi * (uint32_t)sizeof(VelT));
             sx += DX;
             sy += DY;
             if (sx >= G::SW) { sx -= G::SW; sy += 1; }
         }
     }
-    if (tid < G::SW) sm.ux[tid] = ((float)clampi(gx0 + tid, 0, W - 1) + 0.5f) / fW;              // taa.comp:131
-    else if (tid < G::SW + G::SH) sm.uy[tid - G::SW] = ((float)clampi(gy0 + tid - G::SW, 0, H - 1) + 0.5f) / fH;
+    if (tid < G::SW) sux[tid] = ((float)clampi(gx0 + tid, 0, W - 1) + 0.5f) / fW;              // taa.comp:131
+    else if (tid < G::SW + G::SH) suy[tid - G::SW] = ((float)clampi(gy0 + tid - G::SW, 0, H - 1) + 0.5f) / fH;
     __syncthreads();
 
     constexpr float inv9 = 1.0f / 9.0f;
     constexpr int RDY = G::NT / G::RW, RDX = G::NT % G::RW;
     int ry = tid / G::RW, rx = tid - ry * G::RW;
-    for (int i = tid; i < G::RW * G::RH; i += G::NT, rx += RDX, ry += RDY) {
-        if (rx >= G::RW) { rx -= G::RW; ry += 1; }
+    typename G::ResT keep[G::CAS ? G::NIT : 1];  // resolved texels for the fused CAS, written to LDS afterwards
+#pragma unroll
+    for (int it = 0; it < G::NIT; ++it) {
+        if (it > 0) {
+            rx += RDX;
+            ry += RDY;
+            if (rx >= G::RW) { rx -= G::RW; ry += 1; }
+        }
+        if (tid + it * G::NT >= G::RW * G::RH) break;
         const int px = x0 - G::RA + rx, py = y0 - G::RA + ry;
         const int cx = clampi(px, 0, W - 1), cy = clampi(py, 0, H - 1);
         const bool owned = rx >= G::RA && rx < G::RA + G::TW && ry >= G::RA && ry < G::RA + G::TH && px < fd.x_end && py < H;
@@ -218,7 +272,7 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         float dC;
         int vo = -G::SP - 1, closest = 0;
         {
-            const float4 t = sm.cd[l - G::SP - 1];
+            const float4 t = scd[l - G::SP - 1];
             m1 = {t.x, t.y, t.z};
             m2 = {t.x * t.x, t.y * t.y, t.z * t.z};
             dC = t.w;
@@ -226,7 +280,7 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
 #pragma unroll
         for (int k = 1; k < 9; ++k) {
             const int o = (k / 3 - 1) * G::SP + (k % 3 - 1);
-            const float4 t = sm.cd[l + o];
+            const float4 t = scd[l + o];
             m1.x += t.x; m1.y += t.y; m1.z += t.z;
             m2.x = __builtin_fmaf(t.x, t.x, m2.x);
             m2.y = __builtin_fmaf(t.y, t.y, m2.y);
@@ -241,15 +295,15 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         const f3 sig = {sqrt_fast(fmaxf(__builtin_fmaf(-mean.x, mean.x, m2.x * inv9), 0.f)),
                         sqrt_fast(fmaxf(__builtin_fmaf(-mean.y, mean.y, m2.y * inv9), 0.f)),
                         sqrt_fast(fmaxf(__builtin_fmaf(-mean.z, mean.z, m2.z * inv9), 0.f))};
-        const float4 ctr = sm.cd[l];
+        const float4 ctr = scd[l];
         const f3 cur = {ctr.x, ctr.y, ctr.z};
         const float depth = ctr.w;
         if (!closest_mode) {  // velocity of the centre tap
             vo = 0;
             closest = -1;
         }
-        const float2 v = vel_xy<S16>(sm.v[l + vo]);
-        const float hu = sm.ux[lx] - v.x, hv = sm.uy[ly] - v.y;  // taa.comp:421
+        const float2 v = vel_xy<S16>(sv[l + vo]);
+        const float hu = sux[lx] - v.x, hv = suy[ly] - v.y;  // taa.comp:421
 
         // ---- Catmull-Rom history (taa.comp:441-514)
         const float tX = hu * fW - 0.5f, tY = hv * fH - 0.5f;
@@ -272,7 +326,7 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         if (depth_cull) {
             dtx = to_int(hu * fW);
             dty = to_int(hv * fH);
-            const float hd = fd.hdepth[(uint32_t)clampi(dty, 0, H - 1) * pitch + (uint32_t)clampi(dtx, 0, W - 1)];
+            const float hd = ld_f32(fd.hdepth, (uint32_t)clampi(dty, 0, H - 1) * pitch + (uint32_t)clampi(dtx, 0, W - 1));
             bool rej;
             if (lin_depth) {
                 const float n = U.mCamNearPlane, f = U.mCamFarPlane;
@@ -323,12 +377,20 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         }
 
         const uint32_t gi = (uint32_t)cy * pitch + (uint32_t)cx;
+        // fp16 bits of the result, converted once for the history, the screen and the CAS stage
+        uint2 hb = make_uint2(0u, 0u);
+        if constexpr (S16) hb = make_uint2(pack_f2h(aa.x, aa.y), f2h(aa.z));
         if (owned) {
-            st_rgba<S16>(fd.history_out, gi, make_float4(aa.x, aa.y, aa.z, 0.0f));
-            if constexpr (!G::CAS) st_rgba<S16>(fd.screen, gi, make_float4(aa.x, aa.y, aa.z, 1.0f));
+            if constexpr (S16) {
+                gstore<uint2>(fd.history_out, gi * 8u, hb);
+                if constexpr (!G::CAS) gstore<uint2>(fd.screen, gi * 8u, make_uint2(hb.x, hb.y | 0x3c000000u));
+            } else {
+                st_rgba<S16>(fd.history_out, gi, make_float4(aa.x, aa.y, aa.z, 0.0f));
+                if constexpr (!G::CAS) st_rgba<S16>(fd.screen, gi, make_float4(aa.x, aa.y, aa.z, 1.0f));
+            }
             if constexpr (INSP) {
                 if (rejected) flags |= TAA_FLAG_REJECTED;
-                int32_t* w = static_cast<int32_t*>(fd.inspect) + (size_t)gi * TAA_INSPECT_WORDS;
+                int32_t w[TAA_INSPECT_WORDS];
                 w[TAA_INSPECT_HIST_TC_X] = kx;
                 w[TAA_INSPECT_HIST_TC_Y] = ky;
                 w[TAA_INSPECT_CLOSEST_TAP] = closest;
@@ -345,16 +407,25 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
                 w[TAA_INSPECT_HIST_UV_X] = __float_as_int(hu);
                 w[TAA_INSPECT_HIST_UV_Y] = __float_as_int(hv);
                 w[15] = 0;
+                st_inspect(fd.inspect, gi, w);
             }
         }
         if constexpr (G::CAS) {
-            const int li = ry * G::RP + rx;
-            if constexpr (S16) sm.r[li] = make_uint2(f2h(aa.x) | (f2h(aa.y) << 16), f2h(aa.z));
-            else sm.r[li] = make_float4(aa.x, aa.y, aa.z, 0.0f);
+            if constexpr (S16) keep[it] = hb;
+            else keep[it] = make_float4(aa.x, aa.y, aa.z, 0.0f);
         }
     }
 
     if constexpr (G::CAS) {
+        __syncthreads();  // every thread is done with the staged region
+        typename G::ResT* const sr = sm.r();
+#pragma unroll
+        for (int it = 0; it < G::NIT; ++it) {
+            const int i = tid + it * G::NT;
+            if (i >= G::RW * G::RH) break;
+            const int qy = i / G::RW, qx = i - qy * G::RW;
+            sr[qy * G::RP + qx] = keep[it];
+        }
         __syncthreads();
         const float peak = __uint_as_float(fd.cas_peak_bits);
         constexpr int CDY = G::NT / G::TW, CDX = G::NT % G::TW;
@@ -364,17 +435,18 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
             const int px = x0 + ox, py = y0 + oy;
             if (px >= fd.x_end || py >= H) continue;
             const int li = (oy + 1) * G::RP + (ox + 1);
-            auto at = [&](int k) {
-                if constexpr (S16) {
-                    const uint2 q = sm.r[k];
-                    return f3{h2f(q.x & 0xffffu), h2f(q.x >> 16), h2f(q.y & 0xffffu)};
-                } else {
-                    const float4 q = sm.r[k];
+            const uint32_t go = (uint32_t)py * pitch + (uint32_t)px;
+            if constexpr (S16) {
+                gstore<uint2>(fd.screen, go * 8u,
+                    cas_filter_h2(sr[li - G::RP], sr[li - 1], sr[li], sr[li + 1], sr[li + G::RP], peak));
+            } else {
+                auto at = [&](int k) {
+                    const float4 q = sr[k];
                     return f3{q.x, q.y, q.z};
-                }
-            };
-            const f3 o = cas_filter_fma(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak);
-            st_rgba<S16>(fd.screen, (uint32_t)py * pitch + (uint32_t)px, make_float4(o.x, o.y, o.z, 1.0f));
+                };
+                const f3 o = cas_filter_fma(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak);
+                st_rgba<S16>(fd.screen, go, make_float4(o.x, o.y, o.z, 1.0f));
+            }
         }
     }
 }
@@ -393,7 +465,7 @@ __global__ __launch_bounds__(256) void k_resolve_fast(const FastArgs a) {
 }
 
 template <int TW, int TH, bool CAS, bool S16>
-__global__ __launch_bounds__(256) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = FGeo<TW, TH, CAS, S16>;
     __shared__ typename G::Smem sm;
     const int per = (total_tiles + 7) >> 3;
@@ -401,14 +473,8 @@ __global__ __launch_bounds__(256) void k_resolve_fast_batch(const BatchEntry* __
     for (int j = slot; j < per; j += slots) {
         const int t = xcd * per + j;
         if (t >= total_tiles) break;
-        int lo = 0, hi = n_entries - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (entries[mid].tile_begin <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const BatchEntry& e = entries[lo];
-        fast_tile<G, false>(e.fd, *e.U, t - e.tile_begin, sm);
+        const BatchEntry& e = batch_entry(entries, n_entries, t);
+        fast_tile<G, false>(e.fd, e.U, t - e.tile_begin, sm);
         __syncthreads();
     }
 }

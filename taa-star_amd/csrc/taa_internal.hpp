@@ -34,9 +34,9 @@ struct FrameDesc {
 // One frame of a batched launch: frame + device copy of its uniforms + first flattened tile.
 struct BatchEntry {
     FrameDesc fd;
-    const taa_uniforms_t* U;  // device pointer
+    taa_uniforms_t U;         // the frame's uniforms, inline so the kernel reads them with scalar loads
     int32_t tile_begin;       // prefix sum of tiles of the entries before this one
-    int32_t pad;
+    int32_t pad[3];
 };
 
 // Tile shapes the kernels are instantiated for (output tile, 256 threads per workgroup).

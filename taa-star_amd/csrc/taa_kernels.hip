@@ -50,12 +50,12 @@ struct PixelResult {
 // One output texel of taa.comp main (taa.comp:707-960), FP32 arithmetic. `l` indexes the texel
 // in the staged LDS planes (row pitch SP).
 template <bool S16, int SP>
-__device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
+__device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
                                          int x, int y, int l, const float* __restrict__ s0,
                                          const float* __restrict__ s1, const float* __restrict__ s2,
                                          const float* __restrict__ sd, bool write_aux) {
     const int W = fd.W, H = fd.H;
-    const size_t gi = (size_t)y * fd.pitch + x;
+    const uint32_t gi = (uint32_t)(y * fd.pitch + x);
     PixelResult R;
     const float uvx = ((float)x + 0.5f) / (float)W, uvy = ((float)y + 0.5f) / (float)H;  // taa.comp:131
 
@@ -66,9 +66,10 @@ __device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms
         R.history = make_float4(h.x, h.y, h.z, 1.0f);
         if (write_aux && fd.debug) st_rgba16(fd.debug, gi, make_float4(0.f, 0.f, 0.f, 0.f));
         if (write_aux && fd.inspect) {
-            int32_t* w = static_cast<int32_t*>(fd.inspect) + gi * TAA_INSPECT_WORDS;
+            int32_t w[TAA_INSPECT_WORDS];
 #pragma unroll
             for (int k = 0; k < TAA_INSPECT_WORDS; ++k) w[k] = (k == TAA_INSPECT_CLOSEST_TAP) ? -1 : 0;
+            st_inspect(fd.inspect, gi, w);
         }
         return R;
     }
@@ -144,7 +145,7 @@ __device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms
             flags |= TAA_FLAG_USED_VELOCITY;
             if (P.mVelocitySampleMode == 1) {
                 // taa.comp:398-412: the longest vector is tracked but the last tap is assigned
-                float4 last = ld_vel(fd.velocity, fd.vel_fmt, (size_t)min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1));
+                float4 last = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1)));
                 vs.x = last.x;
                 vs.y = last.y;
             } else if (P.mVelocitySampleMode == 2) {
@@ -158,7 +159,7 @@ __device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms
                 }
                 closest = best;
                 int vx = clampi(x + best % 3 - 1, 0, W - 1), vy = clampi(y + best / 3 - 1, 0, H - 1);
-                vs = ld_vel(fd.velocity, fd.vel_fmt, (size_t)vy * fd.pitch + vx);
+                vs = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(vy * fd.pitch + vx));
             }
             hu = uvx - vs.x;
             hv = uvy - vs.y;
@@ -202,7 +203,7 @@ __device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms
     if (P.mDynamicAntiGhosting) {
         const float eps = 1e-5f;
         auto mov = [&](int dx, int dy) {
-            float4 v = ld_vel(fd.velocity, fd.vel_fmt, (size_t)clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1));
+            float4 v = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1)));
             return (fabsf(v.x) > eps || fabsf(v.y) > eps) && (fabsf(v.w) >= 0.5f);
         };
         bool movL = mov(-1, 0), movR = mov(1, 0), movT = mov(0, -1), movB = mov(0, 1), movC = mov(0, 0);
@@ -216,7 +217,7 @@ __device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms
     if (P.mDepthCulling) {
         dtx = to_int(hu * (float)W);
         dty = to_int(hv * (float)H);
-        float historyDepth = fd.hdepth[(size_t)clampi(dty, 0, H - 1) * fd.pitch + clampi(dtx, 0, W - 1)];
+        float historyDepth = ld_f32(fd.hdepth, (uint32_t)(clampi(dty, 0, H - 1) * fd.pitch + clampi(dtx, 0, W - 1)));
         bool rej;
         if (U.pad1 > 0.0f) {
             float n = U.mCamNearPlane, f = U.mCamFarPlane;
@@ -297,7 +298,7 @@ __device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms
     if (write_aux && fd.inspect) {
         if (rejected) flags |= TAA_FLAG_REJECTED;
         if (rectified) flags |= TAA_FLAG_RECTIFIED;
-        int32_t* w = static_cast<int32_t*>(fd.inspect) + gi * TAA_INSPECT_WORDS;
+        int32_t w[TAA_INSPECT_WORDS];
         w[TAA_INSPECT_HIST_TC_X] = to_int(floorf(hu * (float)W - 0.5f));
         w[TAA_INSPECT_HIST_TC_Y] = to_int(floorf(hv * (float)H - 0.5f));
         w[TAA_INSPECT_CLOSEST_TAP] = closest;
@@ -314,6 +315,7 @@ __device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms
         w[TAA_INSPECT_HIST_UV_X] = __float_as_int(hu);
         w[TAA_INSPECT_HIST_UV_Y] = __float_as_int(hv);
         w[15] = 0;
+        st_inspect(fd.inspect, gi, w);
     }
     return R;
 }
@@ -322,7 +324,7 @@ __device__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms
 // one tile
 
 template <class G, bool S16>
-__device__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm) {
+__device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm) {
     const taa_params_t& P = U.param[fd.param_idx];
     const int W = fd.W, H = fd.H, pitch = fd.pitch;
     const int ty = tile / fd.tiles_x, tx = tile - ty * fd.tiles_x;
@@ -334,13 +336,13 @@ __device__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int t
     for (int i = tid; i < G::SW * G::SH; i += G::NT) {
         const int sy = i / G::SW, sx = i - sy * G::SW;
         const int gx = clampi(gx0 + sx, 0, W - 1), gy = clampi(gy0 + sy, 0, H - 1);
-        const size_t gi = (size_t)gy * pitch + gx;
+        const uint32_t gi = (uint32_t)(gy * pitch + gx);
         f3 t = stage_convert(P, xyz(ld_rgba<S16>(fd.color, gi)));
         const int li = sy * G::SP + sx;
         sm.c0[li] = t.x;
         sm.c1[li] = t.y;
         sm.c2[li] = t.z;
-        sm.d[li] = fd.depth[gi];
+        sm.d[li] = ld_f32(fd.depth, gi);
     }
     __syncthreads();
 
@@ -353,7 +355,7 @@ __device__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int t
         const int l = (cy - gy0) * G::SP + (cx - gx0);
         PixelResult r = resolve_pixel_f32<S16, G::SP>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, owned);
         if (owned) {
-            const size_t gi = (size_t)cy * pitch + cx;
+            const uint32_t gi = (uint32_t)(cy * pitch + cx);
             st_rgba<S16>(fd.history_out, gi, r.history);
             if constexpr (!G::CAS) st_rgba<S16>(fd.screen, gi, make_float4(r.screen.x, r.screen.y, r.screen.z, 1.0f));
         }
@@ -379,7 +381,7 @@ __device__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int t
             const int li = (oy + 1) * G::RP + (ox + 1);
             auto at = [&](int k) { return f3{sm.r0[k], sm.r1[k], sm.r2[k]}; };
             f3 o = cas_filter(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak);
-            st_rgba<S16>(fd.screen, (size_t)py * pitch + px, make_float4(o.x, o.y, o.z, 1.0f));
+            st_rgba<S16>(fd.screen, (uint32_t)(py * pitch + px), make_float4(o.x, o.y, o.z, 1.0f));
         }
     }
 }
@@ -390,7 +392,7 @@ struct ResolveArgs {
 };
 
 template <int TW, int TH, int PAD, bool CAS, bool S16>
-__global__ __launch_bounds__(256) void k_resolve(const ResolveArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3))) void k_resolve(const ResolveArgs a) {
     using G = Geo<TW, TH, PAD, CAS>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(256) void k_resolve(const ResolveArgs a) {
 // Persistent batched resolve: the grid is sized to the machine and walks the flattened tile
 // space of every frame in the batch. XCD k walks the k-th eighth of the tiles.
 template <int TW, int TH, int PAD, bool CAS, bool S16>
-__global__ __launch_bounds__(256) void k_resolve_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3))) void k_resolve_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = Geo<TW, TH, PAD, CAS>;
     __shared__ typename G::Smem sm;
     const int per = (total_tiles + 7) >> 3;
@@ -408,14 +410,8 @@ __global__ __launch_bounds__(256) void k_resolve_batch(const BatchEntry* __restr
     for (int j = slot; j < per; j += slots) {
         const int t = xcd * per + j;
         if (t >= total_tiles) break;
-        int lo = 0, hi = n_entries - 1;  // last entry with tile_begin <= t
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (entries[mid].tile_begin <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const BatchEntry& e = entries[lo];
-        resolve_tile<G, S16>(e.fd, *e.U, t - e.tile_begin, sm);
+        const BatchEntry& e = batch_entry(entries, n_entries, t);
+        resolve_tile<G, S16>(e.fd, e.U, t - e.tile_begin, sm);
         __syncthreads();  // LDS is reused by the next tile
     }
 }
@@ -426,10 +422,10 @@ __global__ __launch_bounds__(256) void k_cas(const void* __restrict__ src, void*
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= W || y >= H) return;
     auto ld = [&](int xx, int yy) {
-        return xyz(ld_rgba<S16>(src, (size_t)clampi(yy, 0, H - 1) * pitch + clampi(xx, 0, W - 1)));
+        return xyz(ld_rgba<S16>(src, (uint32_t)(clampi(yy, 0, H - 1) * pitch + clampi(xx, 0, W - 1))));
     };
     f3 o = cas_filter(ld(x, y - 1), ld(x - 1, y), ld(x, y), ld(x + 1, y), ld(x, y + 1), __uint_as_float(peak_bits));
-    st_rgba<S16>(dst, (size_t)y * pitch + x, make_float4(o.x, o.y, o.z, 1.0f));
+    st_rgba<S16>(dst, (uint32_t)(y * pitch + x), make_float4(o.x, o.y, o.z, 1.0f));
 }
 
 // ------------------------------------------------------------------------------------------

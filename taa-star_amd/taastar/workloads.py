@@ -16,7 +16,8 @@ NEAR, FAR = 0.1, 100.0          # camera planes for the linear-depth test (main.
 LINEAR_DEPTH_THRESHOLD = 0.01   # c2/c3: reject when |linear(hd) - linear(expected)| > 0.01
 FP16_COLOR_MAX = 32768.0        # scene colour clamp for FP16 storage (keeps HDR fireflies finite)
 
-RESOLUTIONS = {"720p": (1280, 720), "1080p": (1920, 1080), "1440p": (2560, 1440), "4k": (3840, 2160)}
+RESOLUTIONS = {"360p": (640, 360), "720p": (1280, 720), "1080p": (1920, 1080), "1440p": (2560, 1440),
+               "4k": (3840, 2160)}
 SERVE_MIX = (("720p", 0.40), ("1080p", 0.35), ("1440p", 0.15), ("4k", 0.10))
 
 

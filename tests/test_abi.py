@@ -132,6 +132,14 @@ def test_argument_errors_need_no_gpu():
     assert lib.taa_resolve_frame(C.byref(U), C.byref(img), None, 0, None) == _abi.TAA_ERR_UNSUPPORTED
     assert b"mRayTraceAugment" in lib.taa_last_error()
     assert lib.taa_cas_apply(None, None, 4, 4, 4, 0, None, None) == _abi.TAA_ERR_INVALID_ARG
+    # images address with 32-bit byte offsets: a frame whose 16-byte image would reach 4 GiB is refused
+    U.param[0].mRayTraceAugment = 0
+    big = taastar.FrameImages()
+    for f in ("color", "depth", "history", "history_depth", "result_screen", "velocity", "result_history"):
+        setattr(big, f, 4096 * (1 + len(f)))
+    big.width, big.height, big.pitch = 16384, 16384, 16384
+    assert lib.taa_resolve_frame(C.byref(U), C.byref(big), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
+    assert b"4 GiB" in lib.taa_last_error()
     assert lib.taa_session_close(None) == _abi.TAA_ERR_INVALID_ARG
 
 
