@@ -465,7 +465,7 @@ __global__ __launch_bounds__(256) void k_resolve_fast(const FastArgs a) {
 }
 
 template <int TW, int TH, bool CAS, bool S16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = FGeo<TW, TH, CAS, S16>;
     __shared__ typename G::Smem sm;
     const int per = (total_tiles + 7) >> 3;
