@@ -191,7 +191,7 @@ MODE_NAMES = {0: "served open loop (Poisson arrivals, churn)", 1: "served closed
 
 
 def simulate(mode, mix, sessions, fmt, arith, seconds, exact=True, rate=0.0, churn=0.0, budget=0, seed=11,
-             max_batch=64, max_inflight=3, warmup=2):
+             max_batch=64, max_inflight=3, warmup=2, own_inputs=True, pan_max=24.0):
     from taastar import ServeReport, ServeSimConfig, check, lib
     from taastar.workloads import RESOLUTIONS
     cfg = ServeSimConfig()
@@ -204,6 +204,7 @@ def simulate(mode, mix, sessions, fmt, arith, seconds, exact=True, rate=0.0, chu
     cfg.hbm_budget_bytes = budget
     cfg.color_format, cfg.arith, cfg.cas_sharpness = fmt, arith, 0.5
     cfg.max_batch, cfg.max_inflight, cfg.warmup_frames, cfg.exact_mix = max_batch, max_inflight, warmup, int(exact)
+    cfg.per_session_inputs, cfg.pan_max = int(own_inputs), pan_max
     rep = ServeReport()
     check(lib().taa_serve_simulate(C.byref(cfg), C.byref(rep)))
     d = rep.as_dict()
@@ -244,7 +245,7 @@ def extras(a):
     # 64 sessions of the serving mix: per-frame plain launches vs the batched and served forms
     mix = list(SERVE_MIX)
     m64 = {"mix": "64 sessions: 26x1280x720, 22x1920x1080, 10x2560x1440, 6x3840x2160 (FP16 storage and "
-                  "arithmetic, fused CAS 0.5)"}
+                  "arithmetic, fused CAS 0.5; every session its own scene, jitter phase and pan of 0-24 px/frame)"}
     for mode in (3, 2, 4, 1):
         m64[f"mode{mode}"] = simulate(mode, mix, 64, F16, A16, s)
     base = m64["mode3"]["gpix_per_s"]
@@ -265,7 +266,8 @@ def extras(a):
     # 48 streams in one persistent graph-captured launch, both storages
     mix48 = [("720p", 16), ("1080p", 16), ("1440p", 8), ("4k", 8)]
     x["batched48"] = {
-        "mix": "16x1280x720, 16x1920x1080, 8x2560x1440, 8x3840x2160",
+        "mix": "16x1280x720, 16x1920x1080, 8x2560x1440, 8x3840x2160; every stream its own scene, jitter phase and "
+               "pan of 0-24 px/frame",
         "fp16_storage_fp16_arith": simulate(4, mix48, 48, F16, A16, s),
         "fp32_storage_fp32_arith": simulate(4, mix48, 48, F32, A32, s),
         "fp16_per_frame_async": simulate(2, mix48, 48, F16, A16, s),
@@ -274,11 +276,11 @@ def extras(a):
     # served load: 256 sessions, Poisson arrivals at 6000 frames/s, 10% churn per second
     x["served256"] = {
         "load": "256 sessions from {720p 40%, 1080p 35%, 1440p 15%, 4K 10%}, Poisson 6000 frames/s, churn 10%/s, "
-                "FP16 storage and arithmetic, fused CAS 0.5",
+                "FP16 storage and arithmetic, fused CAS 0.5; sessions of a class share one pre-rendered input frame",
         "budget_200GB": simulate(0, mix, 256, F16, A16, 2 * s, exact=False, rate=6000.0, churn=0.10,
-                                 budget=200 * 10**9, max_batch=256),
+                                 budget=200 * 10**9, max_batch=256, own_inputs=False),
         "budget_16GB": simulate(0, mix, 256, F16, A16, 2 * s, exact=False, rate=6000.0, churn=0.10,
-                                budget=16 * 10**9, max_batch=256),
+                                budget=16 * 10**9, max_batch=256, own_inputs=False),
     }
     return x
 

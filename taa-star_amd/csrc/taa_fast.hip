@@ -65,6 +65,20 @@ __device__ __forceinline__ float2 vel_xy(typename VelSlot<S16>::T v) {
     else return v;
 }
 
+// Linear-depth rejection |L(hd) - L(d)| > thr with L(x) = n*f / (f + x*(n - f)), decided exactly as
+// with the IEEE divisions of the FP32 path. The quotients through the hardware reciprocal are
+// within 2^-22 (relative) of the correctly rounded ones, so their difference is within
+// 2^-21 * (|a| + |b|) of the FP32 path's; every case farther than twice that from the threshold is
+// decided by them, the rest (and any non-finite case) by the IEEE divisions themselves.
+__device__ __forceinline__ bool linear_depth_reject(float hd, float d, float n, float f, float thr) {
+    const float nf = n * f, q1 = f + hd * (n - f), q2 = f + d * (n - f);  // linearize_depth's operations
+    const float a = nf * rcp_fast(q1), b = nf * rcp_fast(q2);
+    const float diff = fabsf(a - b), m = 0x1p-20f * (fabsf(a) + fabsf(b));
+    if (diff > thr + m) return true;
+    if (diff < thr - m) return false;
+    return fabsf(nf / q1 - nf / q2) > thr;
+}
+
 // Catmull-Rom weights (taa.comp:464-467) for one axis, fp32
 struct W4 { float w[4]; };
 __device__ __forceinline__ W4 cr_weights(float f) {
@@ -330,7 +344,7 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
             bool rej;
             if (lin_depth) {
                 const float n = U.mCamNearPlane, f = U.mCamFarPlane;
-                rej = fabsf(linearize_depth(hd, n, f) - linearize_depth(depth, n, f)) > U.pad1;
+                rej = linear_depth_reject(hd, depth, n, f, U.pad1);
             } else {
                 rej = fabsf(hd - depth) > 0.1f * (1.0f - hd);
             }
@@ -344,13 +358,15 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         const f3 mn = {__builtin_fmaf(-gamma, sig.x, mean.x), __builtin_fmaf(-gamma, sig.y, mean.y), __builtin_fmaf(-gamma, sig.z, mean.z)};
         const f3 mx = {__builtin_fmaf(gamma, sig.x, mean.x), __builtin_fmaf(gamma, sig.y, mean.y), __builtin_fmaf(gamma, sig.z, mean.z)};
         if (rect == 2) {
+            // clipAabb (taa.comp:323-345): ma = max_i |v_i| / e_i, clip when ma > 1, with one
+            // reciprocal: ma = M / D for M = max_i |v_i| * prod_{j != i} e_j and D = e_x e_y e_z
             const f3 vc = h - mean;
-            const float ax = fabsf(vc.x) * rcp_fast(__builtin_fmaf(gamma, sig.x, 1e-7f));
-            const float ay = fabsf(vc.y) * rcp_fast(__builtin_fmaf(gamma, sig.y, 1e-7f));
-            const float az = fabsf(vc.z) * rcp_fast(__builtin_fmaf(gamma, sig.z, 1e-7f));
-            const float ma = fmaxf(ax, fmaxf(ay, az));
-            if (ma > 1.0f) {
-                const float s = rcp_fast(ma);
+            const float ex = __builtin_fmaf(gamma, sig.x, 1e-7f), ey = __builtin_fmaf(gamma, sig.y, 1e-7f),
+                        ez = __builtin_fmaf(gamma, sig.z, 1e-7f);
+            const float exy = ex * ey, D = exy * ez;
+            const float M = fmaxf(fabsf(vc.x) * (ey * ez), fmaxf(fabsf(vc.y) * (ex * ez), fabsf(vc.z) * exy));
+            if (M > D) {
+                const float s = D * rcp_fast(M);
                 h = {__builtin_fmaf(vc.x, s, mean.x), __builtin_fmaf(vc.y, s, mean.y), __builtin_fmaf(vc.z, s, mean.z)};
             }
         } else if (rect == 1) {

@@ -382,11 +382,17 @@ struct PoolFrame {
     float jitter[8][2];
 };
 
-int make_pool_frame(PoolFrame& pf, int w, int h, int fmt, uint64_t seed) {
+int make_pool_frame(PoolFrame& pf, int w, int h, int fmt, uint64_t seed, int pan_mode = 0, float pan = -1.0f,
+                    int jitter_phase = 0) {
     taa_scene_config_t sc;
     taa_scene_defaults(&sc);
     sc.width = w;
     sc.height = h;
+    if (pan >= 0.0f) {
+        sc.pan_mode = pan_mode;
+        sc.pan = pan;
+    }
+    sc.jitter_phase = jitter_phase;
     sc.num_polygons = std::max(1, (int)std::lround(2000.0 * w * h / (1920.0 * 1080.0)));
     sc.seed = seed;
     sc.color_max = fmt == TAA_FMT_RGBA16F ? 32768.0f : 0.0f;
@@ -401,7 +407,8 @@ int make_pool_frame(PoolFrame& pf, int w, int h, int fmt, uint64_t seed) {
     jc.sample_distribution = 2;
     jc.fixed_index = -1;
     jc.extra_scale = 1.0f;
-    for (int k = 0; k < 8; ++k) taa_jitter_offset(&jc, (uint64_t)k, (uint32_t)w, (uint32_t)h, pf.jitter[k], nullptr);
+    for (int k = 0; k < 8; ++k)
+        taa_jitter_offset(&jc, (uint64_t)(k + jitter_phase), (uint32_t)w, (uint32_t)h, pf.jitter[k], nullptr);
     taa_scene_destroy(scene);
     pf.w = w;
     pf.h = h;
@@ -448,13 +455,13 @@ taa_params_t sim_params() {
     return p;
 }
 
-void sim_uniforms(taa_uniforms_t& U, const PoolFrame& pf, uint64_t frame) {
+void sim_uniforms(taa_uniforms_t& U, const float (&jitter)[8][2], uint64_t frame) {
     std::memset(&U, 0, sizeof(U));
     U.param[0] = sim_params();
     U.param[1] = U.param[0];
     for (int i : {0, 5, 10, 15}) U.mHistoryViewProjMatrix[i] = U.mInverseViewProjMatrix[i] = 1.0f;
-    U.mJitterNdc[0] = pf.jitter[frame % 8][0];
-    U.mJitterNdc[1] = pf.jitter[frame % 8][1];
+    U.mJitterNdc[0] = jitter[frame % 8][0];
+    U.mJitterNdc[1] = jitter[frame % 8][1];
     U.mCamNearPlane = 0.1f;
     U.mCamFarPlane = 100.0f;
     U.pad1 = 0.01f;
@@ -490,6 +497,7 @@ struct SimSession {
     int cls;
     uint64_t frame = 0;
     bool departing = false;
+    float jitter[8][2];
 };
 
 }  // namespace
@@ -540,7 +548,15 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         if (r) return r;
         taa_session_io_t io;
         taa_server_io(srv, sid, &io, nullptr);
-        const PoolFrame& pf = pool[cls];
+        PoolFrame own;
+        if (cfg->per_session_inputs) {  // this session's own scene, pan and jitter phase
+            const int k = (int)out->admissions - 1;
+            const float pan = (float)(U01(rng) * (double)std::max(0.0f, cfg->pan_max));
+            int r = make_pool_frame(own, cfg->class_w[cls], cfg->class_h[cls], cfg->color_format,
+                                    cfg->seed ^ (0x7AA50004ull + (uint64_t)k), 1, pan, k % 8);
+            if (r) return r;
+        }
+        const PoolFrame& pf = cfg->per_session_inputs ? own : pool[cls];
         const size_t px = (size_t)pf.w * pf.h;
         hipMemcpyAsync(io.color, pf.color, px * color_bpp(cfg->color_format), hipMemcpyDeviceToDevice, copy_st);
         hipMemcpyAsync(io.depth, pf.depth, px * 4, hipMemcpyDeviceToDevice, copy_st);
@@ -550,6 +566,12 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         SimSession x;
         x.sid = sid;
         x.cls = cls;
+        std::memcpy(x.jitter, pf.jitter, sizeof(x.jitter));
+        if (cfg->per_session_inputs) {
+            hipFree(own.color);
+            hipFree(own.depth);
+            hipFree(own.vel);
+        }
         ss.push_back(x);
         out->sessions_peak = std::max<int32_t>(out->sessions_peak, (int32_t)ss.size());
         return 0;
@@ -584,7 +606,7 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
     };
     auto submit = [&](SimSession& x, double t) {
         taa_uniforms_t Uf;
-        sim_uniforms(Uf, pool[x.cls], x.frame);
+        sim_uniforms(Uf, x.jitter, x.frame);
         taa_server_submit(srv, x.sid, &Uf, x.frame, t);
         x.frame++;
     };
@@ -688,7 +710,7 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         taa_batch_plan_t* plan = nullptr;
         rc = taa_batch_plan_create(srv->ctx, (int)items.size(), items.data(), cfg->arith, &plan);
         auto refresh = [&]() {
-            for (size_t i = 0; i < ss.size(); ++i) sim_uniforms(us[i], pool[ss[i].cls], ss[i].frame++);
+            for (size_t i = 0; i < ss.size(); ++i) sim_uniforms(us[i], ss[i].jitter, ss[i].frame++);
         };
         for (int k = 0; rc == 0 && k < std::max(1, cfg->warmup_frames); ++k) {
             refresh();
@@ -735,7 +757,7 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         }
         auto one = [&](size_t i) {
             taa_uniforms_t Uf;
-            sim_uniforms(Uf, pool[ss[i].cls], ss[i].frame++);
+            sim_uniforms(Uf, ss[i].jitter, ss[i].frame++);
             return taa_session_resolve(hs[i], &Uf, srv->cas_on ? &cs[i] : nullptr, scr[i], cfg->arith, st);
         };
         for (int k = 0; k < std::max(1, cfg->warmup_frames); ++k)

@@ -128,7 +128,8 @@ class ServeSimConfig(C.Structure):
                 ("class_p", C.c_float * 8), ("arrival_rate", C.c_double), ("churn_per_s", C.c_double),
                 ("duration_s", C.c_double), ("hbm_budget_bytes", C.c_uint64), ("color_format", C.c_int32),
                 ("arith", C.c_int32), ("cas_sharpness", C.c_float), ("max_batch", C.c_int32),
-                ("max_inflight", C.c_int32), ("warmup_frames", C.c_int32), ("exact_mix", C.c_int32)]
+                ("max_inflight", C.c_int32), ("warmup_frames", C.c_int32), ("exact_mix", C.c_int32),
+                ("per_session_inputs", C.c_int32), ("pan_max", C.c_float)]
 
 
 class ServeReport(C.Structure):
