@@ -24,7 +24,10 @@ sys.path.insert(0, os.path.join(ROOT, "taa-star_amd"))
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each divided by what the same
 # counter reads for a device-to-device copy of known size), keyed by (workload, arith)
-TRAFFIC = {}
+TRAFFIC = {
+    # profiles/k_resolve_fast_3840x2160_fp16_cas.md: (124,156.0 KiB x 1.9997 + 129,775.5 KiB x 1.0) x 1024
+    ("c3", "fp16"): 387128672,
+}
 
 
 def parse():
