@@ -190,7 +190,7 @@ def tune(W, H, fmt, arith, cas, iters=20):
 
 MODE_NAMES = {0: "served open loop (Poisson arrivals, churn)", 1: "served closed loop (continuous batching)",
               2: "per-frame launches, async", 3: "per-frame launches, host waits each frame",
-              4: "batched: one graph-captured persistent launch per round of all sessions"}
+              4: "batched: one graph-captured launch per round of all sessions"}
 
 
 def simulate(mode, mix, sessions, fmt, arith, seconds, exact=True, rate=0.0, churn=0.0, budget=0, seed=11,

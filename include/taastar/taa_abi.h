@@ -278,11 +278,11 @@ typedef struct taa_batch_item {
 } taa_batch_item_t;
 
 /* Resolve frames of many sessions (any mix of resolutions, one colour format, one arith) in
- * ONE persistent kernel launch. Sessions must be distinct. Stream-ordered. */
+ * ONE kernel launch over the tiles of all frames. Sessions must be distinct. Stream-ordered. */
 int32_t taa_batch_submit(taa_ctx_t* ctx, int32_t n, const taa_batch_item_t* items,
                          int32_t arith, void* stream);
 
-/* A batch plan captures descriptor upload + the persistent kernel in a hipGraph for a fixed set
+/* A batch plan captures descriptor upload + the batched kernel in a hipGraph for a fixed set
  * of sessions; each launch re-reads the items' uniforms and advances every session by one frame.
  * Tile shape per plan comes from the context's autotuner table. */
 typedef struct taa_batch_plan taa_batch_plan_t;

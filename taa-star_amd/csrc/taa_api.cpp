@@ -516,10 +516,9 @@ int32_t taa_batch_submit(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items,
     hipError_t e = hipMemcpyAsync(slot.dev, slot.host, sizeof(taastar::BatchEntry) * (size_t)n, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "descriptor upload");
     const auto* d_ents = static_cast<const taastar::BatchEntry*>(slot.dev);
-    int le = fast ? taastar::launch_batch_fast(d_ents, n, total, shape, cas, s16,
-                                               taastar::batch_grid_size_fast(shape, cas, s16, c->device), stream)
-                  : taastar::launch_batch(d_ents, n, total, shape, cas, s16, arith,
-                                          taastar::batch_grid_size(shape, cas, s16, arith, c->device), stream);
+    const int grid = taastar::batch_grid(total);
+    int le = fast ? taastar::launch_batch_fast(d_ents, n, total, shape, cas, s16, grid, stream)
+                  : taastar::launch_batch(d_ents, n, total, shape, cas, s16, arith, grid, stream);
     if (le) return hip_fail((hipError_t)le, "batch launch");
     if ((e = hipEventRecord(slot.done, st)) != hipSuccess) return hip_fail(e, "event record");
     slot.used = true;
@@ -567,8 +566,7 @@ void plan_free(taa_batch_plan* p) {
 
 // Capture descriptor upload + persistent kernel for one buffer and kernel family.
 int plan_capture(taa_batch_plan* p, taa_batch_plan::Buf& b, bool fast, int shape, bool cas, bool s16, int total) {
-    const int grid = fast ? taastar::batch_grid_size_fast(shape, cas, s16, p->ctx->device)
-                          : taastar::batch_grid_size(shape, cas, s16, p->arith, p->ctx->device);
+    const int grid = taastar::batch_grid(total);
     hipGraph_t g = nullptr;
     hipError_t e;
     if ((e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");

@@ -481,7 +481,7 @@ __global__ __launch_bounds__(256) void k_resolve_fast(const FastArgs a) {
 }
 
 template <int TW, int TH, bool CAS, bool S16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
+__global__ __launch_bounds__(256) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = FGeo<TW, TH, CAS, S16>;
     __shared__ typename G::Smem sm;
     const int per = (total_tiles + 7) >> 3;
@@ -521,17 +521,6 @@ hipError_t launch_fast_batch_t(const BatchEntry* e, int n, int total, int shape,
         default: hipLaunchKernelGGL((k_resolve_fast_batch<S[3].tw, S[3].th, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
     }
     return hipGetLastError();
-}
-
-template <bool CAS, bool S16>
-const void* fast_batch_ptr(int shape) {
-    constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
-    switch (shape) {
-        case 0: return (const void*)&k_resolve_fast_batch<S[0].tw, S[0].th, CAS, S16>;
-        case 1: return (const void*)&k_resolve_fast_batch<S[1].tw, S[1].th, CAS, S16>;
-        case 2: return (const void*)&k_resolve_fast_batch<S[2].tw, S[2].th, CAS, S16>;
-        default: return (const void*)&k_resolve_fast_batch<S[3].tw, S[3].th, CAS, S16>;
-    }
 }
 
 }  // namespace
@@ -577,13 +566,5 @@ int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tile
     return (int)e;
 }
 
-int batch_grid_size_fast(int shape, bool cas, bool s16, int device) {
-    const void* k = cas ? (s16 ? fast_batch_ptr<true, true>(shape) : fast_batch_ptr<true, false>(shape))
-                        : (s16 ? fast_batch_ptr<false, true>(shape) : fast_batch_ptr<false, false>(shape));
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
-    return ((per_cu * cus) + 7) & ~7;
-}
 
 }  // namespace taastar

@@ -399,8 +399,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3
     resolve_tile<G, S16>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
 }
 
-// Persistent batched resolve: the grid is sized to the machine and walks the flattened tile
-// space of every frame in the batch. XCD k walks the k-th eighth of the tiles.
+// Batched resolve: the grid walks the flattened tile space of every frame in the batch with a
+// grid stride (one tile per workgroup at the launch size batch_grid() picks). XCD k takes the
+// k-th contiguous eighth of the tiles.
 template <int TW, int TH, int PAD, bool CAS, bool S16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3))) void k_resolve_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = Geo<TW, TH, PAD, CAS>;
@@ -459,17 +460,6 @@ hipError_t launch_batch_t(const BatchEntry* e, int n, int total, int shape, int 
     return hipGetLastError();
 }
 
-template <bool CAS, bool S16>
-const void* batch_kernel_ptr(int shape) {
-    constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
-    switch (shape) {
-        case 0: return (const void*)&k_resolve_batch<S[0].tw, S[0].th, S[0].pad, CAS, S16>;
-        case 1: return (const void*)&k_resolve_batch<S[1].tw, S[1].th, S[1].pad, CAS, S16>;
-        case 2: return (const void*)&k_resolve_batch<S[2].tw, S[2].th, S[2].pad, CAS, S16>;
-        default: return (const void*)&k_resolve_batch<S[3].tw, S[3].th, S[3].pad, CAS, S16>;
-    }
-}
-
 }  // namespace
 
 int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int arith, void* stream) {
@@ -502,15 +492,5 @@ int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, in
     return (int)e;
 }
 
-int batch_grid_size(int shape, bool cas, bool s16, int arith, int device) {
-    (void)arith;
-    const void* k = cas ? (s16 ? batch_kernel_ptr<true, true>(shape) : batch_kernel_ptr<true, false>(shape))
-                        : (s16 ? batch_kernel_ptr<false, true>(shape) : batch_kernel_ptr<false, false>(shape));
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
-    int g = per_cu * cus;
-    return (g + 7) & ~7;  // whole XCD slots
-}
 
 }  // namespace taastar
