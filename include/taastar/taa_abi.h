@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define TAASTAR_ABI_VERSION 1
+#define TAASTAR_ABI_VERSION 2
 
 typedef int32_t taa_bool32; /* VkBool32: 0 = false, anything else = true */
 
@@ -115,7 +115,7 @@ typedef struct taa_uniforms {
     float        mSinTime[4];                /* 496 */
     taa_bool32   splitScreen;                /* 512 */
     int32_t      splitX;                     /* 516 */
-    taa_bool32   mUpsampling;                /* 520  must be 0: upsampling (TAAU) is not built */
+    taa_bool32   mUpsampling;                /* 520  TAAU sample selection (taa_resolve_frame only) */
     taa_bool32   mBypassHistoryUpdate;       /* 524 */
     taa_bool32   mResetHistory;              /* 528 */
     float        mCamNearPlane;              /* 532 */
@@ -151,7 +151,15 @@ typedef struct taa_frame_images {
     int32_t      pitch;          /* pixels per row of every image; 0 = width            */
     int32_t      color_format;   /* TAA_FMT_*                                           */
     int32_t      velocity_format;/* TAA_VEL_*                                           */
-    int32_t      reserved;       /* must be 0                                           */
+    /* Input (render) size of color, depth, velocity and history_depth; 0 = width / height (and
+     * input_pitch 0 = pitch when input_width is 0, else input_width). A smaller input is upsampled to width x height (TAAU, taa.comp:222-257, 724, 750-755):
+     * the neighbourhood comes from the input around int(uv * input size), depth and velocity are
+     * sampled at the output texel's uv, and with uniforms.mUpsampling the current sample is the
+     * input sample whose jittered position falls into the output texel. */
+    int32_t      input_width;
+    int32_t      input_height;
+    int32_t      input_pitch;
+    int32_t      reserved[2];    /* must be 0                                           */
 } taa_frame_images_t;
 
 /* word indices of the optional inspect output (int32 unless marked float) */
@@ -216,8 +224,9 @@ int32_t taa_jitter_offset(const taa_jitter_config_t* cfg, uint64_t frame_id, uin
  * NULL, the CAS pass of source/taa.hpp:1127-1134 fused into the same kernel: result_screen then
  * receives the sharpened screen, exactly what the reference's separate pass would write.
  * Stream-ordered and asynchronous; allocates nothing.
- * Not built (TAA_ERR_UNSUPPORTED): mUpsampling, mRayTraceAugment. A split screen with CAS runs
- * as resolve + separate CAS pass (still one call). */
+ * An input size different from the output size, or mUpsampling, takes the upsampling path (exact
+ * FP32 arithmetic whatever `arith` asks). Not built (TAA_ERR_UNSUPPORTED): mRayTraceAugment; a
+ * split screen together with CAS (resolve without CAS, then taa_cas_apply). */
 int32_t taa_resolve_frame(const taa_uniforms_t* uniforms, const taa_frame_images_t* images,
                           const taa_cas_consts_t* cas, int32_t arith, void* stream);
 
@@ -264,6 +273,7 @@ int32_t taa_session_io(taa_session_t* s, taa_session_io_t* out);
 /* Resolve the frame in the session's input slots. The first frame of a session, and the first
  * after taa_session_reset, runs with mResetHistory forced on (the reference instead skips TAA
  * on its very first frame, source/taa.hpp:989-990). */
+/* Sessions and batches resolve at one size: mUpsampling there is TAA_ERR_UNSUPPORTED. */
 int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* uniforms,
                             const taa_cas_consts_t* cas, void* result_screen, int32_t arith,
                             void* stream);

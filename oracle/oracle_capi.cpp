@@ -26,12 +26,14 @@
 namespace {
 
 template <class T>
-int resolve_frame(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+int resolve_frame(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, const float* color, const float* depth,
                   const float* vel4, const T* hist, const float* hdepth, T* screen, T* hist_out,
                   T* debug, int32_t* inspect, int cr16, int threads) {
-    if (!U || W <= 0 || H <= 0 || !color || !depth || !vel4 || !hist || !hdepth || !screen || !hist_out) return -1;
+    if (!U || W <= 0 || H <= 0 || Wl <= 0 || Hl <= 0 || !color || !depth || !vel4 || !hist || !hdepth || !screen || !hist_out)
+        return -1;
     oracle::Frame<T> F;
-    F.W = W; F.H = H; F.color = color; F.depth = depth; F.vel = vel4; F.hist = hist; F.hdepth = hdepth;
+    F.W = W; F.H = H; F.Wl = Wl; F.Hl = Hl;
+    F.color = color; F.depth = depth; F.vel = vel4; F.hist = hist; F.hdepth = hdepth;
     if (threads <= 0) threads = 1;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
     for (int y = 0; y < H; ++y) {
@@ -76,16 +78,19 @@ int cas_frame(const T* src, T* dst, int W, int H, uint32_t const1_x, int threads
 
 extern "C" {
 
-int oracle_resolve_f32(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+// W x H: output (history, screen) size; Wl x Hl: input (colour, depth, velocity, history depth)
+int oracle_resolve_f32(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, const float* color, const float* depth,
                        const float* vel4, const float* hist, const float* hdepth, float* screen,
                        float* hist_out, float* debug, int32_t* inspect, int cr16, int threads) {
-    return resolve_frame<float>(U, W, H, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, cr16, threads);
+    return resolve_frame<float>(U, W, H, Wl, Hl, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, cr16,
+                                threads);
 }
 
-int oracle_resolve_f64(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+int oracle_resolve_f64(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, const float* color, const float* depth,
                        const float* vel4, const double* hist, const float* hdepth, double* screen,
                        double* hist_out, double* debug, int32_t* inspect, int cr16, int threads) {
-    return resolve_frame<double>(U, W, H, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, cr16, threads);
+    return resolve_frame<double>(U, W, H, Wl, Hl, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, cr16,
+                                 threads);
 }
 
 int oracle_cas_f32(const float* src, float* dst, int W, int H, uint32_t const1_x, int threads) {

@@ -69,20 +69,22 @@ inline int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v
 // ------------------------------------------------------------------------------------------
 // one frame's images, decoded to float (storage rounding is applied by the caller)
 template <class T> struct Frame {
-    int W = 0, H = 0;
+    int W = 0, H = 0;               // output size: uHistoryFrame, uResultScreen, uResultHistory
+    int Wl = 0, Hl = 0;             // input size: uCurrentFrame, uCurrentDepth, uCurrentVelocity,
+                                    // uHistoryDepth (the render resolution; == W, H without TAAU)
     const float* color = nullptr;   // RGBA  uCurrentFrame
     const float* depth = nullptr;   // D     uCurrentDepth
     const float* vel = nullptr;     // RGBA  uCurrentVelocity (RG formats expanded to r,g,0,1)
     const T* hist = nullptr;        // RGBA  uHistoryFrame
     const float* hdepth = nullptr;  // D     uHistoryDepth
     v4<T> color_at(int x, int y) const {
-        x = iclamp(x, 0, W - 1); y = iclamp(y, 0, H - 1);
-        const float* p = color + 4 * ((size_t)y * W + x);
+        x = iclamp(x, 0, Wl - 1); y = iclamp(y, 0, Hl - 1);
+        const float* p = color + 4 * ((size_t)y * Wl + x);
         return {T(p[0]), T(p[1]), T(p[2]), T(p[3])};
     }
     v4<T> vel_at(int x, int y) const {
-        x = iclamp(x, 0, W - 1); y = iclamp(y, 0, H - 1);
-        const float* p = vel + 4 * ((size_t)y * W + x);
+        x = iclamp(x, 0, Wl - 1); y = iclamp(y, 0, Hl - 1);
+        const float* p = vel + 4 * ((size_t)y * Wl + x);
         return {T(p[0]), T(p[1]), T(p[2]), T(p[3])};
     }
     v4<T> hist_at(int x, int y) const {
@@ -91,12 +93,12 @@ template <class T> struct Frame {
         return {p[0], p[1], p[2], p[3]};
     }
     T depth_at(int x, int y) const {
-        x = iclamp(x, 0, W - 1); y = iclamp(y, 0, H - 1);
-        return T(depth[(size_t)y * W + x]);
+        x = iclamp(x, 0, Wl - 1); y = iclamp(y, 0, Hl - 1);
+        return T(depth[(size_t)y * Wl + x]);
     }
     T hdepth_at(int x, int y) const {
-        x = iclamp(x, 0, W - 1); y = iclamp(y, 0, H - 1);
-        return T(hdepth[(size_t)y * W + x]);
+        x = iclamp(x, 0, Wl - 1); y = iclamp(y, 0, Hl - 1);
+        return T(hdepth[(size_t)y * Wl + x]);
     }
 };
 
@@ -170,10 +172,10 @@ template <class T> inline v3<T> xyz(v4<T> v) { return {v.x, v.y, v.z}; }
 template <class T>
 inline v3<T> current_tap(const Frame<T>& F, iv2 iuv, int dx, int dy, bool unjitter, v2<T> jitter_uv) {
     if (!unjitter) return xyz(F.color_at(iuv.x + dx, iuv.y + dy));
-    T invx = T(1) / T(F.W), invy = T(1) / T(F.H);
+    T invx = T(1) / T(F.Wl), invy = T(1) / T(F.Hl);
     v2<T> uv = {jitter_uv.x + (T(iuv.x + dx) + T(0.5)) * invx, jitter_uv.y + (T(iuv.y + dy) + T(0.5)) * invy};
     auto fetch = [&](int x, int y) { return F.color_at(x, y); };
-    return xyz(bilinear<T>(fetch, F.W, F.H, uv));
+    return xyz(bilinear<T>(fetch, F.Wl, F.Hl, uv));
 }
 
 // taa.comp:199-213 (getNeighbourhood) + taa.comp:259-319 (getColorAndAabb)
@@ -349,18 +351,53 @@ template <class T> struct PixelOut {
 
 template <class T> inline void put_f(int32_t* w, int i, T v) { float f = float(v); std::memcpy(&w[i], &f, 4); }
 
-// taa.comp:707-960 for output texel iuv (no upsampling: lo-res == hi-res). `cr16` selects the
-// 16-tap Catmull-Rom form (default) instead of the 9-tap bilinear form.
+// getCurrentUpsampledColor (taa.comp:222-257): the input sample whose jittered position falls into
+// output texel currentTc after scaling, else beta = 0. The chosen sample is fetched at its texel
+// (texture() at the texel centre).
+template <class T>
+inline v3<T> current_upsampled(const taa_params_t& P, const Frame<T>& F, iv2 currentTc, v2<T> jitter_uv, T& beta) {
+    const T lx = T(F.Wl), ly = T(F.Hl), hx = T(F.W), hy = T(F.H);
+    const v2<T> scale = {hx / lx, hy / ly};
+    const v2<T> texelJitter = {jitter_uv.x * lx * T(-1), jitter_uv.y * ly * T(-1)};
+    const T almostOne = T(0.999999f);
+    v2<T> foundTc = {T(-1), T(-1)};
+    const T offs[4][2] = {{T(0), T(0)}, {almostOne, T(0)}, {T(0), almostOne}, {almostOne, almostOne}};
+    for (int k = 0; k < 4; ++k) {
+        v2<T> s = {std::floor(lx * (T(currentTc.x) + offs[k][0]) / hx) + T(0.5) + texelJitter.x,
+                   std::floor(ly * (T(currentTc.y) + offs[k][1]) / hy) + T(0.5) + texelJitter.y};
+        if (to_int(s.x * scale.x) == currentTc.x && to_int(s.y * scale.y) == currentTc.y) foundTc = s;
+    }
+    if (foundTc.x >= T(0)) {
+        beta = T(1);
+        return maybe_rgb_to_ycocg(P, tonemap_rgb(P, xyz(F.color_at(to_int(std::floor(foundTc.x)), to_int(std::floor(foundTc.y))))));
+    }
+    beta = T(0);
+    return {T(0), T(0), T(0)};
+}
+
+// taa.comp:707-960 for output texel iuv. `cr16` selects the 16-tap Catmull-Rom form (default)
+// instead of the 9-tap bilinear form. With the input (render) size equal to the output size and
+// mUpsampling off, every texture() of the current frame lands on a texel centre and is an exact
+// fetch. Otherwise (TAAU) depth and velocity are sampled at the output texel's uv with bilinear
+// weights (texture() at non-centre positions), the neighbourhood is the input's around
+// iuv_lores, and mUpsampling selects the current sample by getCurrentUpsampledColor.
 template <class T>
 inline PixelOut<T> resolve_pixel(const taa_uniforms_t& U, const Frame<T>& F, iv2 iuv, bool cr16 = true) {
     PixelOut<T> o;
     std::memset(o.inspect, 0, sizeof(o.inspect));
     o.inspect[TAA_INSPECT_CLOSEST_TAP] = -1;
     iv2 size = {F.W, F.H};
+    iv2 lsize = {F.Wl, F.Hl};
+    const bool taau = F.Wl != F.W || F.Hl != F.H || U.mUpsampling;
     v2<T> uv = tc_to_uv<T>(iuv, size);
     int paramsIdx = (U.splitScreen && iuv.x > U.splitX) ? 1 : 0;
     const taa_params_t& P = U.param[paramsIdx];
-    iv2 iuv_lores = uv_to_tc<T>(uv, size);  // == iuv without upsampling
+    iv2 iuv_lores = uv_to_tc<T>(uv, lsize);  // == iuv without upsampling
+    const v2<T> toUv = {T(1) / T(F.Wl), T(1) / T(F.Hl)};  // 1 / textureSize of depth and velocity
+    auto sample_vel = [&](v2<T> p) { return bilinear<T>([&](int x, int y) { return F.vel_at(x, y); }, F.Wl, F.Hl, p); };
+    auto sample_depth = [&](v2<T> p) {
+        return bilinear<T>([&](int x, int y) { T d = F.depth_at(x, y); return v4<T>{d, d, d, d}; }, F.Wl, F.Hl, p).x;
+    };
 
     if (P.mPassThrough) {  // taa.comp:726-731
         v3<T> c = xyz(F.color_at(iuv_lores.x, iuv_lores.y));
@@ -384,29 +421,42 @@ inline PixelOut<T> resolve_pixel(const taa_uniforms_t& U, const Frame<T>& F, iv2
     v3<T> currentColor, colMin, colMax, colClipTowards;
     getColorAndAabb(P, F, iuv_lores, jitter_uv, currentColor, colMin, colMax, colClipTowards);
     // getCurrentColor (taa.comp:216-220)
-    currentColor = maybe_rgb_to_ycocg(P, tonemap_rgb(P, current_tap(F, iuv_lores, 0, 0, P.mUnjitterCurrentSample != 0, jitter_uv)));
     T beta = T(1.0);
+    if (U.mUpsampling) currentColor = current_upsampled(P, F, iuv, jitter_uv, beta);
+    else currentColor = maybe_rgb_to_ycocg(P, tonemap_rgb(P, current_tap(F, iuv_lores, 0, 0, P.mUnjitterCurrentSample != 0, jitter_uv)));
     T depth = F.depth_at(iuv_lores.x, iuv_lores.y);  // taa.comp:756
 
     // getHistoryPosition (taa.comp:391-438)
     v2<T> historyUv;
     T expectedHistoryDepth;
     {
-        v4<T> vs = F.vel_at(iuv.x, iuv.y);
+        v4<T> vs = taau ? sample_vel(uv) : F.vel_at(iuv.x, iuv.y);
         bool canUseVelocity = true;
         if (P.mUseVelocityVectors == 0 || (P.mUseVelocityVectors == 1 && vs.w < T(0.5))) canUseVelocity = false;
         if (canUseVelocity) {
             o.inspect[TAA_INSPECT_FLAGS] |= TAA_FLAG_USED_VELOCITY;
             if (P.mVelocitySampleMode == 1) {
                 // taa.comp:398-412: the loop tracks the longest vector but assigns the last tap
-                v4<T> last = F.vel_at(iuv.x + 1, iuv.y + 1);
+                v4<T> last = taau ? sample_vel(v2<T>{uv.x + toUv.x, uv.y + toUv.y}) : F.vel_at(iuv.x + 1, iuv.y + 1);
                 vs.x = last.x;
                 vs.y = last.y;
             } else if (P.mVelocitySampleMode == 2) {
-                T dC;
-                int k = findClosestTap_3x3(F, iuv, dC);
+                int k;
+                if (taau) {  // taa.comp:371-389 at the output uv: depth taps interpolated
+                    static const int off[9][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {0, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
+                    k = 0;
+                    T dC = sample_depth(v2<T>{uv.x + toUv.x * T(-1), uv.y + toUv.y * T(-1)});
+                    for (int j = 1; j < 9; ++j) {
+                        T d = sample_depth(v2<T>{uv.x + toUv.x * T(off[j][0]), uv.y + toUv.y * T(off[j][1])});
+                        if (d < dC) { dC = d; k = j; }
+                    }
+                    vs = sample_vel(v2<T>{uv.x + toUv.x * T(off[k][0]), uv.y + toUv.y * T(off[k][1])});
+                } else {
+                    T dC;
+                    k = findClosestTap_3x3(F, iuv, dC);
+                    vs = F.vel_at(iuv.x + (k % 3) - 1, iuv.y + (k / 3) - 1);
+                }
                 o.inspect[TAA_INSPECT_CLOSEST_TAP] = k;
-                vs = F.vel_at(iuv.x + (k % 3) - 1, iuv.y + (k / 3) - 1);
             }
             historyUv = {uv.x - vs.x, uv.y - vs.y};
             expectedHistoryDepth = depth - vs.z;
@@ -459,7 +509,7 @@ inline PixelOut<T> resolve_pixel(const taa_uniforms_t& U, const Frame<T>& F, iv2
     if (P.mDynamicAntiGhosting) {
         const T eps = T(1e-5);
         auto mov = [&](int dx, int dy) {
-            v4<T> v = F.vel_at(iuv.x + dx, iuv.y + dy);
+            v4<T> v = taau ? sample_vel(v2<T>{uv.x + toUv.x * T(dx), uv.y + toUv.y * T(dy)}) : F.vel_at(iuv.x + dx, iuv.y + dy);
             v = {std::fabs(v.x), std::fabs(v.y), std::fabs(v.z), std::fabs(v.w)};
             return (v.x > eps || v.y > eps) && (v.w >= T(0.5));
         };
@@ -472,7 +522,7 @@ inline PixelOut<T> resolve_pixel(const taa_uniforms_t& U, const Frame<T>& F, iv2
         writeDynamicMask = movC ? T(1.0) : T(0.0);
     }
     if (P.mDepthCulling) {
-        iv2 dtc = uv_to_tc<T>(historyUv, size);
+        iv2 dtc = uv_to_tc<T>(historyUv, lsize);  // uHistoryDepth has the input size
         o.inspect[TAA_INSPECT_DEPTH_TC_X] = dtc.x;
         o.inspect[TAA_INSPECT_DEPTH_TC_Y] = dtc.y;
         T historyDepth = F.hdepth_at(dtc.x, dtc.y);

@@ -96,9 +96,15 @@ int make_desc(const taa_frame_images_t* img, const taa_cas_consts_t* cas, taasta
     if (pitch < img->width) return fail(TAA_ERR_INVALID_ARG, "pitch %d < width %d", pitch, img->width);
     if (!fits_offsets(pitch, img->height, 16) || (img->inspect && !fits_offsets(pitch, img->height, 4 * TAA_INSPECT_WORDS)))
         return fail(TAA_ERR_INVALID_ARG, "frame of %d x %d texels: an image would exceed 4 GiB", pitch, img->height);
+    const int wl = img->input_width == 0 ? img->width : img->input_width;
+    const int hl = img->input_height == 0 ? img->height : img->input_height;
+    const int pl = img->input_pitch != 0 ? img->input_pitch : (img->input_width == 0 ? pitch : wl);
+    if (wl <= 0 || hl <= 0 || wl > 32768 || hl > 32768 || pl < wl)
+        return fail(TAA_ERR_INVALID_ARG, "bad input size %dx%d (pitch %d)", wl, hl, pl);
+    if (!fits_offsets(pl, hl, 16)) return fail(TAA_ERR_INVALID_ARG, "input of %d x %d texels exceeds 4 GiB", pl, hl);
+    if (img->reserved[0] != 0 || img->reserved[1] != 0) return fail(TAA_ERR_INVALID_ARG, "reserved must be 0");
     if (!valid_color_fmt(img->color_format)) return fail(TAA_ERR_INVALID_ARG, "bad color_format %d", img->color_format);
     if (!valid_vel_fmt(img->velocity_format)) return fail(TAA_ERR_INVALID_ARG, "bad velocity_format %d", img->velocity_format);
-    if (img->reserved != 0) return fail(TAA_ERR_INVALID_ARG, "reserved must be 0");
     if (!img->color || !img->depth || !img->history || !img->history_depth || !img->result_screen || !img->velocity ||
         !img->result_history)
         return fail(TAA_ERR_INVALID_ARG, "a required image pointer is NULL");
@@ -126,12 +132,15 @@ int make_desc(const taa_frame_images_t* img, const taa_cas_consts_t* cas, taasta
     fd.param_idx = 0;
     fd.cas_on = cas ? 1 : 0;
     fd.cas_peak_bits = cas ? cas->const1[0] : 0u;
+    fd.Wl = wl;
+    fd.Hl = hl;
+    fd.pitch_l = pl;
+    fd.up = (wl != img->width || hl != img->height) ? 1 : 0;
     return TAA_OK;
 }
 
 int check_uniforms(const taa_uniforms_t* u) {
     if (!u) return fail(TAA_ERR_INVALID_ARG, "uniforms is NULL");
-    if (u->mUpsampling) return fail(TAA_ERR_UNSUPPORTED, "mUpsampling (TAAU) is not built");
     for (int i = 0; i < 2; ++i) {
         if (u->param[i].mRayTraceAugment) return fail(TAA_ERR_UNSUPPORTED, "mRayTraceAugment is not built");
         if (u->param[i].mInterpolationMode < 0 || u->param[i].mInterpolationMode > 2)
@@ -151,6 +160,15 @@ int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, in
     fd.param_idx = param_idx;
     fd.tiles_x = taastar::tiles_for(x_end - x_begin, s.tw);
     fd.tiles_y = taastar::tiles_for(fd.H, s.th);
+    if (fd.up || U.mUpsampling) {  // upsampling path: its own (default) tile shape
+        fd.up = 1;
+        const taastar::TileShapeDef d = shape_def(0, cas);
+        fd.tiles_x = taastar::tiles_for(x_end - x_begin, d.tw);
+        fd.tiles_y = taastar::tiles_for(fd.H, d.th);
+        const int e = taastar::launch_resolve_up(fd, U, stream);
+        if (e != 0) return hip_fail((hipError_t)e, "resolve launch");
+        return TAA_OK;
+    }
     int e = (arith == TAA_ARITH_FP16 && taastar::fast_path_supported(fd, U, param_idx))
                 ? taastar::launch_resolve_fast(fd, U, shape, stream)
                 : taastar::launch_resolve(fd, U, shape, arith, stream);
@@ -236,6 +254,10 @@ void session_desc(taa_session* s, void* screen, const taa_cas_consts_t* cas, taa
     fd.x_end = s->W;
     fd.cas_on = cas ? 1 : 0;
     fd.cas_peak_bits = cas ? cas->const1[0] : 0u;
+    fd.Wl = s->W;
+    fd.Hl = s->H;
+    fd.pitch_l = s->pitch;
+    fd.up = 0;
 }
 
 void session_advance(taa_session* s) {
@@ -431,6 +453,7 @@ int32_t taa_session_reset(taa_session_t* s) {
 int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* u, const taa_cas_consts_t* cas, void* screen,
                             int32_t arith, void* stream) {
     if (!s || !u || !screen) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
+    if (u->mUpsampling) return fail(TAA_ERR_UNSUPPORTED, "mUpsampling in a session: use taa_resolve_frame with an input size");
     taa_uniforms_t U = *u;
     if (s->force_reset) U.mResetHistory = 1;
     const int k = (int)(s->frames & 1);
@@ -472,6 +495,7 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
             if (items[j].session == it.session) return fail(TAA_ERR_INVALID_ARG, "item %d: session repeated", i);
         int rc = check_uniforms(it.uniforms);
         if (rc) return rc;
+        if (it.uniforms->mUpsampling) return fail(TAA_ERR_UNSUPPORTED, "item %d: mUpsampling in a batch", i);
         if (it.uniforms->splitScreen) return fail(TAA_ERR_UNSUPPORTED, "item %d: split screen in a batch", i);
         const int px = it.session->W * it.session->H;
         if (px > best_px) {

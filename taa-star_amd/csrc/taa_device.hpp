@@ -166,6 +166,62 @@ __device__ float4 bilinear_rgba(const void* img, int W, int H, int pitch, float 
     return make_float4(top.x * uy + bot.x * ty, top.y * uy + bot.y * ty, top.z * uy + bot.z * ty, top.w * uy + bot.w * ty);
 }
 
+// Bilinear samples of the velocity (format fmt) and of a depth image at a non-centre uv, with the
+// weights and the order of operations of bilinear_rgba: texture() of the upsampling path, where
+// the output texel's uv is not a centre of the input-sized depth and velocity images.
+__device__ float4 bilinear_vel(const void* img, int fmt, int W, int H, int pitch, float u, float v) {
+    float px = u * (float)W - 0.5f, py = v * (float)H - 0.5f;
+    float fx = floorf(px), fy = floorf(py);
+    float tx = px - fx, ty = py - fy;
+    int x0 = clampi(to_int(fx), -1, W), y0 = clampi(to_int(fy), -1, H);
+    int xa = clampi(x0, 0, W - 1), xb = clampi(x0 + 1, 0, W - 1);
+    int ya = clampi(y0, 0, H - 1), yb = clampi(y0 + 1, 0, H - 1);
+    float4 a = ld_vel(img, fmt, (uint32_t)(ya * pitch + xa)), b = ld_vel(img, fmt, (uint32_t)(ya * pitch + xb));
+    float4 c = ld_vel(img, fmt, (uint32_t)(yb * pitch + xa)), d = ld_vel(img, fmt, (uint32_t)(yb * pitch + xb));
+    float ux = 1.0f - tx, uy = 1.0f - ty;
+    float4 top = make_float4(a.x * ux + b.x * tx, a.y * ux + b.y * tx, a.z * ux + b.z * tx, a.w * ux + b.w * tx);
+    float4 bot = make_float4(c.x * ux + d.x * tx, c.y * ux + d.y * tx, c.z * ux + d.z * tx, c.w * ux + d.w * tx);
+    return make_float4(top.x * uy + bot.x * ty, top.y * uy + bot.y * ty, top.z * uy + bot.z * ty, top.w * uy + bot.w * ty);
+}
+__device__ float bilinear_f32(const float* img, int W, int H, int pitch, float u, float v) {
+    float px = u * (float)W - 0.5f, py = v * (float)H - 0.5f;
+    float fx = floorf(px), fy = floorf(py);
+    float tx = px - fx, ty = py - fy;
+    int x0 = clampi(to_int(fx), -1, W), y0 = clampi(to_int(fy), -1, H);
+    int xa = clampi(x0, 0, W - 1), xb = clampi(x0 + 1, 0, W - 1);
+    int ya = clampi(y0, 0, H - 1), yb = clampi(y0 + 1, 0, H - 1);
+    float a = ld_f32(img, (uint32_t)(ya * pitch + xa)), b = ld_f32(img, (uint32_t)(ya * pitch + xb));
+    float c = ld_f32(img, (uint32_t)(yb * pitch + xa)), d = ld_f32(img, (uint32_t)(yb * pitch + xb));
+    float ux = 1.0f - tx, uy = 1.0f - ty;
+    return (a * ux + b * tx) * uy + (c * ux + d * tx) * ty;
+}
+
+// getCurrentUpsampledColor (taa.comp:222-257): the input sample whose jittered position lands in
+// output texel (tcx, tcy) after scaling, fetched at its texel; beta = 0 when there is none.
+template <bool S16>
+__device__ f3 current_upsampled(const FrameDesc& fd, const taa_params_t& P, int tcx, int tcy, float jux, float juy,
+                                float& beta) {
+    const float lx = (float)fd.Wl, ly = (float)fd.Hl, hx = (float)fd.W, hy = (float)fd.H;
+    const float scx = hx / lx, scy = hy / ly;
+    const float tjx = jux * lx * -1.0f, tjy = juy * ly * -1.0f;
+    const float a1 = 0.999999f;
+    const float ox[4] = {0.0f, a1, 0.0f, a1}, oy[4] = {0.0f, 0.0f, a1, a1};
+    float fx = -1.0f, fy = -1.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float sx = floorf(lx * ((float)tcx + ox[k]) / hx) + 0.5f + tjx;
+        const float sy = floorf(ly * ((float)tcy + oy[k]) / hy) + 0.5f + tjy;
+        if (to_int(sx * scx) == tcx && to_int(sy * scy) == tcy) { fx = sx; fy = sy; }
+    }
+    if (fx >= 0.0f) {
+        beta = 1.0f;
+        const int ix = clampi(to_int(floorf(fx)), 0, fd.Wl - 1), iy = clampi(to_int(floorf(fy)), 0, fd.Hl - 1);
+        return stage_convert(P, xyz(ld_rgba<S16>(fd.color, (uint32_t)(iy * fd.pitch_l + ix))));
+    }
+    beta = 0.0f;
+    return f3{0.0f, 0.0f, 0.0f};
+}
+
 // Catmull-Rom history fetch (taa.comp:441-514), 16-tap texelFetch form with the edge clamp
 // at size-1; identical to the 9-tap bilinear form up to rounding.
 template <bool S16>

@@ -528,7 +528,7 @@ hipError_t launch_fast_batch_t(const BatchEntry* e, int n, int total, int shape,
 bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx) {
     const taa_params_t& P = U.param[param_idx];
     if (fd.vel_fmt != (fd.color_fmt == TAA_FMT_RGBA16F ? TAA_VEL_RG16F : TAA_VEL_RG32F)) return false;
-    if (fd.debug) return false;
+    if (fd.debug || fd.up || U.mUpsampling) return false;
     if (U.mBypassHistoryUpdate || P.mPassThrough || P.mToneMapLumaKaris || P.mUnjitterNeighbourhood ||
         P.mUnjitterCurrentSample)
         return false;

@@ -28,7 +28,8 @@ struct FrameDesc {
     int32_t param_idx;        // which Parameters block of the uniforms applies
     int32_t cas_on;
     uint32_t cas_peak_bits;   // const1[0] of CasSetup
-    int32_t pad0;
+    int32_t Wl, Hl, pitch_l;  // input (render) size of colour, depth, velocity, previous depth
+    int32_t up;               // upsampling path: input size != output size, or mUpsampling
 };
 
 // One frame of a batched launch: frame + device copy of its uniforms + first flattened tile.
@@ -36,7 +37,7 @@ struct BatchEntry {
     FrameDesc fd;
     taa_uniforms_t U;         // the frame's uniforms, inline so the kernel reads them with scalar loads
     int32_t tile_begin;       // prefix sum of tiles of the entries before this one
-    int32_t pad[3];
+    int32_t pad;
 };
 
 // Tile shapes the kernels are instantiated for (output tile, 256 threads per workgroup).
@@ -55,6 +56,8 @@ inline int tiles_for(int extent, int t) { return (extent + t - 1) / t; }
 
 // Launchers (taa_kernels.hip). All return hipError_t as int.
 int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int arith, void* stream);
+// the upsampling (TAAU) path: exact FP32 arithmetic, default tile shape
+int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, void* stream);
 int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uint32_t peak_bits, void* stream);
 // Workgroups of a batched launch: one per tile, rounded to whole XCD slots. The batch kernels walk
 // the tile space with a grid stride, so a smaller (persistent) grid is also correct; measured on

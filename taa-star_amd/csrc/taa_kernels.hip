@@ -27,17 +27,17 @@ namespace {
 // ------------------------------------------------------------------------------------------
 // tile geometry and LDS
 
-template <int TW_, int TH_, int PAD_, bool CAS_>
+template <int TW_, int TH_, int PAD_, bool CAS_, bool UP_ = false>
 struct Geo {
     static constexpr int TW = TW_, TH = TH_, PAD = PAD_;
-    static constexpr bool CAS = CAS_;
+    static constexpr bool CAS = CAS_, UP = UP_;  // UP: upsampling path, inputs read from global memory
     static constexpr int NT = 256;
     static constexpr int A = CAS ? 2 : 1;  // input apron
     static constexpr int SW = TW + 2 * A, SH = TH + 2 * A, SP = SW + PAD;
     static constexpr int RA = CAS ? 1 : 0;  // resolve apron (CAS needs the resolved cross)
     static constexpr int RW = TW + 2 * RA, RH = TH + 2 * RA, RP = RW + PAD;
     struct Smem {
-        float c0[SH * SP], c1[SH * SP], c2[SH * SP], d[SH * SP];
+        float c0[UP ? 1 : SH * SP], c1[UP ? 1 : SH * SP], c2[UP ? 1 : SH * SP], d[UP ? 1 : SH * SP];
         float r0[CAS ? RH * RP : 1], r1[CAS ? RH * RP : 1], r2[CAS ? RH * RP : 1];
     };
 };
@@ -49,7 +49,7 @@ struct PixelResult {
 
 // One output texel of taa.comp main (taa.comp:707-960), FP32 arithmetic. `l` indexes the texel
 // in the staged LDS planes (row pitch SP).
-template <bool S16, int SP>
+template <bool S16, int SP, bool UP>
 __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
                                          int x, int y, int l, const float* __restrict__ s0,
                                          const float* __restrict__ s1, const float* __restrict__ s2,
@@ -58,10 +58,19 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     const uint32_t gi = (uint32_t)(y * fd.pitch + x);
     PixelResult R;
     const float uvx = ((float)x + 0.5f) / (float)W, uvy = ((float)y + 0.5f) / (float)H;  // taa.comp:131
+    // input (render) resolution: iuv_lores = int(uv * input size) (taa.comp:724); == (x, y) at one size
+    const int Wl = fd.Wl, Hl = fd.Hl, pl = fd.pitch_l;
+    int xl = x, yl = y;
+    if constexpr (UP) {
+        xl = to_int(uvx * (float)Wl);
+        yl = to_int(uvy * (float)Hl);
+    }
+    auto lo = [&](int xx, int yy) { return (uint32_t)(clampi(yy, 0, Hl - 1) * pl + clampi(xx, 0, Wl - 1)); };
+    const float toUx = 1.0f / (float)Wl, toUy = 1.0f / (float)Hl;  // 1 / textureSize(depth, velocity)
 
     if (P.mPassThrough || U.mBypassHistoryUpdate) {  // taa.comp:726-737
         float4 h = ld_rgba<S16>(fd.history, gi);
-        f3 s = P.mPassThrough ? xyz(ld_rgba<S16>(fd.color, gi)) : xyz(h);
+        f3 s = P.mPassThrough ? xyz(ld_rgba<S16>(fd.color, lo(xl, yl))) : xyz(h);
         R.screen = s;
         R.history = make_float4(h.x, h.y, h.z, 1.0f);
         if (write_aux && fd.debug) st_rgba16(fd.debug, gi, make_float4(0.f, 0.f, 0.f, 0.f));
@@ -82,14 +91,18 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     if (!P.mUnjitterNeighbourhood) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            int li = l + oy[k] * SP + ox[k];
-            c[k] = {s0[li], s1[li], s2[li]};
+            if constexpr (UP) {
+                c[k] = stage_convert(P, xyz(ld_rgba<S16>(fd.color, lo(xl + ox[k], yl + oy[k]))));
+            } else {
+                int li = l + oy[k] * SP + ox[k];
+                c[k] = {s0[li], s1[li], s2[li]};
+            }
         }
     } else {
-        const float ix = 1.0f / (float)W, iy = 1.0f / (float)H;
+        const float ix = 1.0f / (float)Wl, iy = 1.0f / (float)Hl;
         for (int k = 0; k < 9; ++k) {
-            float u = jux + ((float)(x + ox[k]) + 0.5f) * ix, v = juy + ((float)(y + oy[k]) + 0.5f) * iy;
-            c[k] = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, W, H, fd.pitch, u, v)));
+            float u = jux + ((float)(xl + ox[k]) + 0.5f) * ix, v = juy + ((float)(yl + oy[k]) + 0.5f) * iy;
+            c[k] = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, Wl, Hl, pl, u, v)));
         }
     }
     f3 colMin, colMax, clipTowards;
@@ -127,39 +140,64 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
                    clipTowards.x > colMax.x || clipTowards.y > colMax.y || clipTowards.z > colMax.z;
         if (out) clipTowards = center;
     }
-    // getCurrentColor (taa.comp:216-220)
+    // getCurrentColor (taa.comp:216-220), or getCurrentUpsampledColor (taa.comp:222-257, 750-755)
     f3 currentColor = c[0];
-    if (P.mUnjitterCurrentSample) {
-        float u = jux + ((float)x + 0.5f) * (1.0f / (float)W), v = juy + ((float)y + 0.5f) * (1.0f / (float)H);
-        currentColor = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, W, H, fd.pitch, u, v)));
+    float beta = 1.0f;
+    bool upsampled = false;
+    if constexpr (UP) {
+        if (U.mUpsampling) {
+            currentColor = current_upsampled<S16>(fd, P, x, y, jux, juy, beta);
+            upsampled = true;
+        }
     }
-    const float depth = sd[l];  // taa.comp:756
+    if (!upsampled && P.mUnjitterCurrentSample) {
+        float u = jux + ((float)xl + 0.5f) * (1.0f / (float)Wl), v = juy + ((float)yl + 0.5f) * (1.0f / (float)Hl);
+        currentColor = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, Wl, Hl, pl, u, v)));
+    }
+    float depth;  // taa.comp:756
+    if constexpr (UP) depth = ld_f32(fd.depth, lo(xl, yl));
+    else depth = sd[l];
 
     // ---- getHistoryPosition (taa.comp:391-438)
     int flags = 0, closest = -1;
     float hu, hv, expectedHistoryDepth;
     {
-        float4 vs = ld_vel(fd.velocity, fd.vel_fmt, gi);
+        // SAMPLE_TEX(uCurrentVelocity, uv): a texel fetch at one size, bilinear when upsampling
+        auto vel_at_uv = [&](float u, float v) { return bilinear_vel(fd.velocity, fd.vel_fmt, Wl, Hl, pl, u, v); };
+        float4 vs;
+        if constexpr (UP) vs = vel_at_uv(uvx, uvy);
+        else vs = ld_vel(fd.velocity, fd.vel_fmt, gi);
         bool canUseVelocity = !(P.mUseVelocityVectors == 0 || (P.mUseVelocityVectors == 1 && vs.w < 0.5f));
         if (canUseVelocity) {
             flags |= TAA_FLAG_USED_VELOCITY;
             if (P.mVelocitySampleMode == 1) {
                 // taa.comp:398-412: the longest vector is tracked but the last tap is assigned
-                float4 last = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1)));
+                float4 last;
+                if constexpr (UP) last = vel_at_uv(uvx + toUx, uvy + toUy);
+                else last = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1)));
                 vs.x = last.x;
                 vs.y = last.y;
             } else if (P.mVelocitySampleMode == 2) {
                 // findClosestUvAndZ_3x3 (taa.comp:371-389): scan order row-major, strict <
-                float dC = sd[l - SP - 1];
                 int best = 0;
+                if constexpr (UP) {  // depth taps at the output uv, interpolated
+                    float dC = bilinear_f32(fd.depth, Wl, Hl, pl, uvx + toUx * -1.0f, uvy + toUy * -1.0f);
+                    for (int k = 1; k < 9; ++k) {
+                        float d = bilinear_f32(fd.depth, Wl, Hl, pl, uvx + toUx * (float)(k % 3 - 1), uvy + toUy * (float)(k / 3 - 1));
+                        if (d < dC) { dC = d; best = k; }
+                    }
+                    vs = vel_at_uv(uvx + toUx * (float)(best % 3 - 1), uvy + toUy * (float)(best / 3 - 1));
+                } else {
+                    float dC = sd[l - SP - 1];
 #pragma unroll
-                for (int k = 1; k < 9; ++k) {
-                    float d = sd[l + (k / 3 - 1) * SP + (k % 3 - 1)];
-                    if (d < dC) { dC = d; best = k; }
+                    for (int k = 1; k < 9; ++k) {
+                        float d = sd[l + (k / 3 - 1) * SP + (k % 3 - 1)];
+                        if (d < dC) { dC = d; best = k; }
+                    }
+                    int vx = clampi(x + best % 3 - 1, 0, W - 1), vy = clampi(y + best / 3 - 1, 0, H - 1);
+                    vs = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(vy * fd.pitch + vx));
                 }
                 closest = best;
-                int vx = clampi(x + best % 3 - 1, 0, W - 1), vy = clampi(y + best / 3 - 1, 0, H - 1);
-                vs = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(vy * fd.pitch + vx));
             }
             hu = uvx - vs.x;
             hv = uvy - vs.y;
@@ -188,7 +226,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     else historyRaw = history_catmullrom<S16>(fd, hu, hv);
     f3 historyColor = xyz(historyRaw);
     if (P.mUseYCoCg) historyColor = rgb_to_ycocg(historyColor);
-    float alpha = P.mAlpha, beta = 1.0f;
+    float alpha = P.mAlpha;
     bool rejected = false;
 
     // ---- rejection (taa.comp:787-823)
@@ -203,7 +241,9 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     if (P.mDynamicAntiGhosting) {
         const float eps = 1e-5f;
         auto mov = [&](int dx, int dy) {
-            float4 v = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1)));
+            float4 v;
+            if constexpr (UP) v = bilinear_vel(fd.velocity, fd.vel_fmt, Wl, Hl, pl, uvx + toUx * (float)dx, uvy + toUy * (float)dy);
+            else v = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1)));
             return (fabsf(v.x) > eps || fabsf(v.y) > eps) && (fabsf(v.w) >= 0.5f);
         };
         bool movL = mov(-1, 0), movR = mov(1, 0), movT = mov(0, -1), movB = mov(0, 1), movC = mov(0, 0);
@@ -215,9 +255,9 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     }
     int dtx = 0, dty = 0;
     if (P.mDepthCulling) {
-        dtx = to_int(hu * (float)W);
-        dty = to_int(hv * (float)H);
-        float historyDepth = ld_f32(fd.hdepth, (uint32_t)(clampi(dty, 0, H - 1) * fd.pitch + clampi(dtx, 0, W - 1)));
+        dtx = to_int(hu * (float)Wl);  // uHistoryDepth has the input size (taa.comp:818)
+        dty = to_int(hv * (float)Hl);
+        float historyDepth = ld_f32(fd.hdepth, lo(dtx, dty));
         bool rej;
         if (U.pad1 > 0.0f) {
             float n = U.mCamNearPlane, f = U.mCamFarPlane;
@@ -283,7 +323,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
             case 1: { f3 t = colMax - colMin; float p = t.x * t.y * t.z; g = make_float4(p, p, p, 0.f); break; }
             case 2: g = make_float4(rejected ? 1.f : 0.f, sqrtf(rdiff.x * rdiff.x + rdiff.y * rdiff.y + rdiff.z * rdiff.z), 0.f, 0.f); break;
             case 3: g = make_float4(alpha, alpha, alpha, 0.f); break;
-            case 4: g = ld_vel(fd.velocity, fd.vel_fmt, gi); break;
+            case 4: g = ld_vel(fd.velocity, fd.vel_fmt, lo(x, y)); break;
             case 5: g = make_float4(pixelSpeed, 0.f, 0.f, 0.f); break;
             case 6: g = make_float4(R.screen.x, R.screen.y, R.screen.z, 1.0f); break;
             case 7: g = R.history; break;
@@ -332,8 +372,9 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
     const int gx0 = x0 - G::A, gy0 = y0 - G::A;  // image coordinates of LDS texel (0,0)
     const int tid = threadIdx.x;
 
-    // 1. stage colour (converted once per texel) and depth, clamped to the edge
-    for (int i = tid; i < G::SW * G::SH; i += G::NT) {
+    // 1. stage colour (converted once per texel) and depth, clamped to the edge (one size only:
+    // the upsampling path reads its inputs where the output texel maps to)
+    for (int i = tid; i < (G::UP ? 0 : G::SW * G::SH); i += G::NT) {
         const int sy = i / G::SW, sx = i - sy * G::SW;
         const int gx = clampi(gx0 + sx, 0, W - 1), gy = clampi(gy0 + sy, 0, H - 1);
         const uint32_t gi = (uint32_t)(gy * pitch + gx);
@@ -353,7 +394,7 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         const int cx = clampi(px, 0, W - 1), cy = clampi(py, 0, H - 1);
         const bool owned = rx >= G::RA && rx < G::RA + G::TW && ry >= G::RA && ry < G::RA + G::TH && px < fd.x_end && py < H;
         const int l = (cy - gy0) * G::SP + (cx - gx0);
-        PixelResult r = resolve_pixel_f32<S16, G::SP>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, owned);
+        PixelResult r = resolve_pixel_f32<S16, G::SP, G::UP>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, owned);
         if (owned) {
             const uint32_t gi = (uint32_t)(cy * pitch + cx);
             st_rgba<S16>(fd.history_out, gi, r.history);
@@ -394,6 +435,16 @@ struct ResolveArgs {
 template <int TW, int TH, int PAD, bool CAS, bool S16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3))) void k_resolve(const ResolveArgs a) {
     using G = Geo<TW, TH, PAD, CAS>;
+    __shared__ typename G::Smem sm;
+    const int n = a.fd.tiles_x * a.fd.tiles_y;
+    resolve_tile<G, S16>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
+}
+
+// Upsampling path (input size != output size, or mUpsampling): exact FP32 arithmetic, inputs read
+// from global memory around each output texel's input position, fused CAS as above.
+template <int TW, int TH, int PAD, bool CAS, bool S16>
+__global__ __launch_bounds__(256) void k_resolve_up(const ResolveArgs a) {
+    using G = Geo<TW, TH, PAD, CAS, true>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
     resolve_tile<G, S16>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
@@ -470,6 +521,24 @@ int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int 
     if (fd.cas_on) e = s16 ? launch_resolve_t<true, true>(fd, U, shape, st) : launch_resolve_t<true, false>(fd, U, shape, st);
     else e = s16 ? launch_resolve_t<false, true>(fd, U, shape, st) : launch_resolve_t<false, false>(fd, U, shape, st);
     return (int)e;
+}
+
+int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    ResolveArgs a;
+    a.fd = fd;
+    a.U = U;
+    const int n = fd.tiles_x * fd.tiles_y;
+    const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
+    constexpr TileShapeDef C = kShapesCas[0], Pl = kShapesPlain[0];
+    if (fd.cas_on) {
+        if (s16) hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, true>), dim3(n), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, false>), dim3(n), dim3(256), 0, st, a);
+    } else {
+        if (s16) hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, true>), dim3(n), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, false>), dim3(n), dim3(256), 0, st, a);
+    }
+    return (int)hipGetLastError();
 }
 
 int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uint32_t peak_bits, void* stream) {

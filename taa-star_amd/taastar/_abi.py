@@ -66,7 +66,8 @@ class FrameImages(C.Structure):
         ("result_screen", C.c_void_p), ("debug", C.c_void_p), ("velocity", C.c_void_p),
         ("result_history", C.c_void_p), ("inspect", C.c_void_p),
         ("width", C.c_int32), ("height", C.c_int32), ("pitch", C.c_int32), ("color_format", C.c_int32),
-        ("velocity_format", C.c_int32), ("reserved", C.c_int32),
+        ("velocity_format", C.c_int32), ("input_width", C.c_int32), ("input_height", C.c_int32),
+        ("input_pitch", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 

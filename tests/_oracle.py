@@ -18,8 +18,8 @@ def lib():
             raise RuntimeError("oracle/liboracle.so is missing: run `make -C oracle`")
         L = C.CDLL(LIB)
         fp, dp, ip = C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int32)
-        L.oracle_resolve_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp, fp, fp, ip, C.c_int, C.c_int]
-        L.oracle_resolve_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, fp, fp, fp, dp, fp, dp, dp, dp, ip, C.c_int, C.c_int]
+        L.oracle_resolve_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp, fp, fp, ip, C.c_int, C.c_int]
+        L.oracle_resolve_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp, dp, fp, dp, dp, dp, ip, C.c_int, C.c_int]
         L.oracle_cas_f32.argtypes = [fp, fp, C.c_int, C.c_int, C.c_uint32, C.c_int]
         L.oracle_cas_f64.argtypes = [dp, dp, C.c_int, C.c_int, C.c_uint32, C.c_int]
         L.oracle_cas_setup.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)] + [C.c_float] * 5
@@ -50,8 +50,10 @@ def _p(a, t):
 
 def resolve(U, color4, depth, vel4, hist4, hdepth, dtype=np.float32, cr16=True, threads=0, debug=True, inspect=True):
     """One frame of the oracle. Inputs are float32 arrays as the resolve reads them (vel4 already
-    expanded to 4 channels); hist4 in `dtype`. Returns dict(screen, history, debug, inspect)."""
-    H, W = depth.shape
+    expanded to 4 channels); hist4 in `dtype`. The output size is the history's, the input size the
+    colour's (they differ for TAAU). Returns dict(screen, history, debug, inspect)."""
+    Hl, Wl = depth.shape
+    H, W = hist4.shape[:2]
     threads = threads or os.cpu_count() or 1
     f = np.float32
     color4 = np.ascontiguousarray(color4, f)
@@ -65,7 +67,7 @@ def resolve(U, color4, depth, vel4, hist4, hdepth, dtype=np.float32, cr16=True, 
     ins = np.empty((H, W, 16), np.int32) if inspect else None
     T = C.c_float if dtype == np.float32 else C.c_double
     fn = lib().oracle_resolve_f32 if dtype == np.float32 else lib().oracle_resolve_f64
-    rc = fn(C.byref(U), W, H, _p(color4, C.c_float), _p(depth, C.c_float), _p(vel4, C.c_float), _p(hist4, T),
+    rc = fn(C.byref(U), W, H, Wl, Hl, _p(color4, C.c_float), _p(depth, C.c_float), _p(vel4, C.c_float), _p(hist4, T),
             _p(hdepth, C.c_float), _p(scr, T), _p(his, T), _p(dbg, T) if debug else None,
             _p(ins, C.c_int32) if inspect else None, int(cr16), threads)
     assert rc == 0

@@ -47,7 +47,7 @@ def test_struct_layouts_match_reference():
 
 
 def test_abi_version(taastar_lib):
-    assert taastar_lib.taa_abi_version() == 1
+    assert taastar_lib.taa_abi_version() == 2
 
 
 def test_param_defaults_match_reference():
@@ -125,9 +125,6 @@ def test_argument_errors_need_no_gpu():
     # NULLs, bad sizes and unsupported features are refused before anything touches a device
     assert lib.taa_resolve_frame(None, C.byref(img), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
     assert lib.taa_resolve_frame(C.byref(U), C.byref(img), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
-    U.mUpsampling = 1
-    assert lib.taa_resolve_frame(C.byref(U), C.byref(img), None, 0, None) == _abi.TAA_ERR_UNSUPPORTED
-    U.mUpsampling = 0
     U.param[0].mRayTraceAugment = 1
     assert lib.taa_resolve_frame(C.byref(U), C.byref(img), None, 0, None) == _abi.TAA_ERR_UNSUPPORTED
     assert b"mRayTraceAugment" in lib.taa_last_error()
@@ -140,6 +137,16 @@ def test_argument_errors_need_no_gpu():
     big.width, big.height, big.pitch = 16384, 16384, 16384
     assert lib.taa_resolve_frame(C.byref(U), C.byref(big), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
     assert b"4 GiB" in lib.taa_last_error()
+    # a bad input (render) size and non-zero reserved words are refused as well
+    big.width, big.height, big.pitch = 64, 32, 64
+    big.input_width, big.input_height = -3, 16
+    assert lib.taa_resolve_frame(C.byref(U), C.byref(big), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
+    big.input_width, big.input_height, big.input_pitch = 32, 16, 8
+    assert lib.taa_resolve_frame(C.byref(U), C.byref(big), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
+    big.input_pitch = 0
+    big.reserved[1] = 5
+    assert lib.taa_resolve_frame(C.byref(U), C.byref(big), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
+    assert b"reserved" in lib.taa_last_error()
     assert lib.taa_session_close(None) == _abi.TAA_ERR_INVALID_ARG
 
 

@@ -1,0 +1,137 @@
+"""The upsampling (TAAU) path on the GPU against the FP32 transcription: input (render) size
+different from the output size, and mUpsampling (taa.comp:222-257, 724, 750-755, 818).
+
+Bar: bit-exact like the FP32 path at one size — texel indices, depth-test texels, closest taps,
+flags and the neighbourhood box identical; history and screen within 1e-5 relative (observed 0).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from gpu_helpers import assert_parity, round_storage, to_f16  # noqa: E402
+from taastar import (ARITH_FP16, ARITH_FP32, FMT_RGBA16F, FMT_RGBA32F, INSPECT_WORDS, VEL_RG16F, VEL_RG32F,  # noqa: E402
+                     VEL_RGBA32F, FrameImages, check, lib)
+from taastar.device import color_dtype, vel_shape_dtype  # noqa: E402
+from taastar.workloads import (cas_for, decode, encode_color, encode_velocity, params_c2, scene_for,  # noqa: E402
+                               uniforms_for, velocity_as_read, workload)
+
+
+@pytest.fixture(autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+class Upsampler:
+    """A render-resolution scene resolved into an output-resolution history, GPU and oracle."""
+
+    def __init__(self, w, lo, hi, cas=True, upsampling=True, params=None, arith=ARITH_FP32):
+        self.w, self.fmt, self.vfmt = w, w["fmt"], w["vel"]
+        (self.Wl, self.Hl), (self.W, self.H) = lo, hi
+        self.scene = scene_for(w, width=self.Wl, height=self.Hl)
+        self.cas = cas_for(w, self.W, self.H) if cas else None
+        self.upsampling, self.params, self.arith = upsampling, params, arith
+        cd = color_dtype(self.fmt)
+        vc, vd = vel_shape_dtype(self.vfmt)
+        z = lambda *s, dtype: torch.zeros(*s, dtype=dtype, device="cuda")
+        self.color, self.depth = z(self.Hl, self.Wl, 4, dtype=cd), z(self.Hl, self.Wl, dtype=torch.float32)
+        self.vel, self.prev_depth = z(self.Hl, self.Wl, vc, dtype=vd), z(self.Hl, self.Wl, dtype=torch.float32)
+        self.hist = [z(self.H, self.W, 4, dtype=cd), z(self.H, self.W, 4, dtype=cd)]
+        self.screen = z(self.H, self.W, 4, dtype=cd)
+        self.debug = z(self.H, self.W, 4, dtype=torch.float16)
+        self.inspect = z(self.H, self.W, INSPECT_WORDS, dtype=torch.int32)
+        self.hist_o = np.zeros((self.H, self.W, 4), np.float32)
+        self.prev_o = np.zeros((self.Hl, self.Wl), np.float32)
+        self.k = 0
+
+    def step(self, oracle):
+        col, dep, vel4, jit = self.scene.render(self.k)
+        col_s, vel_s = encode_color(col, self.fmt), encode_velocity(vel4, self.vfmt)
+        self.prev_depth.copy_(self.depth)
+        self.color.copy_(torch.from_numpy(col_s))
+        self.depth.copy_(torch.from_numpy(dep))
+        self.vel.copy_(torch.from_numpy(np.ascontiguousarray(vel_s)))
+        U = uniforms_for(self.w, self.k, jit, params=self.params)
+        U.mUpsampling = int(self.upsampling)
+        img = FrameImages()
+        img.color, img.depth, img.velocity = self.color.data_ptr(), self.depth.data_ptr(), self.vel.data_ptr()
+        img.history, img.result_history = self.hist[(self.k + 1) & 1].data_ptr(), self.hist[self.k & 1].data_ptr()
+        img.history_depth, img.result_screen = self.prev_depth.data_ptr(), self.screen.data_ptr()
+        img.debug, img.inspect = self.debug.data_ptr(), self.inspect.data_ptr()
+        img.width, img.height, img.pitch = self.W, self.H, self.W
+        img.input_width, img.input_height, img.input_pitch = self.Wl, self.Hl, self.Wl
+        img.color_format, img.velocity_format = self.fmt, self.vfmt
+        check(lib().taa_resolve_frame(C.byref(U), C.byref(img), C.byref(self.cas) if self.cas else None, self.arith,
+                                      None))
+        torch.cuda.synchronize()
+        o = oracle.resolve(U, decode(col_s), dep, velocity_as_read(vel_s), self.hist_o, self.prev_o)
+        o_hist = round_storage(o["history"], self.fmt)
+        o_scr = round_storage(o["screen"], self.fmt)
+        if self.cas is not None:
+            o_scr = round_storage(oracle.cas(o_scr, self.cas.const1[0]), self.fmt)
+        out = dict(gpu_history=decode(self.hist[self.k & 1].cpu().numpy()), gpu_screen=decode(self.screen.cpu().numpy()),
+                   gpu_inspect=self.inspect.cpu().numpy(), gpu_debug=self.debug.cpu().numpy().astype(np.float32),
+                   oracle_history=o_hist, oracle_screen=o_scr, oracle_inspect=o["inspect"],
+                   oracle_debug=to_f16(o["debug"]).astype(np.float32))
+        self.hist_o, self.prev_o = o_hist, dep
+        self.k += 1
+        return out
+
+
+CASES = {
+    "2x_upsampled": dict(lo=(96, 54), hi=(192, 108), upsampling=True),
+    "1.5x_upsampled": dict(lo=(128, 72), hi=(192, 108), upsampling=True),
+    "2x_nearest": dict(lo=(96, 54), hi=(192, 108), upsampling=False),
+    "1.33x_ragged_nocas": dict(lo=(75, 41), hi=(101, 57), upsampling=True, cas=False),
+    "1x_upsampling_flag": dict(lo=(160, 90), hi=(160, 90), upsampling=True),
+}
+
+
+@pytest.mark.parametrize("fmt", [FMT_RGBA32F, FMT_RGBA16F])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_taau_sequence(oracle, name, fmt):
+    c = dict(CASES[name])
+    print(f"case taau {name} fmt={fmt}", flush=True)
+    w = workload("c2", fmt=fmt, vel=VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F)
+    up = Upsampler(w, c.pop("lo"), c.pop("hi"), **c)
+    for _ in range(5):
+        assert_parity(up.step(oracle))
+
+
+VARIANTS = {
+    "simple_velocity": dict(mVelocitySampleMode=0),
+    "longest_velocity": dict(mVelocitySampleMode=1),
+    "antighost_unjitter": dict(mDynamicAntiGhosting=1, mUnjitterNeighbourhood=1, mUnjitterCurrentSample=1),
+    "matrix_reproj_raw_depth": dict(mUseVelocityVectors=0),
+    "debug_velocity": dict(mDebugMode=4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(VARIANTS))
+def test_taau_feature_variants(oracle, name):
+    print(f"case taau variant {name}", flush=True)
+    p = params_c2()
+    for k, v in VARIANTS[name].items():
+        setattr(p, k, v)
+    w = workload("c2", vel=VEL_RGBA32F)
+    up = Upsampler(w, (80, 48), (160, 96), params=p)
+    if name == "matrix_reproj_raw_depth":
+        up.w = dict(up.w, linear_depth=0.0)
+    for _ in range(4):
+        out = up.step(oracle)
+        assert_parity(out)
+        if name == "debug_velocity":
+            assert np.array_equal(out["gpu_debug"], out["oracle_debug"])
+
+
+def test_taau_fp16_request_takes_the_exact_path(oracle):
+    """ARITH_FP16 on the upsampling path resolves with the FP32 arithmetic (bit-exact)."""
+    w = workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F)
+    up = Upsampler(w, (96, 54), (192, 108), arith=ARITH_FP16)
+    for _ in range(3):
+        assert_parity(up.step(oracle))

@@ -515,3 +515,111 @@ def test_fp32_transcription_tracks_fp64(oracle):
         d = np.abs(s32 - s64)
         assert d.max() < 1e-3 and d.mean() < 1e-6
         h32, h64, prev = o32["history"], o64["history"], dep
+
+
+# ---------------------------------------------------------------------------------------------
+# TAAU: input (render) size differs from the output size, or mUpsampling is set
+
+def test_taau_scale1_zero_jitter_equals_plain(oracle):
+    """At equal sizes with power-of-two dimensions every uv lands exactly on a texel centre, the
+    bilinear depth / velocity samples reduce to fetches and the upsampled current sample is the
+    texel itself: mUpsampling changes nothing (all features that sample the current frame on)."""
+    rng = np.random.default_rng(41)
+    H, W = 32, 64
+    col, dep, vel, hist = frame(H, W, rng, vel=(0.015625, -0.03125))
+    vel[..., 0] += rng.integers(-4, 5, (H, W)).astype(f32) / 256.0  # dyadic: exact uv arithmetic
+    p = params(mUseYCoCg=1, mVarianceClipping=1, mColorClampingOrClipping=2, mVelocitySampleMode=2,
+               mDepthCulling=1, mRejectOutside=1, mLumaWeightingLottes=1, mAlpha=0.1)
+    a = run(oracle, uniforms(p), col, dep, vel, hist)
+    U = uniforms(p)
+    U.mUpsampling = 1
+    b = run(oracle, U, col, dep, vel, hist)
+    for k in ("screen", "history"):
+        np.testing.assert_array_equal(a[k], b[k])
+    np.testing.assert_array_equal(a["inspect"], b["inspect"])
+
+
+def test_taau_nearest_input_without_upsampling(oracle):
+    """2x output without mUpsampling: iuv_lores = int(uv * lo) = output texel // 2, so a reset frame
+    is the nearest-neighbour enlargement of the input (taa.comp:724, 753)."""
+    rng = np.random.default_rng(42)
+    Hl, Wl = 12, 20
+    col, dep, vel, _ = frame(Hl, Wl, rng)
+    hist = np.zeros((2 * Hl, 2 * Wl, 4))
+    o = run(oracle, uniforms(params(), reset=1), col, dep, vel, hist)
+    want = np.repeat(np.repeat(col[..., :3].astype(np.float64), 2, 0), 2, 1)
+    np.testing.assert_array_equal(o["screen"][..., :3], want)
+
+
+def test_taau_upsampled_sample_selection(oracle):
+    """2x output with mUpsampling and no jitter: input texel i (centre i + 0.5) lands on output
+    texel 2i + 1, so only odd/odd output texels receive a current sample (beta = 1); the others get
+    beta = 0. On a reset frame (alpha = beta = 1) the former are the input texel, the latter the
+    zero colour getCurrentUpsampledColor returns when nothing was found (taa.comp:222-257, 897)."""
+    rng = np.random.default_rng(43)
+    Hl, Wl = 10, 14
+    col, dep, vel, _ = frame(Hl, Wl, rng)
+    H, W = 2 * Hl, 2 * Wl
+    U = uniforms(params(), reset=1)
+    U.mUpsampling = 1
+    o = run(oracle, U, col, dep, vel, np.zeros((H, W, 4)))
+    scr = o["screen"][..., :3]
+    odd = np.zeros((H, W), bool)
+    odd[1::2, 1::2] = True
+    np.testing.assert_array_equal(scr[odd].reshape(Hl, Wl, 3), col[..., :3].astype(np.float64))
+    assert not scr[~odd].any()
+    # a blended frame (alpha 0.5): texels without a sample keep the history (alpha * beta = 0)
+    hist = rng.uniform(0.1, 1.0, (H, W, 4))
+    U.mResetHistory = 0
+    p = params(mAlpha=0.5)
+    U.param[0] = p
+    U.param[1] = p
+    o = run(oracle, U, col, dep, vel, hist)
+    np.testing.assert_allclose(o["history"][~odd][:, :3], hist[~odd][:, :3], rtol=1e-12)
+    np.testing.assert_allclose(o["history"][odd][:, :3].reshape(Hl, Wl, 3),
+                               0.5 * hist[odd][:, :3].reshape(Hl, Wl, 3) + 0.5 * col[..., :3], rtol=1e-12)
+
+
+def test_taau_jittered_sample_moves_with_the_jitter(oracle):
+    """texelJitter = -JITTER_UV * lo. +1/4 input texel moves input texel i's sample to i + 0.75,
+    output 2i + 1.5 -> texel 2i + 1 (odd columns); -1/4 moves it to i + 0.25, output 2i + 0.5 ->
+    texel 2i (even columns) (taa.comp:238-245)."""
+    rng = np.random.default_rng(44)
+    Hl, Wl = 6, 8
+    col, dep, vel, _ = frame(Hl, Wl, rng)
+    H, W = 2 * Hl, 2 * Wl
+    for tj, first in ((0.25, 1), (-0.25, 0)):
+        U = uniforms(params(mUnjitterFactor=1.0), reset=1)
+        U.mUpsampling = 1
+        # JITTER_UV = mJitterNdc * 0.5 * mUnjitterFactor; texelJitter = -JITTER_UV * lo
+        U.mJitterNdc[0] = -2.0 * tj / Wl
+        U.mJitterNdc[1] = 0.0
+        o = run(oracle, U, col, dep, vel, np.zeros((H, W, 4)))
+        got = o["screen"][1::2, :, :3]  # odd rows carry samples (no vertical jitter)
+        lit = np.flatnonzero(got.any(-1).any(0))
+        assert lit[0] == first and np.all(np.diff(lit) == 2), (tj, lit)
+
+
+def test_taau_depth_test_and_velocity_at_input_size(oracle):
+    """Velocity is sampled bilinearly at the output uv (a linear ramp is reproduced), and the depth
+    test fetches uHistoryDepth at int(historyUv * input size) (taa.comp:393, 818)."""
+    rng = np.random.default_rng(45)
+    Hl, Wl, H, W = 16, 24, 40, 60
+    col, dep, vel, _ = frame(Hl, Wl, rng)
+    xs = (np.arange(Wl) + 0.5) / Wl
+    vel[..., 0] = (0.01 + 0.02 * xs)[None, :]
+    vel[..., 1] = -0.005
+    p = params(mDepthCulling=1)
+    o = run(oracle, uniforms(p), col, dep, vel, np.zeros((H, W, 4)))
+    u = (np.arange(W) + 0.5) / W
+    v = (np.arange(H) + 0.5) / H
+    # inside the outermost half input texel the clamp makes the ramp flat
+    inner = (u > 0.5 / Wl) & (u < 1 - 0.5 / Wl)
+    hu = o["inspect"][..., INSPECT["HIST_UV_X"]].copy().view(f32)
+    hv = o["inspect"][..., INSPECT["HIST_UV_Y"]].copy().view(f32)
+    np.testing.assert_allclose(hu[:, inner], (u - (0.01 + 0.02 * u))[None, inner].repeat(H, 0), atol=2e-7)
+    np.testing.assert_allclose(hv, (v + 0.005)[:, None].repeat(W, 1), atol=1e-7)
+    dtx = o["inspect"][..., INSPECT["DEPTH_TC_X"]]
+    dty = o["inspect"][..., INSPECT["DEPTH_TC_Y"]]
+    np.testing.assert_array_equal(dtx, (hu.astype(np.float64) * Wl).astype(np.int64))
+    np.testing.assert_array_equal(dty, (hv.astype(np.float64) * Hl).astype(np.int64))
