@@ -223,8 +223,11 @@ __device__ f3 current_upsampled(const FrameDesc& fd, const taa_params_t& P, int 
 }
 
 // Catmull-Rom history fetch (taa.comp:441-514), 16-tap texelFetch form with the edge clamp
-// at size-1; identical to the 9-tap bilinear form up to rounding.
-template <bool S16>
+// at size-1; identical to the 9-tap bilinear form up to rounding. Every tap is
+// texel * w_x * w_y summed in the order of taa.comp:480-495. The alpha lane (read only by the
+// dynamic anti-ghosting test) is accumulated when ALPHA; an interior footprint reads each row's
+// four texels from one base address.
+template <bool S16, bool ALPHA>
 __device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
     const int W = fd.W, H = fd.H;
     float iTx = hu * (float)W, iTy = hv * (float)H;
@@ -237,11 +240,12 @@ __device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
     float wy[4] = {-0.5f * f3y + f2y - 0.5f * fy, 1.5f * f3y - 2.5f * f2y + 1.0f,
                    -1.5f * f3y + 2.0f * f2y + 0.5f * fy, 0.5f * f3y - 0.5f * f2y};
     int kx = clampi(to_int(fbx), -2, W + 1), ky = clampi(to_int(fby), -2, H + 1);
+    const bool interior = kx >= 1 && kx + 2 <= W - 1 && ky >= 1 && ky + 2 <= H - 1;
     int xs[4], ys[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        xs[i] = clampi(kx - 1 + i, 0, W - 1);
-        ys[i] = clampi(ky - 1 + i, 0, H - 1) * fd.pitch;
+        xs[i] = interior ? kx - 1 + i : clampi(kx - 1 + i, 0, W - 1);
+        ys[i] = (interior ? ky - 1 + i : clampi(ky - 1 + i, 0, H - 1)) * fd.pitch;
     }
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -250,9 +254,9 @@ __device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
         for (int i = 0; i < 4; ++i) {
             float4 t = ld_rgba<S16>(fd.history, (uint32_t)(ys[j] + xs[i]));
             float a = wx[i], b = wy[j];
-            float4 s = make_float4(t.x * a * b, t.y * a * b, t.z * a * b, t.w * a * b);
+            float4 s = make_float4(t.x * a * b, t.y * a * b, t.z * a * b, ALPHA ? t.w * a * b : 0.0f);
             if (i == 0 && j == 0) acc = s;
-            else acc = make_float4(acc.x + s.x, acc.y + s.y, acc.z + s.z, acc.w + s.w);
+            else acc = make_float4(acc.x + s.x, acc.y + s.y, acc.z + s.z, ALPHA ? acc.w + s.w : 0.0f);
         }
     return acc;
 }

@@ -223,7 +223,8 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     float4 historyRaw;
     if (P.mInterpolationMode == 0) historyRaw = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, hu, hv);
     else if (P.mInterpolationMode == 1) historyRaw = history_bspline<S16>(fd, hu, hv);
-    else historyRaw = history_catmullrom<S16>(fd, hu, hv);
+    else if (P.mDynamicAntiGhosting) historyRaw = history_catmullrom<S16, true>(fd, hu, hv);
+    else historyRaw = history_catmullrom<S16, false>(fd, hu, hv);
     f3 historyColor = xyz(historyRaw);
     if (P.mUseYCoCg) historyColor = rgb_to_ycocg(historyColor);
     float alpha = P.mAlpha;
