@@ -17,6 +17,7 @@
 //                              output in [0,1]
 // Parity unpinned (no reference run is possible here; documented definitions are used):
 //   sampler weight quantisation, out-of-bounds loads, GLSL sin() noise, FMA contraction.
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -91,6 +92,32 @@ int oracle_resolve_f64(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, co
                        double* hist_out, double* debug, int32_t* inspect, int cr16, int threads) {
     return resolve_frame<double>(U, W, H, Wl, Hl, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, cr16,
                                  threads);
+}
+
+// Check of a numerical identity the GPU kernels use (not part of the restatement): the number of
+// single-precision bit patterns x in [begin, end) for which q0 = x * RN(1/9),
+// r = fma(-q0, 9, x), q = (q0 == 0 or inf) ? q0 : fma(r, RN(1/9), q0) differs from the IEEE
+// quotient x / 9 (two NaNs compare equal).
+uint64_t oracle_div9_mismatches(uint64_t begin, uint64_t end, int threads) {
+    if (threads <= 0) threads = 1;
+    const float y = 1.0f / 9.0f;
+    uint64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static) num_threads(threads)
+    for (long long u = (long long)begin; u < (long long)end; ++u) {
+        const uint32_t ui = (uint32_t)u;
+        float x;
+        std::memcpy(&x, &ui, 4);
+        volatile float exact = x / 9.0f;
+        const float q0 = x * y;
+        const float r = std::fma(-q0, 9.0f, x);
+        const float q = (q0 == 0.0f || std::isinf(q0)) ? q0 : std::fma(r, y, q0);
+        const float e = exact;
+        uint32_t a, b;
+        std::memcpy(&a, &e, 4);
+        std::memcpy(&b, &q, 4);
+        if (a != b && !(std::isnan(e) && std::isnan(q))) bad++;
+    }
+    return bad;
 }
 
 int oracle_cas_f32(const float* src, float* dst, int W, int H, uint32_t const1_x, int threads) {

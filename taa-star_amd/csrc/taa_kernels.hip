@@ -39,6 +39,7 @@ struct Geo {
     struct Smem {
         float c0[UP ? 1 : SH * SP], c1[UP ? 1 : SH * SP], c2[UP ? 1 : SH * SP], d[UP ? 1 : SH * SP];
         float r0[CAS ? RH * RP : 1], r1[CAS ? RH * RP : 1], r2[CAS ? RH * RP : 1];
+        float ux[SW], uy[SH];  // uv of the tile's (clamped) columns and rows
     };
 };
 
@@ -53,11 +54,11 @@ template <bool S16, int SP, bool UP>
 __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
                                          int x, int y, int l, const float* __restrict__ s0,
                                          const float* __restrict__ s1, const float* __restrict__ s2,
-                                         const float* __restrict__ sd, bool write_aux) {
+                                         const float* __restrict__ sd, float uvx, float uvy, bool write_aux) {
+    // uvx, uvy: tc_to_uv(x, y) = ((x, y) + 0.5) / size (taa.comp:131), tabulated per tile
     const int W = fd.W, H = fd.H;
     const uint32_t gi = (uint32_t)(y * fd.pitch + x);
     PixelResult R;
-    const float uvx = ((float)x + 0.5f) / (float)W, uvy = ((float)y + 0.5f) / (float)H;  // taa.comp:131
     // input (render) resolution: iuv_lores = int(uv * input size) (taa.comp:724); == (x, y) at one size
     const int Wl = fd.Wl, Hl = fd.Hl, pl = fd.pitch_l;
     int xl = x, yl = y;
@@ -107,12 +108,12 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     }
     f3 colMin, colMax, clipTowards;
     if (P.mVarianceClipping) {
-        const float N = 9.0f;
+
         f3 m1 = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
         f3 m2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + c[3] * c[3] + c[4] * c[4] + c[5] * c[5] + c[6] * c[6] +
                 c[7] * c[7] + c[8] * c[8];
-        f3 mean = m1 / N;
-        f3 var = m2 / N - mean * mean;
+        f3 mean = {div9(m1.x), div9(m1.y), div9(m1.z)};  // m1 / 9
+        f3 var = f3{div9(m2.x), div9(m2.y), div9(m2.z)} - mean * mean;
         f3 sigma = {sqrtf(fmaxf(0.0f, var.x)), sqrtf(fmaxf(0.0f, var.y)), sqrtf(fmaxf(0.0f, var.z))};
         colMin = mean - P.mVarClipGamma * sigma;
         colMax = mean + P.mVarClipGamma * sigma;
@@ -386,6 +387,8 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         sm.c2[li] = t.z;
         sm.d[li] = ld_f32(fd.depth, gi);
     }
+    if (tid < G::SW) sm.ux[tid] = ((float)clampi(gx0 + tid, 0, W - 1) + 0.5f) / (float)W;
+    else if (tid < G::SW + G::SH) sm.uy[tid - G::SW] = ((float)clampi(gy0 + tid - G::SW, 0, H - 1) + 0.5f) / (float)H;
     __syncthreads();
 
     // 2. resolve the region
@@ -395,7 +398,8 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         const int cx = clampi(px, 0, W - 1), cy = clampi(py, 0, H - 1);
         const bool owned = rx >= G::RA && rx < G::RA + G::TW && ry >= G::RA && ry < G::RA + G::TH && px < fd.x_end && py < H;
         const int l = (cy - gy0) * G::SP + (cx - gx0);
-        PixelResult r = resolve_pixel_f32<S16, G::SP, G::UP>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, owned);
+        PixelResult r = resolve_pixel_f32<S16, G::SP, G::UP>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
+                                                             sm.ux[cx - gx0], sm.uy[cy - gy0], owned);
         if (owned) {
             const uint32_t gi = (uint32_t)(cy * pitch + cx);
             st_rgba<S16>(fd.history_out, gi, r.history);

@@ -623,3 +623,13 @@ def test_taau_depth_test_and_velocity_at_input_size(oracle):
     dty = o["inspect"][..., INSPECT["DEPTH_TC_Y"]]
     np.testing.assert_array_equal(dtx, (hu.astype(np.float64) * Wl).astype(np.int64))
     np.testing.assert_array_equal(dty, (hv.astype(np.float64) * Hl).astype(np.int64))
+
+
+def test_kernels_division_by_nine_is_the_ieee_division(oracle):
+    """The FP32 kernel divides the variance moments by 9 with a multiply and two FMAs; for every
+    single-precision input that equals the correctly rounded IEEE division m / 9 the transcription
+    performs (exhaustive over all 2^32 bit patterns, OpenMP)."""
+    L = oracle.lib()
+    L.oracle_div9_mismatches.argtypes = [C.c_uint64, C.c_uint64, C.c_int]
+    L.oracle_div9_mismatches.restype = C.c_uint64
+    assert L.oracle_div9_mismatches(0, 1 << 32, os.cpu_count() or 1) == 0
