@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--sim-seconds", type=float, default=2.0, help="measured time of each session-load run")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="after the timed loop, copy MIB MiB device-to-device (calibrates HBM byte counters)")
+    ap.add_argument("--velocity-only", action="store_true",
+                    help="time only the velocity producer (4K, RG16F, 3 moving objects) and print its line")
     return ap.parse_args()
 
 
@@ -178,6 +180,50 @@ def time_resolve(name, arith_name, steps, warmup, frames, autotune=False):
     return r
 
 
+def time_velocity(W, H, vfmt_name, objects, steps, warmup):
+    """K back-to-back velocity-producer launches (taa_velocity_from_depth) over one W x H depth
+    image resident in HBM, on a private stream, timed with HIP events on that stream."""
+    import torch
+    import taastar
+    from taastar.device import vel_shape_dtype, velocity_from_depth
+    from taastar.workloads import camera_case
+
+    vfmt = {"rg16f": taastar.VEL_RG16F, "rg32f": taastar.VEL_RG32F, "rgba16f": taastar.VEL_RGBA16F,
+            "rgba32f": taastar.VEL_RGBA32F}[vfmt_name]
+    dev = torch.device("cuda:0")
+    case = camera_case(W, H, seed=5, n_objects=objects)
+    dep = torch.from_numpy(case["depth"]).to(dev)
+    ids = torch.from_numpy(case["ids"].astype("int32")).to(dev) if objects else None
+    objs = torch.from_numpy(case["objects"]).to(dev) if objects else None
+    ch, dt = vel_shape_dtype(vfmt)
+    vel = torch.empty(H, W, ch, dtype=dt, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            velocity_from_depth(case["reproject"], dep, vel, vfmt, ids=ids, objects=objs, stream=stream)
+    stream.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    with torch.cuda.stream(stream):
+        ev[0].record(stream)
+        for i in range(steps):
+            velocity_from_depth(case["reproject"], dep, vel, vfmt, ids=ids, objects=objs, stream=stream)
+            ev[i + 1].record(stream)
+    stream.synchronize()
+    total_ms = ev[0].elapsed_time(ev[steps])
+    per = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+    bpp = 4 + (4 if objects else 0) + {"rg16f": 4, "rg32f": 8, "rgba16f": 8, "rgba32f": 16}[vfmt_name]
+    avg_ms = total_ms / steps
+    r = dict(width=W, height=H, velocity_format=vfmt_name, objects=objects, steps=steps, warmup=warmup,
+             gpix_per_s=round(steps * W * H / (total_ms * 1e-3) / 1e9, 3), avg_ms=round(avg_ms, 5),
+             median_ms=round(per[steps // 2], 5), bytes_per_pixel=bpp,
+             achieved_gbs=round(bpp * W * H / (avg_ms * 1e-3) / 1e9, 1),
+             hbm_frac=round(bpp * W * H / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+    del dep, ids, objs, vel
+    torch.cuda.empty_cache()
+    return r
+
+
 def tune(W, H, fmt, arith, cas, iters=20):
     from taastar import check, lib
     ctx = C.c_void_p()
@@ -240,6 +286,11 @@ def extras(a):
         x[f"resolve_{name}_{r['width']}x{r['height']}_{ar}arith"] = {
             k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items() if k not in ("w", "host")}
         print(f"# {name} {ar}: {r['gpix_per_s']:.3f} Gpix/s", file=sys.stderr, flush=True)
+    for vf, nobj in (("rg16f", 0), ("rgba16f", 3)):
+        r = time_velocity(3840, 2160, vf, nobj, 200, 20)
+        x[f"velocity_3840x2160_{vf}_{nobj}objects"] = r
+        print(f"# velocity {vf} objects={nobj}: {r['gpix_per_s']:.1f} Gpix/s, {r['hbm_frac']:.3f} of HBM",
+              file=sys.stderr, flush=True)
     for cls in RESOLUTIONS:  # autotune every class the mixed loads use (FP16 and FP32 storage)
         W, H = RESOLUTIONS[cls]
         tune(W, H, F16, A16, True)
@@ -298,6 +349,9 @@ def main():
     import taastar
 
     torch.cuda.set_device(0)
+    if a.velocity_only:
+        print(json.dumps(dict(kernel="k_velocity", **time_velocity(3840, 2160, "rgba16f", 3, a.steps, a.warmup))))
+        return
     r = time_resolve(a.workload, a.arith, a.steps, a.warmup, a.frames, autotune=a.autotune)
     w = r["w"]
     W, H = r["width"], r["height"]

@@ -10,6 +10,9 @@
 //                              against numpy mean/std); invariants min<=centre<=max
 //   closest-depth tap          brute force incl. all-equal ties -> tap 0
 //   velocity / reprojection    constant fields give historyUv = uv - v; identity matrices
+//   velocity producer          jitter cancels on a still camera; closed-form shifts of a
+//                              translated perspective camera and of a moved object; with zero
+//                              jitter uv - v.xy and depth - v.z equal the resolve's matrix path
 //   Catmull-Rom                9-tap (bilinear) form == 16-tap form; exact on linear ramps
 //                              and at texel centres; textbook Keys(a=-1/2) kernel in numpy
 //   clip / blend / rejection   closed forms: inside box -> identity, alpha=1 -> current, ...
@@ -75,6 +78,23 @@ int cas_frame(const T* src, T* dst, int W, int H, uint32_t const1_x, int threads
     return 0;
 }
 
+template <class T>
+int velocity_frame(const taa_reproject_t& R, const float* depth, const uint32_t* ids, int W, int H, T* out,
+                   int threads) {
+#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            T u = (T(x) + T(0.5)) / T(W), v = (T(y) + T(0.5)) / T(H);
+            oracle::v4<T> m = oracle::velocity_texel<T>(R, u, v, T(depth[i]), ids ? ids[i] : 0u);
+            out[4 * i + 0] = m.x;
+            out[4 * i + 1] = m.y;
+            out[4 * i + 2] = m.z;
+            out[4 * i + 3] = m.w;
+        }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -92,6 +112,18 @@ int oracle_resolve_f64(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, co
                        double* hist_out, double* debug, int32_t* inspect, int cr16, int threads) {
     return resolve_frame<double>(U, W, H, Wl, Hl, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, cr16,
                                  threads);
+}
+
+// Velocity image (xyzw per texel) of a W x H depth image; ids may be NULL (all static). The object
+// matrices are host memory here.
+int oracle_velocity_f32(const taa_reproject_t* R, const float* depth, const uint32_t* ids, int W, int H, float* out,
+                        int threads) {
+    return velocity_frame<float>(*R, depth, ids, W, H, out, threads);
+}
+
+int oracle_velocity_f64(const taa_reproject_t* R, const float* depth, const uint32_t* ids, int W, int H, double* out,
+                        int threads) {
+    return velocity_frame<double>(*R, depth, ids, W, H, out, threads);
 }
 
 // Check of a numerical identity the GPU kernels use (not part of the restatement): the number of

@@ -26,6 +26,7 @@
 #include <cstring>
 
 #include "taastar/taa_abi.h"
+#include "taastar/taa_velocity.h"
 
 namespace oracle {
 
@@ -720,6 +721,45 @@ inline float halton(int i, int b) {
         i = i / b;
     }
     return r;
+}
+
+// ---- velocity producer ----------------------------------------------------------------------
+// The motion vector of one texel as the geometry pass writes it (fwd_geometry.frag:288-295),
+// with the current clip position taken from the texel centre and its depth instead of from the
+// interpolated vertex position:
+//   clip      = (uv * 2 - 1, depth, 1)                                  (taa.comp:426)
+//   world     = inverse_view_proj * clip           (homogeneous, not divided; taa.comp:427)
+//   world     = object_prev_from_current[id - 1] * world   when 1 <= id <= n_objects
+//               (prev_modelMatrix, transform_and_pass_on.vert:99 / animatedObject.vert:100)
+//   prev      = prev_view_proj * world                                   (main.cpp:423)
+//   ndc       = clip.xyz,  ndc_prev = prev.xyz * (1 / prev.w)            (fwd_geometry.frag:289-290)
+//   ndc.xy   -= jitter.xy, ndc_prev.xy -= jitter.zw                      (:292-293)
+//   v.xyz     = (ndc - ndc_prev) * (0.5, 0.5, 1),  v.w = id              (:294-295)
+// Matrices are column-major; M * v is evaluated row by row as
+// M[r] v0 + M[4 + r] v1 + M[8 + r] v2 + M[12 + r] v3, left to right (the resolve's matrix path
+// uses the same order). GLSL's vector-by-scalar division is taken as one correctly rounded
+// reciprocal and three products, the usual lowering of that division (within GLSL's 2.5 ulp).
+template <class T>
+inline void mat4_apply(const float* M, const T (&v)[4], T (&o)[4]) {
+    for (int r = 0; r < 4; ++r) o[r] = T(M[r]) * v[0] + T(M[4 + r]) * v[1] + T(M[8 + r]) * v[2] + T(M[12 + r]) * v[3];
+}
+
+template <class T>
+inline v4<T> velocity_texel(const taa_reproject_t& R, T u, T v, T depth, uint32_t id) {
+    T clip[4] = {u * T(2.0) - T(1.0), v * T(2.0) - T(1.0), depth, T(1)};
+    T world[4], prev[4];
+    mat4_apply<T>(R.inverse_view_proj, clip, world);
+    if (id >= 1u && id <= (uint32_t)R.n_objects) {
+        T moved[4];
+        mat4_apply<T>(R.object_prev_from_current + 16 * (id - 1u), world, moved);
+        for (int r = 0; r < 4; ++r) world[r] = moved[r];
+    }
+    mat4_apply<T>(R.prev_view_proj, world, prev);
+    const float* j = R.jitter_current_prev;
+    T nx = clip[0] - T(j[0]), ny = clip[1] - T(j[1]), nz = clip[2];
+    const T inv = T(1) / prev[3];
+    T px = prev[0] * inv - T(j[2]), py = prev[1] * inv - T(j[3]), pz = prev[2] * inv;
+    return v4<T>{(nx - px) * T(0.5), (ny - py) * T(0.5), (nz - pz) * T(1.0), T(float(id))};
 }
 
 }  // namespace oracle

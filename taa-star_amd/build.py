@@ -31,6 +31,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fopenmp", "-Wa
 SOURCES = [
     ("taa_kernels.hip", "hip"),
     ("taa_fast.hip", "hip"),
+    ("taa_velocity.hip", "hip"),
     ("taa_api.cpp", "cxx"),
     ("taa_host.cpp", "cxx"),
     ("taa_scene.cpp", "cxx"),

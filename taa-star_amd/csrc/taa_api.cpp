@@ -318,6 +318,30 @@ int32_t taa_cas_apply(const void* src, void* dst, int32_t width, int32_t height,
     return ok();
 }
 
+// fwd_geometry.frag:288-295 from depth (include/taastar/taa_velocity.h)
+int32_t taa_velocity_from_depth(const taa_reproject_t* r, const float* depth, const uint32_t* object_id,
+                                void* velocity, int32_t velocity_format, int32_t width, int32_t height, int32_t pitch,
+                                void* stream) {
+    if (!r || !depth || !velocity) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
+    if (width <= 0 || height <= 0) return fail(TAA_ERR_INVALID_ARG, "bad size %dx%d", width, height);
+    if (!valid_vel_fmt(velocity_format)) return fail(TAA_ERR_INVALID_ARG, "bad velocity_format %d", velocity_format);
+    if (pitch == 0) pitch = width;
+    if (pitch < width) return fail(TAA_ERR_INVALID_ARG, "pitch < width");
+    if (!fits_offsets(pitch, height, 16)) return fail(TAA_ERR_INVALID_ARG, "image exceeds 4 GiB");
+    if (r->reserved[0] || r->reserved[1] || r->reserved[2]) return fail(TAA_ERR_INVALID_ARG, "reserved fields must be 0");
+    if (r->n_objects < 0) return fail(TAA_ERR_INVALID_ARG, "n_objects < 0");
+    if (r->n_objects > 0 && !r->object_prev_from_current)
+        return fail(TAA_ERR_INVALID_ARG, "n_objects > 0 needs object_prev_from_current");
+    if (r->n_objects > (1 << 20)) return fail(TAA_ERR_INVALID_ARG, "n_objects above 2^20");
+    if (reinterpret_cast<uintptr_t>(r->object_prev_from_current) % 16u)
+        return fail(TAA_ERR_INVALID_ARG, "object_prev_from_current must be 16-byte aligned");
+    if (static_cast<const void*>(depth) == velocity || static_cast<const void*>(object_id) == velocity)
+        return fail(TAA_ERR_INVALID_ARG, "velocity aliases an input");
+    int e = taastar::launch_velocity(*r, depth, object_id, velocity, velocity_format, width, height, pitch, stream);
+    if (e) return hip_fail((hipError_t)e, "velocity launch");
+    return ok();
+}
+
 int32_t taa_ctx_create(int32_t device, uint64_t hbm_budget_bytes, taa_ctx_t** out) {
     if (!out) return fail(TAA_ERR_INVALID_ARG, "out is NULL");
     *out = nullptr;

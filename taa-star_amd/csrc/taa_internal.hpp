@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "taastar/taa_abi.h"
+#include "taastar/taa_velocity.h"
 
 namespace taastar {
 
@@ -67,7 +68,6 @@ inline int batch_grid(int total_tiles) { return (total_tiles + 7) & ~7; }
 
 int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas,
                  bool s16, int arith, int grid, void* stream);
-// workgroups of one batched launch that keep every CU busy
 
 // Throughput kernel (taa_fast.hip): variance clip + Catmull-Rom + simple/closest-depth velocity
 // with RG velocity, fp16-fed mixed arithmetic. fast_path_supported() says whether a parameter
@@ -76,5 +76,9 @@ bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param
 int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape, void* stream);
 int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16,
                       int grid, void* stream);
+
+// Velocity producer (taa_velocity.hip): one launch over width x height; ids may be NULL.
+int launch_velocity(const taa_reproject_t& r, const float* depth, const uint32_t* ids, void* velocity, int fmt, int W,
+                    int H, int pitch, void* stream);
 
 }  // namespace taastar

@@ -1,6 +1,6 @@
 """taastar: MI355X-native TAA resolve + fused CAS (cg-tuwien/TAA-STAR taa.comp + sharpen_cas.comp).
 
-Python face of the C ABI in include/taastar/taa_abi.h. The compute runs in libtaastar.so
+Python face of the C ABI in include/taastar/ (taa_abi.h, taa_velocity.h, taa_serve.h, taa_scene.h). The compute runs in libtaastar.so
 (hand-written gfx950 kernels); torch is used only to own device buffers and streams.
 """
 import ctypes as C
@@ -11,11 +11,11 @@ from ._abi import (ARITH_FP16, ARITH_FP32, FLAG_DEPTH_REJECT, FLAG_GHOST_REJECT,
                    FLAG_REJECTED, FLAG_USED_VELOCITY, FMT_RGBA16F, FMT_RGBA32F, INSPECT, INSPECT_WORDS, LIB_PATH,
                    PROTOTYPES, VEL_RG16F, VEL_RG32F, VEL_RGBA16F, VEL_RGBA32F, BatchItem, CasConsts, FrameImages,
                    JitterConfig, Params, SceneConfig, ServeReport, ServerConfig, ServerStats, ServeSimConfig,
-                   Completion, SessionIO, TaaError, TileShape, Uniforms, check, lib)
+                   Completion, Reproject, SessionIO, TaaError, TileShape, Uniforms, check, lib)
 
 __all__ = [
     "lib", "check", "TaaError", "Params", "Uniforms", "CasConsts", "FrameImages", "JitterConfig", "SessionIO",
-    "BatchItem", "TileShape", "SceneConfig", "ServerConfig", "ServerStats", "ServeSimConfig", "ServeReport",
+    "BatchItem", "Reproject", "TileShape", "SceneConfig", "ServerConfig", "ServerStats", "ServeSimConfig", "ServeReport",
     "Completion", "default_params", "cas_setup", "halton", "jitter_offset", "Scene",
     "FMT_RGBA32F", "FMT_RGBA16F", "VEL_RG32F", "VEL_RG16F", "VEL_RGBA32F", "VEL_RGBA16F", "ARITH_FP32",
     "ARITH_FP16", "INSPECT", "INSPECT_WORDS", "LIB_PATH", "PROTOTYPES",

@@ -1,4 +1,4 @@
-"""ctypes mirror of include/taastar/taa_abi.h and taa_scene.h (struct layouts are asserted)."""
+"""ctypes mirror of include/taastar/taa_abi.h, taa_velocity.h and taa_scene.h (struct layouts are asserted)."""
 import ctypes as C
 import os
 
@@ -150,6 +150,18 @@ assert C.sizeof(Uniforms) == 544 and Uniforms.param.offset == 128 and Uniforms.m
 assert Uniforms.mResetHistory.offset == 528 and Uniforms.pad1.offset == 540
 assert C.sizeof(CasConsts) == 32
 
+
+
+class Reproject(C.Structure):
+    """taa_reproject_t (include/taastar/taa_velocity.h): the velocity producer's camera and objects."""
+    _fields_ = [
+        ("inverse_view_proj", C.c_float * 16), ("prev_view_proj", C.c_float * 16), ("jitter_current_prev", C.c_float * 4),
+        ("object_prev_from_current", C.c_void_p), ("n_objects", C.c_int32), ("reserved", C.c_int32 * 3),
+    ]
+
+
+assert C.sizeof(Reproject) == 168 and Reproject.n_objects.offset == 152
+
 P = C.POINTER
 I32, U32, U64, F32, VP = C.c_int32, C.c_uint32, C.c_uint64, C.c_float, C.c_void_p
 
@@ -163,6 +175,7 @@ PROTOTYPES = {
     "taa_jitter_offset": (I32, [P(JitterConfig), U64, U32, U32, P(F32), P(U32)]),
     "taa_resolve_frame": (I32, [P(Uniforms), P(FrameImages), P(CasConsts), I32, VP]),
     "taa_cas_apply": (I32, [VP, VP, I32, I32, I32, I32, P(CasConsts), VP]),
+    "taa_velocity_from_depth": (I32, [P(Reproject), VP, VP, VP, I32, I32, I32, I32, VP]),
     "taa_ctx_create": (I32, [I32, U64, P(VP)]),
     "taa_ctx_destroy": (I32, [VP]),
     "taa_ctx_usage": (I32, [VP, P(U64), P(U64)]),

@@ -4,7 +4,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from . import (FMT_RGBA16F, INSPECT_WORDS, VEL_RG16F, VEL_RG32F, VEL_RGBA32F, FrameImages, check, lib)
+from . import (FMT_RGBA16F, INSPECT_WORDS, VEL_RG16F, VEL_RG32F, VEL_RGBA32F, FrameImages, Reproject, check, lib)
 
 
 def _stream_ptr(stream):
@@ -86,3 +86,15 @@ def cas_apply(src, dst, fmt, cas, stream=None):
     H, W = src.shape[0], src.shape[1]
     check(lib().taa_cas_apply(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), W, H, W, fmt, C.byref(cas),
                               _stream_ptr(stream)))
+
+
+def velocity_from_depth(reproject, depth, velocity, vfmt, ids=None, objects=None, width=None, stream=None):
+    """taa_velocity_from_depth on torch tensors. depth [H, P] float32, velocity [H, P, ch] in vfmt,
+    ids [H, P] int32/uint32 or None, objects a device float32 tensor [n, 16] or None; width <= P
+    (default P) is the image width, P the pitch."""
+    H, P = depth.shape[0], depth.shape[1]
+    r = Reproject.from_buffer_copy(reproject)
+    r.object_prev_from_current = None if objects is None else objects.data_ptr()
+    check(lib().taa_velocity_from_depth(C.byref(r), C.c_void_p(depth.data_ptr()),
+                                        None if ids is None else C.c_void_p(ids.data_ptr()),
+                                        C.c_void_p(velocity.data_ptr()), vfmt, width or P, H, P, _stream_ptr(stream)))

@@ -9,8 +9,8 @@ c5  served load: 256 sessions, Poisson arrivals, FP16 storage, HBM-budget admiss
 """
 import numpy as np
 
-from . import (FMT_RGBA16F, FMT_RGBA32F, VEL_RG16F, VEL_RG32F, VEL_RGBA16F, VEL_RGBA32F, Scene, Uniforms,
-               cas_setup, default_params)
+from . import (FMT_RGBA16F, FMT_RGBA32F, VEL_RG16F, VEL_RG32F, VEL_RGBA16F, VEL_RGBA32F, Reproject, Scene,
+               Uniforms, cas_setup, default_params)
 
 NEAR, FAR = 0.1, 100.0          # camera planes for the linear-depth test (main.cpp:3780 uses near 0.1)
 LINEAR_DEPTH_THRESHOLD = 0.01   # c2/c3: reject when |linear(hd) - linear(expected)| > 0.01
@@ -137,3 +137,84 @@ def bytes_per_pixel(w, cas=True):
         reads += 4        # previous depth
     writes = cb + cb      # history, screen (sharpened when CAS is fused)
     return reads + writes
+
+
+# ---- camera matrices for the velocity producer --------------------------------------------
+# Row-major float64 math (A @ v); colmajor() gives glm's column-major float32 layout.
+
+def perspective(fovy_deg=60.0, aspect=16.0 / 9.0, near=NEAR, far=FAR):
+    """Right-handed perspective with depth 0 at the near plane and 1 at the far plane
+    (glm::perspectiveRH_ZO): a point at view-space z = -d projects to NDC z in [0, 1]."""
+    f = 1.0 / np.tan(np.radians(fovy_deg) / 2.0)
+    P = np.zeros((4, 4))
+    P[0, 0], P[1, 1] = f / aspect, f
+    P[2, 2], P[2, 3] = far / (near - far), -(far * near) / (far - near)
+    P[3, 2] = -1.0
+    return P
+
+
+def translate(x, y, z):
+    T = np.eye(4)
+    T[:3, 3] = (x, y, z)
+    return T
+
+
+def rotate_y(deg):
+    c, s = np.cos(np.radians(deg)), np.sin(np.radians(deg))
+    R = np.eye(4)
+    R[0, 0], R[0, 2], R[2, 0], R[2, 2] = c, s, -s, c
+    return R
+
+
+def jittered(P, jx, jy):
+    """The jittered projection of get_jittered_projection_matrix (taa.hpp:247): NDC translated by the jitter."""
+    return translate(jx, jy, 0.0) @ P
+
+
+def colmajor(A):
+    return np.ascontiguousarray(np.asarray(A, np.float64).T.reshape(-1).astype(np.float32))
+
+
+def reproject_for(P, V_cur, V_prev, jit_cur, jit_prev, objects=None):
+    """Reproject for a camera moving from V_prev to V_cur under projection P, with jitters (NDC).
+    objects: [n, 4, 4] float64 previous-from-current object matrices (row-major math), or None.
+    Returns (Reproject, column-major float32 object array or None); the caller points
+    object_prev_from_current at that array (host memory for the oracle, device for the GPU)."""
+    r = Reproject()
+    Mi = np.linalg.inv(jittered(P, *jit_cur) @ V_cur)
+    Mp = jittered(P, *jit_prev) @ V_prev
+    r.inverse_view_proj[:] = colmajor(Mi).tolist()
+    r.prev_view_proj[:] = colmajor(Mp).tolist()
+    r.jitter_current_prev[:] = [jit_cur[0], jit_cur[1], jit_prev[0], jit_prev[1]]
+    obj = None
+    if objects is not None and len(objects):
+        obj = np.ascontiguousarray(np.stack([colmajor(o) for o in objects]))
+        r.n_objects = len(objects)
+    return r, obj
+
+
+def depth_from_view_distance(dist, P):
+    """NDC depth of points at view-space distance `dist` (positive) straight through P."""
+    return (P[2, 2] * -dist + P[2, 3]) / dist
+
+
+def camera_case(W, H, seed=0, n_objects=0, move=(5e-4, -3e-4, 1e-3), turn_deg=0.2):
+    """A frame for the velocity producer: the seeded scene's depth at W x H, a camera that moves
+    by `move` and turns by `turn_deg` between frames, Halton-sized jitters, and n_objects moved
+    objects covering horizontal bands of the frame (ids 1..n_objects, id n_objects + 1 beyond the
+    table). The scene's depths lie 0.13-0.15 units in front of the camera, so these motions are a
+    few texels at 1080p. Returns dict(depth, ids, reproject, objects)."""
+    rng = np.random.default_rng(seed)
+    d = scene_for(workload("c2"), seed=seed, width=W, height=H).render(1)[1]
+    P = perspective(aspect=W / H)
+    V_prev = translate(0.0, -1.5, -4.0)
+    V_cur = rotate_y(turn_deg) @ translate(-move[0], -move[1], -move[2]) @ V_prev
+    jc = (float(rng.uniform(-1, 1)) / W, float(rng.uniform(-1, 1)) / H)
+    jp = (float(rng.uniform(-1, 1)) / W, float(rng.uniform(-1, 1)) / H)
+    objects = [translate(*rng.uniform(-5e-4, 5e-4, 3)) @ rotate_y(float(rng.uniform(-0.02, 0.02))) for _ in range(n_objects)]
+    r, obj = reproject_for(P, V_cur, V_prev, jc, jp, objects)
+    ids = None
+    if n_objects:
+        band = np.minimum((np.arange(H) * (n_objects + 2)) // max(H, 1), n_objects + 1).astype(np.uint32)
+        ids = np.ascontiguousarray(np.broadcast_to(band[:, None], (H, W)))
+    return dict(depth=np.ascontiguousarray(d, np.float32), ids=ids, reproject=r, objects=obj)

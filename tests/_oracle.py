@@ -101,3 +101,23 @@ def jitter(dist, frame, W, H, slow=0, fixed=-1, rot=0.0, scale=1.0, user=None):
                              None if u is None else _p(u, C.c_float), W, H, out, C.byref(n))
     assert rc == 0
     return (out[0], out[1]), n.value
+
+
+def velocity(R, depth, ids=None, objects=None, dtype=np.float32, threads=0):
+    """Velocity image (H, W, 4) of the oracle's producer; objects: host float32 [n, 16] column-major."""
+    H, W = depth.shape
+    threads = threads or os.cpu_count() or 1
+    L = lib()
+    T = C.c_float if dtype == np.float32 else C.c_double
+    fn = L.oracle_velocity_f32 if dtype == np.float32 else L.oracle_velocity_f64
+    fn.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_int, C.c_int, C.POINTER(T), C.c_int]
+    r = type(R).from_buffer_copy(R)
+    obj = None if objects is None else np.ascontiguousarray(objects, np.float32)
+    r.object_prev_from_current = None if obj is None else obj.ctypes.data
+    depth = np.ascontiguousarray(depth, np.float32)
+    ids_c = None if ids is None else np.ascontiguousarray(ids, np.uint32)
+    out = np.empty((H, W, 4), dtype)
+    rc = fn(C.byref(r), _p(depth, C.c_float), None if ids_c is None else _p(ids_c, C.c_uint32), W, H, _p(out, T),
+            threads)
+    assert rc == 0
+    return out

@@ -150,6 +150,36 @@ def test_argument_errors_need_no_gpu():
     assert lib.taa_session_close(None) == _abi.TAA_ERR_INVALID_ARG
 
 
+def test_velocity_argument_errors_need_no_gpu():
+    lib = taastar.lib()
+    R = taastar.Reproject()
+    dep, vel, ids = 4096, 1 << 20, 1 << 21   # never dereferenced: every call below is refused first
+    call = lambda r, d, i, v, fmt=_abi.VEL_RG16F, w=64, h=32, p=0: lib.taa_velocity_from_depth(
+        C.byref(r) if r is not None else None, d, i, v, fmt, w, h, p, None)
+    assert call(None, dep, None, vel) == _abi.TAA_ERR_INVALID_ARG
+    assert call(R, None, None, vel) == _abi.TAA_ERR_INVALID_ARG
+    assert call(R, dep, None, None) == _abi.TAA_ERR_INVALID_ARG
+    assert call(R, dep, None, vel, fmt=7) == _abi.TAA_ERR_INVALID_ARG
+    assert call(R, dep, None, vel, w=0) == _abi.TAA_ERR_INVALID_ARG
+    assert call(R, dep, None, vel, p=10) == _abi.TAA_ERR_INVALID_ARG
+    assert call(R, dep, None, vel, w=16384, h=16384) == _abi.TAA_ERR_INVALID_ARG
+    assert b"4 GiB" in lib.taa_last_error()
+    assert call(R, dep, None, dep) == _abi.TAA_ERR_INVALID_ARG
+    assert call(R, dep, vel, vel) == _abi.TAA_ERR_INVALID_ARG
+    R.n_objects = 2
+    assert call(R, dep, ids, vel) == _abi.TAA_ERR_INVALID_ARG
+    assert b"object_prev_from_current" in lib.taa_last_error()
+    R.object_prev_from_current = 4096 + 8
+    assert call(R, dep, ids, vel) == _abi.TAA_ERR_INVALID_ARG
+    assert b"aligned" in lib.taa_last_error()
+    R.n_objects, R.object_prev_from_current = -1, None
+    assert call(R, dep, ids, vel) == _abi.TAA_ERR_INVALID_ARG
+    R.n_objects = 0
+    R.reserved[2] = 1
+    assert call(R, dep, None, vel) == _abi.TAA_ERR_INVALID_ARG
+    assert b"reserved" in lib.taa_last_error()
+
+
 def test_tile_shapes_table():
     lib = taastar.lib()
     n = lib.taa_tile_shape_count()

@@ -633,3 +633,121 @@ def test_kernels_division_by_nine_is_the_ieee_division(oracle):
     L.oracle_div9_mismatches.argtypes = [C.c_uint64, C.c_uint64, C.c_int]
     L.oracle_div9_mismatches.restype = C.c_uint64
     assert L.oracle_div9_mismatches(0, 1 << 32, os.cpu_count() or 1) == 0
+
+
+# ---------------------------------------------------------------------------------------------
+# velocity producer (fwd_geometry.frag:288-295 from depth)
+
+def view_distances(H, W, rng):
+    """Piecewise-constant view distances (0.5 .. 40 units) in vertical bands plus a slanted plane."""
+    d = np.repeat(rng.uniform(0.5, 40.0, W // 8 + 1), 8)[:W][None, :].repeat(H, 0)
+    d[H // 2:] = 2.0 + 0.05 * np.arange(W)[None, :]
+    return d
+
+
+def depth_and_distance(dist, P):
+    """NDC depth of each distance, stored as float32, and the distance that float32 depth encodes."""
+    from taastar.workloads import depth_from_view_distance
+    d32 = depth_from_view_distance(dist, P).astype(f32)
+    return d32, P[2, 3] / (d32.astype(np.float64) + P[2, 2])
+
+
+def test_velocity_still_camera_jitter_cancels(oracle):
+    """A still camera with different current and previous jitters: the depth is rendered through the
+    jittered projection and the jitters are removed in NDC (fwd_geometry.frag:292-293), so every
+    texel's motion is zero to the float32 rounding of the matrices."""
+    from taastar.workloads import perspective, reproject_for, translate
+    rng = np.random.default_rng(70)
+    H, W = 24, 40
+    P = perspective(aspect=W / H)
+    V = translate(0.3, -1.0, -5.0)
+    d32, _ = depth_and_distance(view_distances(H, W, rng), P)
+    R, _ = reproject_for(P, V, V, (0.7 / W, -0.4 / H), (-0.9 / W, 0.6 / H))
+    v = oracle.velocity(R, d32, dtype=np.float64)
+    assert np.abs(v[..., :3]).max() < 2e-6          # a dropped or mis-signed jitter term gives > 1e-2
+    assert np.all(v[..., 3] == 0.0)
+
+
+def test_velocity_translated_camera_closed_form(oracle):
+    """Camera moving by (dx, 0, dz) between frames, no jitter: a point at view distance r with NDC
+    (nx, ny) was at r + dz, x + dx in the previous view, so v.x = (nx - P00 (nx r / P00 + dx) / (r + dz)) / 2,
+    v.y = ny (1 - r / (r + dz)) / 2, v.z = depth(r) - depth(r + dz)."""
+    from taastar.workloads import depth_from_view_distance, perspective, reproject_for, translate
+    rng = np.random.default_rng(71)
+    H, W = 30, 48
+    P = perspective(fovy_deg=50.0, aspect=W / H)
+    dx, dz = 0.02, 0.05
+    V_prev = translate(0.0, -1.0, -3.0)
+    V_cur = translate(-dx, 0.0, dz) @ V_prev     # camera moved +dx along x and dz towards -z
+    d32, r = depth_and_distance(view_distances(H, W, rng), P)
+    R, _ = reproject_for(P, V_cur, V_prev, (0.0, 0.0), (0.0, 0.0))
+    v = oracle.velocity(R, d32, dtype=np.float64)
+    nx = ((np.arange(W) + 0.5) / W * 2 - 1)[None, :]
+    ny = ((np.arange(H) + 0.5) / H * 2 - 1)[:, None]
+    xc = nx * r / P[0, 0]
+    ex = 0.5 * (nx - P[0, 0] * (xc + dx) / (r + dz))
+    ey = 0.5 * ny * (1.0 - r / (r + dz))
+    ez = d32.astype(np.float64) - depth_from_view_distance(r + dz, P)
+    np.testing.assert_allclose(v[..., 0], ex, atol=2e-6)
+    np.testing.assert_allclose(v[..., 1], ey, atol=2e-6)
+    np.testing.assert_allclose(v[..., 2], ez, atol=2e-6)
+    assert np.abs(ex).max() > 1e-3 and np.abs(ez).max() > 1e-4  # the motion is far above the tolerance
+
+
+def test_velocity_moving_object_closed_form(oracle):
+    """Still camera, object 1 moved by ox between frames (previous = current + ox): its texels get
+    v.x = -P00 ox / (2 r), v.y = v.z = 0, w = 1; static texels and ids beyond the object table
+    get no motion and keep their id in w (fwd_geometry.frag:295)."""
+    from taastar.workloads import perspective, reproject_for, translate
+    rng = np.random.default_rng(72)
+    H, W = 18, 32
+    P = perspective(aspect=W / H)
+    V = translate(0.0, 0.0, -2.0)
+    ox = 0.01
+    d32, r = depth_and_distance(view_distances(H, W, rng), P)
+    R, obj = reproject_for(P, V, V, (0.0, 0.0), (0.0, 0.0), objects=[translate(ox, 0.0, 0.0)])
+    ids = np.zeros((H, W), np.uint32)
+    ids[: H // 3] = 1
+    ids[H // 3: 2 * H // 3] = 3
+    v = oracle.velocity(R, d32, ids=ids, objects=obj, dtype=np.float64)
+    moved = ids == 1
+    np.testing.assert_allclose(v[..., 0][moved], (-0.5 * P[0, 0] * ox / r)[moved], atol=2e-6)
+    assert np.abs(v[..., 1:3][moved]).max() < 2e-6
+    assert np.abs(v[..., :3][~moved]).max() < 2e-6
+    np.testing.assert_array_equal(v[..., 3], ids.astype(np.float64))
+
+
+def test_velocity_agrees_with_the_resolves_matrix_path(oracle):
+    """With zero jitter the producer and the resolve's matrix reprojection (taa.comp:424-431, the
+    other transcription) describe the same motion: uv - v.xy is the matrix path's history uv and
+    depth - v.z its expected history depth."""
+    from taastar.workloads import perspective, reproject_for, rotate_y, translate
+    rng = np.random.default_rng(73)
+    H, W = 20, 36
+    P = perspective(aspect=W / H)
+    V_prev = translate(0.1, -1.0, -4.0)
+    V_cur = rotate_y(0.7) @ translate(-0.03, 0.01, 0.04) @ V_prev
+    d32, _ = depth_and_distance(view_distances(H, W, rng), P)
+    R, _ = reproject_for(P, V_cur, V_prev, (0.0, 0.0), (0.0, 0.0))
+    v = oracle.velocity(R, d32, dtype=np.float64)
+    p = params(mUseVelocityVectors=0)
+    U = uniforms(p)
+    U.mInverseViewProjMatrix[:] = list(R.inverse_view_proj)
+    U.mHistoryViewProjMatrix[:] = list(R.prev_view_proj)
+    col, _, vel, hist = frame(H, W, rng)
+    o = run(oracle, U, col, d32, vel, hist)
+    hu = o["inspect"][..., INSPECT["HIST_UV_X"]].copy().view(f32)
+    hv = o["inspect"][..., INSPECT["HIST_UV_Y"]].copy().view(f32)
+    u = ((np.arange(W) + 0.5) / W)[None, :]
+    vv = ((np.arange(H) + 0.5) / H)[:, None]
+    np.testing.assert_allclose(u - v[..., 0], hu, atol=1e-6)
+    np.testing.assert_allclose(vv - v[..., 1], hv, atol=1e-6)
+    assert np.abs(v[..., :2]).max() > 5e-3
+
+
+def test_velocity_fp32_tracks_fp64(oracle):
+    from taastar.workloads import camera_case
+    c = camera_case(96, 54, seed=3, n_objects=2)
+    a = oracle.velocity(c["reproject"], c["depth"], c["ids"], c["objects"], dtype=np.float64)
+    b = oracle.velocity(c["reproject"], c["depth"], c["ids"], c["objects"], dtype=np.float32)
+    assert np.abs(a - b).max() < 2e-5
