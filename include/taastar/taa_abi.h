@@ -90,7 +90,7 @@ typedef struct taa_params {
     taa_bool32 mVelBasedAlpha;          /* 100 */
     float      mVelBasedAlphaMax;       /* 104 */
     float      mVelBasedAlphaFactor;    /* 108 */
-    taa_bool32 mRayTraceAugment;        /* 112  must be 0: ray-trace augmentation is not built */
+    taa_bool32 mRayTraceAugment;        /* 112  segmentation mask (taa_resolve_frame_segmented) */
     uint32_t   mRayTraceAugmentFlags;   /* 116 */
     float      mRayTraceAugment_WNrm;   /* 120 */
     float      mRayTraceAugment_WDpt;   /* 124 */
@@ -175,6 +175,8 @@ enum {
     TAA_INSPECT_DEPTH_TC_Y = 12,
     TAA_INSPECT_HIST_UV_X = 13,  /* float historyUv */
     TAA_INSPECT_HIST_UV_Y = 14,
+    TAA_INSPECT_SEG_TOTAL = 15,  /* float weighted edge indicator of the segmentation mask
+                                    (0 when the mask is off or decided before it) */
     TAA_INSPECT_WORDS = 16
 };
 enum {
@@ -218,6 +220,39 @@ typedef struct taa_jitter_config {
 int32_t taa_jitter_offset(const taa_jitter_config_t* cfg, uint64_t frame_id, uint32_t width,
                           uint32_t height, float out_ndc[2], uint32_t* out_num_samples);
 
+/* ---- segmentation mask (mRayTraceAugment) ------------------------------------------------ */
+
+/* mRayTraceAugmentFlags bits (shaders/shader_cpu_common.h:31-40) */
+enum {
+    TAA_RTFLAG_OUT = 0x0001, /* history outside the screen -> edge-AA fallback            */
+    TAA_RTFLAG_DIS = 0x0002, /* disocclusion by a moving object -> ray trace             */
+    TAA_RTFLAG_NRM = 0x0004, /* normal discontinuity indicator                           */
+    TAA_RTFLAG_DPT = 0x0008, /* linear-depth Sobel indicator                              */
+    TAA_RTFLAG_MID = 0x0010, /* material-id edge indicator                                */
+    TAA_RTFLAG_LUM = 0x0020, /* luminance Sobel indicator                                 */
+    TAA_RTFLAG_CNT = 0x0040, /* keep marking for mRayTraceHistoryCount frames            */
+    TAA_RTFLAG_ALL = 0x0080, /* mark every texel for ray tracing (debug)                 */
+    TAA_RTFLAG_FXD = 0x0100, /* a fixed 100-texel border for the edge-AA fallback (debug) */
+    TAA_RTFLAG_FXA = 0x0200  /* run the edge-AA fallback pass on category-1 texels        */
+};
+/* Category of a mask texel (bits 0-1); bits 16-31 hold the remaining history count. */
+enum { TAA_SEG_TAA = 0, TAA_SEG_EDGE_AA = 1, TAA_SEG_RAY_TRACE = 2 };
+
+/* The extra images of the segmentation mask (taa.comp:34-37, 589-702): R32UI material ids of this
+ * and the previous frame (bit 31 marks a moving object, fwd_geometry.frag:286), the RGBA32F
+ * uv + spherical-normal G-buffer (IMAGE_FORMAT_NORMAL; z azimuth, w elevation), the previous
+ * frame's mask and the mask written by this frame (TAA_IMAGE_FORMAT_SEGMASK, R32UI). All at the
+ * frame's size, rows of `pitch` texels (0 = the frame's pitch); segmask must not alias an input. */
+typedef struct taa_segmentation_images {
+    const uint32_t* material;      /* set0 b10 uCurrentMaterial  */
+    const uint32_t* prev_material; /* set0 b11 uPreviousMaterial */
+    const float*    uv_normal;     /* set0 b13 uCurrentUvNrm     */
+    const uint32_t* prev_segmask;  /* set0 b12 uPreviousSegMask  */
+    uint32_t*       segmask;       /* set0 b9  uSegMask (written) */
+    int32_t         pitch;
+    int32_t         reserved[3];   /* must be 0 */
+} taa_segmentation_images_t;
+
 /* ---- per-frame GPU entry points ---------------------------------------------------------- */
 
 /* One taa.comp dispatch over the whole frame (source/taa.hpp:1008-1027) and, when `cas` is not
@@ -225,10 +260,23 @@ int32_t taa_jitter_offset(const taa_jitter_config_t* cfg, uint64_t frame_id, uin
  * receives the sharpened screen, exactly what the reference's separate pass would write.
  * Stream-ordered and asynchronous; allocates nothing.
  * An input size different from the output size, or mUpsampling, takes the upsampling path (exact
- * FP32 arithmetic whatever `arith` asks). Not built (TAA_ERR_UNSUPPORTED): mRayTraceAugment; a
+ * FP32 arithmetic whatever `arith` asks). TAA_ERR_UNSUPPORTED: mRayTraceAugment (it needs the
+ * extra images of taa_resolve_frame_segmented); a
  * split screen together with CAS (resolve without CAS, then taa_cas_apply). */
 int32_t taa_resolve_frame(const taa_uniforms_t* uniforms, const taa_frame_images_t* images,
                           const taa_cas_consts_t* cas, int32_t arith, void* stream);
+
+/* taa_resolve_frame with the segmentation mask of mRayTraceAugment (taa.comp:589-702, 772-783,
+ * 959): where a Parameters block has mRayTraceAugment set, each texel's mask value is computed
+ * from the extra images, written to seg->segmask, and a non-zero category rejects the history
+ * (alpha = mRejectionAlpha) as in the reference; debug mode 8 shows the category. Texels of
+ * mPassThrough / mBypassHistoryUpdate leave the mask untouched, as the reference's early return
+ * does. Exact FP32 arithmetic whatever `arith` asks; one size only (an input size different from
+ * the output size, or mUpsampling, is TAA_ERR_UNSUPPORTED). Category-2 texels are for the
+ * caller's ray tracer; category-1 texels for taa_edge_aa_apply. */
+int32_t taa_resolve_frame_segmented(const taa_uniforms_t* uniforms, const taa_frame_images_t* images,
+                                    const taa_segmentation_images_t* seg, const taa_cas_consts_t* cas,
+                                    int32_t arith, void* stream);
 
 /* The sharpen_cas.comp pass alone (source/taa.hpp:1127-1134): src, dst in `format`, must not
  * alias. Border texels are read clamped to the edge. */

@@ -10,6 +10,11 @@
 //                              against numpy mean/std); invariants min<=centre<=max
 //   closest-depth tap          brute force incl. all-equal ties -> tap 0
 //   velocity / reprojection    constant fields give historyUv = uv - v; identity matrices
+//   segmentation mask          each indicator against an independent numpy form (scipy-style
+//                              Sobel, unit normals from angles, 4-neighbour id compare), the
+//                              decision paths (border, all, outside, moving-object disocclusion,
+//                              history count) on constructed frames; FP32 sin/cos within a few
+//                              ulp of the library functions
 //   velocity producer          jitter cancels on a still camera; closed-form shifts of a
 //                              translated perspective camera and of a moved object; with zero
 //                              jitter uv - v.xy and depth - v.z equal the resolve's matrix path
@@ -29,15 +34,34 @@
 
 namespace {
 
+// Segmentation inputs of one frame (all W x H, one size); mask_out receives uSegMask where the
+// reference stores it and keeps its content elsewhere.
+struct SegIO {
+    const uint32_t* material;
+    const uint32_t* prev_material;
+    const float* uv_normal;
+    const uint32_t* prev_segmask;
+    uint32_t* mask_out;
+};
+
 template <class T>
 int resolve_frame(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, const float* color, const float* depth,
                   const float* vel4, const T* hist, const float* hdepth, T* screen, T* hist_out,
-                  T* debug, int32_t* inspect, int cr16, int threads) {
+                  T* debug, int32_t* inspect, int cr16, int threads, const SegIO* seg = nullptr) {
     if (!U || W <= 0 || H <= 0 || Wl <= 0 || Hl <= 0 || !color || !depth || !vel4 || !hist || !hdepth || !screen || !hist_out)
+        return -1;
+    if (seg && (Wl != W || Hl != H || !seg->material || !seg->prev_material || !seg->uv_normal || !seg->prev_segmask ||
+                !seg->mask_out))
         return -1;
     oracle::Frame<T> F;
     F.W = W; F.H = H; F.Wl = Wl; F.Hl = Hl;
     F.color = color; F.depth = depth; F.vel = vel4; F.hist = hist; F.hdepth = hdepth;
+    if (seg) {
+        F.material = seg->material;
+        F.prev_material = seg->prev_material;
+        F.uv_normal = seg->uv_normal;
+        F.prev_segmask = seg->prev_segmask;
+    }
     if (threads <= 0) threads = 1;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
     for (int y = 0; y < H; ++y) {
@@ -53,6 +77,7 @@ int resolve_frame(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, const f
                 d[0] = o.debug.x; d[1] = o.debug.y; d[2] = o.debug.z; d[3] = o.debug.w;
             }
             if (inspect) std::memcpy(inspect + (size_t)TAA_INSPECT_WORDS * i, o.inspect, sizeof(o.inspect));
+            if (seg && o.seg_written) seg->mask_out[i] = o.segmask;
         }
     }
     return 0;
@@ -112,6 +137,30 @@ int oracle_resolve_f64(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, co
                        double* hist_out, double* debug, int32_t* inspect, int cr16, int threads) {
     return resolve_frame<double>(U, W, H, Wl, Hl, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, cr16,
                                  threads);
+}
+
+// The resolve with the segmentation mask (one size): the extra images and the mask written.
+int oracle_resolve_seg_f32(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+                           const float* vel4, const float* hist, const float* hdepth, float* screen, float* hist_out,
+                           float* debug, int32_t* inspect, const uint32_t* material, const uint32_t* prev_material,
+                           const float* uv_normal, const uint32_t* prev_segmask, uint32_t* mask_out, int threads) {
+    SegIO sg{material, prev_material, uv_normal, prev_segmask, mask_out};
+    return resolve_frame<float>(U, W, H, W, H, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, 1,
+                                threads, &sg);
+}
+
+int oracle_resolve_seg_f64(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+                           const float* vel4, const double* hist, const float* hdepth, double* screen, double* hist_out,
+                           double* debug, int32_t* inspect, const uint32_t* material, const uint32_t* prev_material,
+                           const float* uv_normal, const uint32_t* prev_segmask, uint32_t* mask_out, int threads) {
+    SegIO sg{material, prev_material, uv_normal, prev_segmask, mask_out};
+    return resolve_frame<double>(U, W, H, W, H, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, 1,
+                                 threads, &sg);
+}
+
+// sin / cos of the segmentation normals in FP32 (the evaluation the kernels repeat), elementwise.
+void oracle_seg_sincos_f32(const float* a, float* s, float* c, int n) {
+    for (int i = 0; i < n; ++i) oracle::seg_sincos(a[i], s[i], c[i]);
 }
 
 // Velocity image (xyzw per texel) of a W x H depth image; ids may be NULL (all static). The object

@@ -101,6 +101,15 @@ template <class T> struct Frame {
         x = iclamp(x, 0, Wl - 1); y = iclamp(y, 0, Hl - 1);
         return T(hdepth[(size_t)y * Wl + x]);
     }
+    // segmentation inputs (one size only: W == Wl); material == nullptr: no mask
+    const uint32_t* material = nullptr;      // R32UI  uCurrentMaterial
+    const uint32_t* prev_material = nullptr; // R32UI  uPreviousMaterial
+    const float* uv_normal = nullptr;        // RGBA32F uCurrentUvNrm
+    const uint32_t* prev_segmask = nullptr;  // R32UI  uPreviousSegMask
+    uint32_t u32_at(const uint32_t* img, int x, int y) const {
+        x = iclamp(x, 0, W - 1); y = iclamp(y, 0, H - 1);
+        return img[(size_t)y * W + x];
+    }
 };
 
 // Bilinear filter with clamp-to-edge (the sampler of source/taa.hpp:274) at normalised uv,
@@ -348,6 +357,8 @@ template <class T> struct PixelOut {
     v4<T> history;  // uResultHistory
     v4<T> debug;    // uDebug
     int32_t inspect[TAA_INSPECT_WORDS];
+    uint32_t segmask = 0;     // uSegMask
+    bool seg_written = false; // the reference stores uSegMask for this texel
 };
 
 template <class T> inline void put_f(int32_t* w, int i, T v) { float f = float(v); std::memcpy(&w[i], &f, 4); }
@@ -374,6 +385,116 @@ inline v3<T> current_upsampled(const taa_params_t& P, const Frame<T>& F, iv2 cur
     }
     beta = T(0);
     return {T(0), T(0), T(0)};
+}
+
+// ---- segmentation mask (taa.comp:566-702) ---------------------------------------------------
+
+// sin and cos of the G-buffer's normal angles (taa.comp:574-579). GLSL leaves their precision to
+// the implementation; FP64 takes the library functions, FP32 a fixed evaluation the kernels
+// repeat operation for operation: Cody-Waite reduction by pi/2 with a three-part constant
+// (Cody & Waite, Software Manual for the Elementary Functions, 1980), then the Taylor
+// polynomials of sin (degree 9) and cos (degree 10) on [-pi/4, pi/4] in Horner form, all with FMA.
+inline void seg_sincos(double a, double& s, double& c) { s = std::sin(a); c = std::cos(a); }
+inline void seg_sincos(float a, float& s, float& c) {
+    const float k = std::rint(a * 0x1.45f306p-1f);  // a * 2/pi, nearest even
+    float r = std::fma(-k, 0x1.921fb6p+0f, a);
+    r = std::fma(-k, -0x1.777a5cp-25f, r);
+    r = std::fma(-k, -0x1.ee59dap-50f, r);
+    const float r2 = r * r;
+    float ps = std::fma(0x1.71de3ap-19f, r2, -0x1.a01a02p-13f);  // 1/9!, -1/7!
+    ps = std::fma(ps, r2, 0x1.111112p-7f);                        // 1/5!
+    ps = std::fma(ps, r2, -0x1.555556p-3f);                       // -1/3!
+    const float sr = std::fma(r * r2, ps, r);
+    float pc = std::fma(-0x1.27e4fcp-22f, r2, 0x1.a01a02p-16f);  // -1/10!, 1/8!
+    pc = std::fma(pc, r2, -0x1.6c16c2p-10f);                      // -1/6!
+    pc = std::fma(pc, r2, 0x1.555556p-5f);                        // 1/4!
+    pc = std::fma(pc, r2, -0.5f);                                 // -1/2!
+    const float cr = std::fma(r2, pc, 1.0f);
+    switch (to_int(k) & 3) {
+        case 0: s = sr; c = cr; break;
+        case 1: s = cr; c = -sr; break;
+        case 2: s = -sr; c = -cr; break;
+        default: s = -cr; c = sr; break;
+    }
+}
+
+// sample_normal (taa.comp:574-579): (cos z cos w, sin z cos w, sin w)
+template <class T> inline v3<T> seg_normal(const Frame<T>& F, int x, int y) {
+    x = iclamp(x, 0, F.W - 1); y = iclamp(y, 0, F.H - 1);
+    const float* p = F.uv_normal + 4 * ((size_t)y * F.W + x);
+    T sz, cz, sw, cw;
+    seg_sincos(T(p[2]), sz, cz);
+    seg_sincos(T(p[3]), sw, cw);
+    return {cz * cw, sz * cw, sw};
+}
+
+// sobel (taa.comp:582-587), arguments in the reference's order (row y-1, then x-1 / x+1 of row y,
+// then row y+1), written component by component
+template <class T> inline T sobel_length(const T (&t)[8]) {
+    T gx = t[0] - t[5] + T(2) * t[1] - T(2) * t[6] + t[2] - t[7];
+    T gy = t[0] - t[2] + T(2) * t[3] - T(2) * t[4] + t[5] - t[7];
+    return std::sqrt(gx * gx + gy * gy);  // length()
+}
+
+// calc_segmentation_value (taa.comp:589-702) at texel iuv with the frame's historyUv; `total`
+// receives the weighted indicator sum when the function gets that far (else 0).
+template <class T>
+inline uint32_t segmentation_value(const taa_uniforms_t& U, const taa_params_t& P, const Frame<T>& F, iv2 iuv,
+                                   v2<T> historyUv, T& total) {
+    total = T(0);
+    const uint32_t fl = P.mRayTraceAugmentFlags;
+    const int W = F.W, H = F.H;
+    if (fl & TAA_RTFLAG_FXD) {
+        const int b = 100;
+        if (iuv.x < b || iuv.y < b || iuv.x >= W - b || iuv.y >= H - b) return 1u;
+    }
+    if (fl & TAA_RTFLAG_ALL) return 2u;
+    const bool useHistoryCount = (fl & TAA_RTFLAG_CNT) != 0;
+    const uint32_t newCountValue = useHistoryCount ? (uint32_t)P.mRayTraceHistoryCount << 16 : 0u;
+    if (fl & TAA_RTFLAG_OUT) {
+        if (historyUv.x < T(0) || historyUv.y < T(0) || historyUv.x >= T(1) || historyUv.y >= T(1)) return 1u;
+    }
+    const uint32_t matId = F.u32_at(F.material, iuv.x, iuv.y);
+    if (fl & TAA_RTFLAG_DIS) {
+        iv2 tcPrev = uv_to_tc<T>(historyUv, iv2{W, H});
+        if (tcPrev.x >= 0 && tcPrev.y >= 0 && tcPrev.x < W && tcPrev.y < H) {
+            const uint32_t prevMatId = F.u32_at(F.prev_material, tcPrev.x, tcPrev.y);
+            if (prevMatId != matId && (prevMatId & 0x80000000u) != 0) return 2u | newCountValue;
+        }
+    }
+    T nrmValue = T(0), dptValue = T(0), matValue = T(0), lumValue = T(0);
+    static const int ox[8] = {-1, 0, 1, -1, 1, -1, 0, 1}, oy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+    if (fl & TAA_RTFLAG_NRM) {
+        v3<T> nC = seg_normal(F, iuv.x, iuv.y);
+        auto d = [&](int dx, int dy) { v3<T> n = seg_normal(F, iuv.x + dx, iuv.y + dy); return nC.x * n.x + nC.y * n.y + nC.z * n.z; };
+        T dL = d(-1, 0), dR = d(1, 0), dT = d(0, -1), dB = d(0, 1);
+        T mind = gmax(T(0), gmin(gmin(gmin(dL, dR), dT), dB));
+        nrmValue = T(1.0) - mind;
+    }
+    if (fl & TAA_RTFLAG_DPT) {
+        T t[8];
+        for (int k = 0; k < 8; ++k)
+            t[k] = linearize_depth(F.depth_at(iuv.x + ox[k], iuv.y + oy[k]), T(U.mCamNearPlane), T(U.mCamFarPlane));
+        dptValue = sobel_length(t);
+    }
+    if (fl & TAA_RTFLAG_MID) {
+        if (matId != F.u32_at(F.material, iuv.x - 1, iuv.y) || matId != F.u32_at(F.material, iuv.x + 1, iuv.y) ||
+            matId != F.u32_at(F.material, iuv.x, iuv.y - 1) || matId != F.u32_at(F.material, iuv.x, iuv.y + 1))
+            matValue = T(1.0);
+    }
+    if (fl & TAA_RTFLAG_LUM) {
+        T t[8];
+        for (int k = 0; k < 8; ++k) t[k] = rgb_to_ycocg(xyz(F.color_at(iuv.x + ox[k], iuv.y + oy[k]))).x;
+        lumValue = sobel_length(t);
+    }
+    total = nrmValue * T(P.mRayTraceAugment_WNrm) + dptValue * T(P.mRayTraceAugment_WDpt) +
+            matValue * T(P.mRayTraceAugment_WMId) + lumValue * T(P.mRayTraceAugment_WLum);
+    if (total >= T(P.mRayTraceAugment_Thresh)) return 2u | newCountValue;
+    if (useHistoryCount) {
+        const uint32_t oldCnt = (F.u32_at(F.prev_segmask, iuv.x, iuv.y) & 0xffff0000u) >> 16;
+        if (oldCnt > 0) return 2u | ((oldCnt - 1u) << 16);
+    }
+    return 0u;
 }
 
 // taa.comp:707-960 for output texel iuv. `cr16` selects the 16-tap Catmull-Rom form (default)
@@ -498,6 +619,20 @@ inline PixelOut<T> resolve_pixel(const taa_uniforms_t& U, const Frame<T>& F, iv2
     T alpha = T(P.mAlpha);
     bool rejected = false;
 
+    // ---- segmentation mask (taa.comp:773-784)
+    uint32_t segMaskValue = 0;
+    if (P.mRayTraceAugment && F.material) {
+        T total;
+        segMaskValue = segmentation_value(U, P, F, iuv, historyUv, total);
+        put_f(o.inspect, TAA_INSPECT_SEG_TOTAL, total);
+        if ((segMaskValue & 3u) != 0) {
+            alpha = T(P.mRejectionAlpha);
+            rejected = true;
+        }
+        o.segmask = segMaskValue;
+        o.seg_written = true;
+    }
+
     // ---- history rejection (taa.comp:787-823)
     if (P.mRejectOutside) {
         if (historyUv.x < T(0) || historyUv.y < T(0) || historyUv.x >= T(1) || historyUv.y >= T(1)) {
@@ -614,7 +749,14 @@ inline PixelOut<T> resolve_pixel(const taa_uniforms_t& U, const Frame<T>& F, iv2
         case 5: g = {pixelSpeed, T(0), T(0), T(0)}; break;
         case 6: g = o.screen; break;
         case 7: g = o.history; break;
-        case 8: g = {T(0), T(0), T(1), T(0)}; break;  // segmentation value is always 0 here
+        case 8:  // the mask category (taa.comp:932-938); category 3 does not occur
+            switch (segMaskValue & 3u) {
+                case 0: g = {T(0), T(0), T(1), T(0)}; break;
+                case 1: g = {T(1), T(0), T(0), T(0)}; break;
+                case 2: g = {T(1), T(1), T(0), T(0)}; break;
+                default: break;
+            }
+            break;
         default: break;
     }
     T ds = T(P.mDebugScale);

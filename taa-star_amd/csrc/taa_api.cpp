@@ -139,10 +139,11 @@ int make_desc(const taa_frame_images_t* img, const taa_cas_consts_t* cas, taasta
     return TAA_OK;
 }
 
-int check_uniforms(const taa_uniforms_t* u) {
+int check_uniforms(const taa_uniforms_t* u, bool segmented = false) {
     if (!u) return fail(TAA_ERR_INVALID_ARG, "uniforms is NULL");
     for (int i = 0; i < 2; ++i) {
-        if (u->param[i].mRayTraceAugment) return fail(TAA_ERR_UNSUPPORTED, "mRayTraceAugment is not built");
+        if (u->param[i].mRayTraceAugment && !segmented)
+            return fail(TAA_ERR_UNSUPPORTED, "mRayTraceAugment needs the segmentation images (taa_resolve_frame_segmented)");
         if (u->param[i].mInterpolationMode < 0 || u->param[i].mInterpolationMode > 2)
             return fail(TAA_ERR_INVALID_ARG, "mInterpolationMode %d", u->param[i].mInterpolationMode);
     }
@@ -150,9 +151,21 @@ int check_uniforms(const taa_uniforms_t* u) {
 }
 
 // Launch the resolve over [x_begin, x_end) with the given parameter block.
-int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, int x_end, int param_idx, int arith, void* stream) {
+int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, int x_end, int param_idx, int arith, void* stream,
+                 const taastar::SegDesc* sg = nullptr) {
     if (x_end <= x_begin) return TAA_OK;
     const bool cas = fd.cas_on != 0;
+    if (sg) {  // segmentation mask: exact FP32 kernel, default tile shape
+        const taastar::TileShapeDef d = shape_def(cas ? taastar::kDefaultShapeCas : taastar::kDefaultShapePlain, cas);
+        fd.x_begin = x_begin;
+        fd.x_end = x_end;
+        fd.param_idx = param_idx;
+        fd.tiles_x = taastar::tiles_for(x_end - x_begin, d.tw);
+        fd.tiles_y = taastar::tiles_for(fd.H, d.th);
+        const int e = taastar::launch_resolve_seg(fd, U, *sg, stream);
+        if (e != 0) return hip_fail((hipError_t)e, "resolve launch");
+        return TAA_OK;
+    }
     const int shape = selected_shape(fd.W, fd.H, fd.color_fmt, arith, cas);
     const taastar::TileShapeDef s = shape_def(shape, cas);
     fd.x_begin = x_begin;
@@ -176,20 +189,38 @@ int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, in
     return TAA_OK;
 }
 
-int resolve_impl(const taa_uniforms_t* u, const taa_frame_images_t* img, const taa_cas_consts_t* cas, int arith, void* stream) {
-    int rc = check_uniforms(u);
+int resolve_impl(const taa_uniforms_t* u, const taa_frame_images_t* img, const taa_cas_consts_t* cas, int arith, void* stream,
+                 const taa_segmentation_images_t* seg = nullptr) {
+    int rc = check_uniforms(u, seg != nullptr);
     if (rc) return rc;
     if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith %d", arith);
     taastar::FrameDesc fd;
     rc = make_desc(img, cas, fd);
     if (rc) return rc;
-    if (!u->splitScreen) return launch_range(fd, *u, 0, fd.W, 0, arith, stream);
+    taastar::SegDesc sgd;
+    const taastar::SegDesc* sg = nullptr;
+    if (seg) {
+        if (fd.up || u->mUpsampling) return fail(TAA_ERR_UNSUPPORTED, "the segmentation mask is built for one size (no upsampling)");
+        if (!seg->material || !seg->prev_material || !seg->uv_normal || !seg->prev_segmask || !seg->segmask)
+            return fail(TAA_ERR_INVALID_ARG, "a segmentation image pointer is NULL");
+        if (seg->reserved[0] || seg->reserved[1] || seg->reserved[2]) return fail(TAA_ERR_INVALID_ARG, "reserved must be 0");
+        const int sp = seg->pitch == 0 ? fd.pitch : seg->pitch;
+        if (sp < fd.W) return fail(TAA_ERR_INVALID_ARG, "segmentation pitch %d < width %d", sp, fd.W);
+        if (!fits_offsets(sp, fd.H, 16)) return fail(TAA_ERR_INVALID_ARG, "a segmentation image would exceed 4 GiB");
+        const void* m = seg->segmask;
+        if (m == seg->prev_segmask || m == seg->material || m == seg->prev_material || m == seg->uv_normal || m == fd.color ||
+            m == fd.depth || m == fd.history || m == fd.hdepth || m == fd.velocity || m == fd.history_out || m == fd.screen)
+            return fail(TAA_ERR_INVALID_ARG, "segmask aliases another image");
+        sgd = taastar::SegDesc{seg->material, seg->prev_material, seg->uv_normal, seg->prev_segmask, seg->segmask, sp, 0};
+        sg = &sgd;
+    }
+    if (!u->splitScreen) return launch_range(fd, *u, 0, fd.W, 0, arith, stream, sg);
     if (cas) return fail(TAA_ERR_UNSUPPORTED, "split screen with CAS: resolve without CAS, then taa_cas_apply");
     // paramsIdx = (splitScreen && x > splitX) ? 1 : 0 (taa.comp:717)
     const int split = std::min(std::max(u->splitX + 1, 0), fd.W);
-    rc = launch_range(fd, *u, 0, split, 0, arith, stream);
+    rc = launch_range(fd, *u, 0, split, 0, arith, stream, sg);
     if (rc) return rc;
-    return launch_range(fd, *u, split, fd.W, 1, arith, stream);
+    return launch_range(fd, *u, split, fd.W, 1, arith, stream, sg);
 }
 
 }  // namespace
@@ -302,6 +333,13 @@ int32_t taa_params_defaults(taa_params_t* p) {
 int32_t taa_resolve_frame(const taa_uniforms_t* uniforms, const taa_frame_images_t* images, const taa_cas_consts_t* cas,
                           int32_t arith, void* stream) {
     return resolve_impl(uniforms, images, cas, arith, stream);
+}
+
+int32_t taa_resolve_frame_segmented(const taa_uniforms_t* uniforms, const taa_frame_images_t* images,
+                                    const taa_segmentation_images_t* seg, const taa_cas_consts_t* cas, int32_t arith,
+                                    void* stream) {
+    if (!seg) return fail(TAA_ERR_INVALID_ARG, "seg is NULL");
+    return resolve_impl(uniforms, images, cas, arith, stream, seg);
 }
 
 int32_t taa_cas_apply(const void* src, void* dst, int32_t width, int32_t height, int32_t pitch, int32_t format,

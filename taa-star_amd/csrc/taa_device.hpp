@@ -159,6 +159,34 @@ __device__ __forceinline__ f3 stage_convert(const taa_params_t& P, f3 c) {
 __device__ __forceinline__ float luminance(const taa_params_t& P, f3 c) { return P.mUseYCoCg ? c.x : rgb_to_ycocg(c).x; }
 __device__ __forceinline__ float linearize_depth(float d, float n, float f) { return n * f / (f + d * (n - f)); }
 
+// sin and cos of the G-buffer's normal angles (taa.comp:574-579): Cody-Waite reduction by pi/2
+// with a three-part constant (Cody & Waite, Software Manual for the Elementary Functions, 1980),
+// then the Taylor polynomials of sin (degree 9) and cos (degree 10) on [-pi/4, pi/4] in Horner
+// form, every step an FMA or a correctly rounded operation; within 1.5 ulp of sin / cos on
+// [-200, 200], and the same bits as the host evaluation of the same steps.
+__device__ __forceinline__ void seg_sincos(float a, float& s, float& c) {
+    const float k = rintf(a * 0x1.45f306p-1f);
+    float r = __builtin_fmaf(-k, 0x1.921fb6p+0f, a);
+    r = __builtin_fmaf(-k, -0x1.777a5cp-25f, r);
+    r = __builtin_fmaf(-k, -0x1.ee59dap-50f, r);
+    const float r2 = r * r;
+    float ps = __builtin_fmaf(0x1.71de3ap-19f, r2, -0x1.a01a02p-13f);
+    ps = __builtin_fmaf(ps, r2, 0x1.111112p-7f);
+    ps = __builtin_fmaf(ps, r2, -0x1.555556p-3f);
+    const float sr = __builtin_fmaf(r * r2, ps, r);
+    float pc = __builtin_fmaf(-0x1.27e4fcp-22f, r2, 0x1.a01a02p-16f);
+    pc = __builtin_fmaf(pc, r2, -0x1.6c16c2p-10f);
+    pc = __builtin_fmaf(pc, r2, 0x1.555556p-5f);
+    pc = __builtin_fmaf(pc, r2, -0.5f);
+    const float cr = __builtin_fmaf(r2, pc, 1.0f);
+    switch (to_int(k) & 3) {
+        case 0: s = sr; c = cr; break;
+        case 1: s = cr; c = -sr; break;
+        case 2: s = -sr; c = -cr; break;
+        default: s = -cr; c = sr; break;
+    }
+}
+
 // Bilinear (clamp to edge, exact weights) over an RGBA image -- the sampler of taa.hpp:274 at a
 // position that is not a texel centre.
 template <bool S16>

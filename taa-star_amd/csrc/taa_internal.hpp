@@ -33,6 +33,17 @@ struct FrameDesc {
     int32_t up;               // upsampling path: input size != output size, or mUpsampling
 };
 
+// The segmentation mask's extra images (taa_segmentation_images_t), pitch resolved.
+struct SegDesc {
+    const uint32_t* material;
+    const uint32_t* prev_material;
+    const float* uv_normal;
+    const uint32_t* prev_segmask;
+    uint32_t* segmask;
+    int32_t pitch;
+    int32_t pad;
+};
+
 // One frame of a batched launch: frame + device copy of its uniforms + first flattened tile.
 struct BatchEntry {
     FrameDesc fd;
@@ -59,6 +70,8 @@ inline int tiles_for(int extent, int t) { return (extent + t - 1) / t; }
 int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int arith, void* stream);
 // the upsampling (TAAU) path: exact FP32 arithmetic, default tile shape
 int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, void* stream);
+// the resolve with the segmentation mask (exact FP32 arithmetic, default tile shape)
+int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, void* stream);
 int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uint32_t peak_bits, void* stream);
 // Workgroups of a batched launch: one per tile, rounded to whole XCD slots. The batch kernels walk
 // the tile space with a grid stride, so a smaller (persistent) grid is also correct; measured on

@@ -48,13 +48,92 @@ struct PixelResult {
     float4 history;  // antiAliased, dynamic mask
 };
 
+// sobel (taa.comp:582-587) and length(), arguments in the reference's order: row y-1, then the
+// left and right texels of row y, then row y+1
+__device__ __forceinline__ float sobel_length(const float (&t)[8]) {
+    const float gx = t[0] - t[5] + 2.0f * t[1] - 2.0f * t[6] + t[2] - t[7];
+    const float gy = t[0] - t[2] + 2.0f * t[3] - 2.0f * t[4] + t[5] - t[7];
+    return sqrtf(gx * gx + gy * gy);
+}
+
+// calc_segmentation_value (taa.comp:589-702) at texel (x, y), one size. `total` receives the
+// weighted indicator sum when the evaluation gets that far (else 0). The depth Sobel reads the
+// staged depth plane (its apron is clamped to the edge like CLAMP_TO_TEX); the other inputs come
+// from global memory.
+template <bool S16, int SP>
+__device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, const taa_uniforms_t& U,
+                                       const taa_params_t& P, int x, int y, float hu, float hv,
+                                       const float* __restrict__ sd, int l, float& total) {
+    total = 0.0f;
+    const uint32_t fl = P.mRayTraceAugmentFlags;
+    const int W = fd.W, H = fd.H;
+    auto at_s = [&](int xx, int yy) { return (uint32_t)(clampi(yy, 0, H - 1) * sg.pitch + clampi(xx, 0, W - 1)); };
+    auto at_f = [&](int xx, int yy) { return (uint32_t)(clampi(yy, 0, H - 1) * fd.pitch + clampi(xx, 0, W - 1)); };
+    if (fl & TAA_RTFLAG_FXD) {
+        const int b = 100;
+        if (x < b || y < b || x >= W - b || y >= H - b) return 1u;
+    }
+    if (fl & TAA_RTFLAG_ALL) return 2u;
+    const bool useHistoryCount = (fl & TAA_RTFLAG_CNT) != 0u;
+    const uint32_t newCountValue = useHistoryCount ? (uint32_t)P.mRayTraceHistoryCount << 16 : 0u;
+    if ((fl & TAA_RTFLAG_OUT) && (hu < 0.0f || hv < 0.0f || hu >= 1.0f || hv >= 1.0f)) return 1u;
+    const uint32_t matId = gload<uint32_t>(sg.material, at_s(x, y) * 4u);
+    if (fl & TAA_RTFLAG_DIS) {
+        const int tx = to_int(hu * (float)W), ty = to_int(hv * (float)H);  // uv_to_tc(historyUv)
+        if (tx >= 0 && ty >= 0 && tx < W && ty < H) {
+            const uint32_t prevMatId = gload<uint32_t>(sg.prev_material, (uint32_t)(ty * sg.pitch + tx) * 4u);
+            if (prevMatId != matId && (prevMatId & 0x80000000u) != 0u) return 2u | newCountValue;
+        }
+    }
+    const int ox[8] = {-1, 0, 1, -1, 1, -1, 0, 1}, oy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+    float nrmValue = 0.0f, dptValue = 0.0f, matValue = 0.0f, lumValue = 0.0f;
+    if (fl & TAA_RTFLAG_NRM) {
+        auto normal = [&](int xx, int yy) {  // sample_normal (taa.comp:574-579)
+            const float4 q = gload<float4>(sg.uv_normal, at_s(xx, yy) * 16u);
+            float sz, cz, sw, cw;
+            seg_sincos(q.z, sz, cz);
+            seg_sincos(q.w, sw, cw);
+            return f3{cz * cw, sz * cw, sw};
+        };
+        const f3 nC = normal(x, y);
+        auto dn = [&](int dx, int dy) { const f3 n = normal(x + dx, y + dy); return nC.x * n.x + nC.y * n.y + nC.z * n.z; };
+        const float dL = dn(-1, 0), dR = dn(1, 0), dT = dn(0, -1), dB = dn(0, 1);
+        nrmValue = 1.0f - fmaxf(0.0f, fminf(fminf(fminf(dL, dR), dT), dB));
+    }
+    if (fl & TAA_RTFLAG_DPT) {
+        float t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t[k] = linearize_depth(sd[l + oy[k] * SP + ox[k]], U.mCamNearPlane, U.mCamFarPlane);
+        dptValue = sobel_length(t);
+    }
+    if (fl & TAA_RTFLAG_MID) {
+        auto mid = [&](int xx, int yy) { return gload<uint32_t>(sg.material, at_s(xx, yy) * 4u); };
+        if (matId != mid(x - 1, y) || matId != mid(x + 1, y) || matId != mid(x, y - 1) || matId != mid(x, y + 1)) matValue = 1.0f;
+    }
+    if (fl & TAA_RTFLAG_LUM) {
+        float t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t[k] = rgb_to_ycocg(xyz(ld_rgba<S16>(fd.color, at_f(x + ox[k], y + oy[k])))).x;
+        lumValue = sobel_length(t);
+    }
+    total = nrmValue * P.mRayTraceAugment_WNrm + dptValue * P.mRayTraceAugment_WDpt + matValue * P.mRayTraceAugment_WMId +
+            lumValue * P.mRayTraceAugment_WLum;
+    if (total >= P.mRayTraceAugment_Thresh) return 2u | newCountValue;
+    if (useHistoryCount) {
+        const uint32_t oldCnt = (gload<uint32_t>(sg.prev_segmask, at_s(x, y) * 4u) & 0xffff0000u) >> 16;
+        if (oldCnt > 0u) return 2u | ((oldCnt - 1u) << 16);
+    }
+    return 0u;
+}
+
 // One output texel of taa.comp main (taa.comp:707-960), FP32 arithmetic. `l` indexes the texel
-// in the staged LDS planes (row pitch SP).
-template <bool S16, int SP, bool UP>
+// in the staged LDS planes (row pitch SP). SEG: the segmentation mask's images are in `sg`.
+template <bool S16, int SP, bool UP, bool SEG = false>
 __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
                                          int x, int y, int l, const float* __restrict__ s0,
                                          const float* __restrict__ s1, const float* __restrict__ s2,
-                                         const float* __restrict__ sd, float uvx, float uvy, bool write_aux) {
+                                         const float* __restrict__ sd, float uvx, float uvy, bool write_aux,
+                                         const SegDesc* sg = nullptr) {
     // uvx, uvy: tc_to_uv(x, y) = ((x, y) + 0.5) / size (taa.comp:131), tabulated per tile
     const int W = fd.W, H = fd.H;
     const uint32_t gi = (uint32_t)(y * fd.pitch + x);
@@ -231,6 +310,20 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     float alpha = P.mAlpha;
     bool rejected = false;
 
+    // ---- segmentation mask (taa.comp:773-784, 959)
+    uint32_t segMaskValue = 0u;
+    float segTotal = 0.0f;
+    if constexpr (SEG) {
+        if (P.mRayTraceAugment) {
+            segMaskValue = segmentation_value<S16, SP>(fd, *sg, U, P, x, y, hu, hv, sd, l, segTotal);
+            if ((segMaskValue & 3u) != 0u) {
+                alpha = P.mRejectionAlpha;
+                rejected = true;
+            }
+            if (write_aux) gstore<uint32_t>(sg->segmask, (uint32_t)(y * sg->pitch + x) * 4u, segMaskValue);
+        }
+    }
+
     // ---- rejection (taa.comp:787-823)
     if (P.mRejectOutside) {
         if (hu < 0.0f || hv < 0.0f || hu >= 1.0f || hv >= 1.0f) {
@@ -329,7 +422,11 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
             case 5: g = make_float4(pixelSpeed, 0.f, 0.f, 0.f); break;
             case 6: g = make_float4(R.screen.x, R.screen.y, R.screen.z, 1.0f); break;
             case 7: g = R.history; break;
-            case 8: g = make_float4(0.f, 0.f, 1.f, 0.f); break;
+            case 8:  // the mask category (taa.comp:932-938); category 3 does not occur
+                if ((segMaskValue & 3u) == 0u) g = make_float4(0.f, 0.f, 1.f, 0.f);
+                else if ((segMaskValue & 3u) == 1u) g = make_float4(1.f, 0.f, 0.f, 0.f);
+                else if ((segMaskValue & 3u) == 2u) g = make_float4(1.f, 1.f, 0.f, 0.f);
+                break;
             default: break;
         }
         const float ds = P.mDebugScale;
@@ -356,7 +453,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         w[TAA_INSPECT_DEPTH_TC_Y] = dty;
         w[TAA_INSPECT_HIST_UV_X] = __float_as_int(hu);
         w[TAA_INSPECT_HIST_UV_Y] = __float_as_int(hv);
-        w[15] = 0;
+        w[TAA_INSPECT_SEG_TOTAL] = __float_as_int(segTotal);
         st_inspect(fd.inspect, gi, w);
     }
     return R;
@@ -365,8 +462,9 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
 // ------------------------------------------------------------------------------------------
 // one tile
 
-template <class G, bool S16>
-__device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm) {
+template <class G, bool S16, bool SEG = false>
+__device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm,
+                                             const SegDesc* sg = nullptr) {
     const taa_params_t& P = U.param[fd.param_idx];
     const int W = fd.W, H = fd.H, pitch = fd.pitch;
     const int ty = tile / fd.tiles_x, tx = tile - ty * fd.tiles_x;
@@ -398,8 +496,8 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         const int cx = clampi(px, 0, W - 1), cy = clampi(py, 0, H - 1);
         const bool owned = rx >= G::RA && rx < G::RA + G::TW && ry >= G::RA && ry < G::RA + G::TH && px < fd.x_end && py < H;
         const int l = (cy - gy0) * G::SP + (cx - gx0);
-        PixelResult r = resolve_pixel_f32<S16, G::SP, G::UP>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
-                                                             sm.ux[cx - gx0], sm.uy[cy - gy0], owned);
+        PixelResult r = resolve_pixel_f32<S16, G::SP, G::UP, SEG>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
+                                                                  sm.ux[cx - gx0], sm.uy[cy - gy0], owned, sg);
         if (owned) {
             const uint32_t gi = (uint32_t)(cy * pitch + cx);
             st_rgba<S16>(fd.history_out, gi, r.history);
@@ -447,6 +545,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3
 
 // Upsampling path (input size != output size, or mUpsampling): exact FP32 arithmetic, inputs read
 // from global memory around each output texel's input position, fused CAS as above.
+// the resolve with the segmentation mask of mRayTraceAugment
+template <int TW, int TH, int PAD, bool CAS, bool S16>
+__global__ __launch_bounds__(256) void k_resolve_seg(const ResolveArgs a, const SegDesc sg) {
+    using G = Geo<TW, TH, PAD, CAS>;
+    __shared__ typename G::Smem sm;
+    const int n = a.fd.tiles_x * a.fd.tiles_y;
+    resolve_tile<G, S16, true>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm, &sg);
+}
+
 template <int TW, int TH, int PAD, bool CAS, bool S16>
 __global__ __launch_bounds__(256) void k_resolve_up(const ResolveArgs a) {
     using G = Geo<TW, TH, PAD, CAS, true>;
@@ -542,6 +649,24 @@ int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, void* stream
     } else {
         if (s16) hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, true>), dim3(n), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, false>), dim3(n), dim3(256), 0, st, a);
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    ResolveArgs a;
+    a.fd = fd;
+    a.U = U;
+    const int n = fd.tiles_x * fd.tiles_y;
+    const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
+    constexpr TileShapeDef C = kShapesCas[kDefaultShapeCas], Pl = kShapesPlain[kDefaultShapePlain];
+    if (fd.cas_on) {
+        if (s16) hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, true>), dim3(n), dim3(256), 0, st, a, sg);
+        else hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, false>), dim3(n), dim3(256), 0, st, a, sg);
+    } else {
+        if (s16) hipLaunchKernelGGL((k_resolve_seg<Pl.tw, Pl.th, Pl.pad, false, true>), dim3(n), dim3(256), 0, st, a, sg);
+        else hipLaunchKernelGGL((k_resolve_seg<Pl.tw, Pl.th, Pl.pad, false, false>), dim3(n), dim3(256), 0, st, a, sg);
     }
     return (int)hipGetLastError();
 }

@@ -20,7 +20,9 @@ ARITH_FP32, ARITH_FP16 = 0, 1
 
 INSPECT_WORDS = 16
 INSPECT = dict(HIST_TC_X=0, HIST_TC_Y=1, CLOSEST_TAP=2, FLAGS=3, COLMIN=4, COLMAX=7, BLEND=10,
-               DEPTH_TC_X=11, DEPTH_TC_Y=12, HIST_UV_X=13, HIST_UV_Y=14)
+               DEPTH_TC_X=11, DEPTH_TC_Y=12, HIST_UV_X=13, HIST_UV_Y=14, SEG_TOTAL=15)
+RTFLAG = dict(OUT=0x1, DIS=0x2, NRM=0x4, DPT=0x8, MID=0x10, LUM=0x20, CNT=0x40, ALL=0x80, FXD=0x100, FXA=0x200)
+SEG_TAA, SEG_EDGE_AA, SEG_RAY_TRACE = 0, 1, 2
 FLAG_REJECTED, FLAG_RECTIFIED, FLAG_OUTSIDE, FLAG_DEPTH_REJECT, FLAG_USED_VELOCITY, FLAG_GHOST_REJECT = 1, 2, 4, 8, 16, 32
 
 
@@ -152,6 +154,17 @@ assert C.sizeof(CasConsts) == 32
 
 
 
+class SegmentationImages(C.Structure):
+    """taa_segmentation_images_t: the segmentation mask's extra images (taa.comp:34-37)."""
+    _fields_ = [
+        ("material", C.c_void_p), ("prev_material", C.c_void_p), ("uv_normal", C.c_void_p),
+        ("prev_segmask", C.c_void_p), ("segmask", C.c_void_p), ("pitch", C.c_int32), ("reserved", C.c_int32 * 3),
+    ]
+
+
+assert C.sizeof(SegmentationImages) == 56
+
+
 class Reproject(C.Structure):
     """taa_reproject_t (include/taastar/taa_velocity.h): the velocity producer's camera and objects."""
     _fields_ = [
@@ -174,6 +187,7 @@ PROTOTYPES = {
     "taa_halton": (F32, [I32, I32]),
     "taa_jitter_offset": (I32, [P(JitterConfig), U64, U32, U32, P(F32), P(U32)]),
     "taa_resolve_frame": (I32, [P(Uniforms), P(FrameImages), P(CasConsts), I32, VP]),
+    "taa_resolve_frame_segmented": (I32, [P(Uniforms), P(FrameImages), P(SegmentationImages), P(CasConsts), I32, VP]),
     "taa_cas_apply": (I32, [VP, VP, I32, I32, I32, I32, P(CasConsts), VP]),
     "taa_velocity_from_depth": (I32, [P(Reproject), VP, VP, VP, I32, I32, I32, I32, VP]),
     "taa_ctx_create": (I32, [I32, U64, P(VP)]),

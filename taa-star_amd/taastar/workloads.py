@@ -218,3 +218,23 @@ def camera_case(W, H, seed=0, n_objects=0, move=(5e-4, -3e-4, 1e-3), turn_deg=0.
         band = np.minimum((np.arange(H) * (n_objects + 2)) // max(H, 1), n_objects + 1).astype(np.uint32)
         ids = np.ascontiguousarray(np.broadcast_to(band[:, None], (H, W)))
     return dict(depth=np.ascontiguousarray(d, np.float32), ids=ids, reproject=r, objects=obj)
+
+
+# ---- segmentation-mask inputs ----------------------------------------------------------------
+
+def segmentation_inputs(depth, moving_every=5):
+    """G-buffer images of the segmentation mask for a frame of the seeded scene, whose polygons have
+    distinct constant depths: a polygon's material id is its depth rank modulo 61, plus 1, with bit
+    31 (moving object) on ids divisible by `moving_every`; its flat normal's angles (azimuth z in
+    [0, 2 pi), elevation w in [-pi/2, pi/2)) follow from the id. Returns (material [H, W] uint32,
+    uv_normal [H, W, 4] float32)."""
+    H, W = depth.shape
+    _, inv = np.unique(depth, return_inverse=True)
+    ids = (inv.reshape(H, W) % 61 + 1).astype(np.uint32)
+    mat = np.where(ids % moving_every == 0, ids | np.uint32(0x80000000), ids).astype(np.uint32)
+    uvn = np.empty((H, W, 4), np.float32)
+    uvn[..., 0] = ((np.arange(W) + 0.5) / W)[None, :]
+    uvn[..., 1] = ((np.arange(H) + 0.5) / H)[:, None]
+    uvn[..., 2] = (ids * 0.6180339887) % (2 * np.pi)
+    uvn[..., 3] = (ids * 0.3819660113) % np.pi - np.pi / 2
+    return np.ascontiguousarray(mat), uvn

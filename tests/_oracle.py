@@ -121,3 +121,40 @@ def velocity(R, depth, ids=None, objects=None, dtype=np.float32, threads=0):
             threads)
     assert rc == 0
     return out
+
+
+def resolve_seg(U, color4, depth, vel4, hist4, hdepth, material, prev_material, uv_normal, prev_segmask,
+                mask_init=None, dtype=np.float32, threads=0):
+    """One frame of the oracle with the segmentation mask (one size). Returns the resolve dict plus
+    `segmask`: mask_init (default zeros) with the texels the reference stores replaced."""
+    H, W = depth.shape
+    threads = threads or os.cpu_count() or 1
+    f, u = np.float32, np.uint32
+    arrs = [np.ascontiguousarray(a, t) for a, t in ((color4, f), (depth, f), (vel4, f), (hdepth, f))]
+    hist4 = np.ascontiguousarray(hist4, dtype)
+    mats = [np.ascontiguousarray(a, u) for a in (material, prev_material, prev_segmask)]
+    uvn = np.ascontiguousarray(uv_normal, f)
+    mask = np.zeros((H, W), u) if mask_init is None else np.array(mask_init, u, copy=True)
+    scr, his, dbg = (np.empty((H, W, 4), dtype) for _ in range(3))
+    ins = np.empty((H, W, 16), np.int32)
+    T = C.c_float if dtype == np.float32 else C.c_double
+    L = lib()
+    fn = L.oracle_resolve_seg_f32 if dtype == np.float32 else L.oracle_resolve_seg_f64
+    fp, tp, up = C.POINTER(C.c_float), C.POINTER(T), C.POINTER(C.c_uint32)
+    fn.argtypes = [C.c_void_p, C.c_int, C.c_int, fp, fp, fp, tp, fp, tp, tp, tp, C.POINTER(C.c_int32), up, up, fp, up, up,
+                   C.c_int]
+    rc = fn(C.byref(U), W, H, _p(arrs[0], C.c_float), _p(arrs[1], C.c_float), _p(arrs[2], C.c_float), _p(hist4, T),
+            _p(arrs[3], C.c_float), _p(scr, T), _p(his, T), _p(dbg, T), _p(ins, C.c_int32), _p(mats[0], C.c_uint32),
+            _p(mats[1], C.c_uint32), _p(uvn, C.c_float), _p(mats[2], C.c_uint32), _p(mask, C.c_uint32), threads)
+    assert rc == 0
+    return dict(screen=scr, history=his, debug=dbg, inspect=ins, segmask=mask)
+
+
+def seg_sincos(a):
+    a = np.ascontiguousarray(a, np.float32)
+    s, c = np.empty_like(a), np.empty_like(a)
+    L = lib()
+    fp = C.POINTER(C.c_float)
+    L.oracle_seg_sincos_f32.argtypes = [fp, fp, fp, C.c_int]
+    L.oracle_seg_sincos_f32(_p(a, C.c_float), _p(s, C.c_float), _p(c, C.c_float), a.size)
+    return s, c

@@ -751,3 +751,168 @@ def test_velocity_fp32_tracks_fp64(oracle):
     a = oracle.velocity(c["reproject"], c["depth"], c["ids"], c["objects"], dtype=np.float64)
     b = oracle.velocity(c["reproject"], c["depth"], c["ids"], c["objects"], dtype=np.float32)
     assert np.abs(a - b).max() < 2e-5
+
+
+# ---------------------------------------------------------------------------------------------
+# segmentation mask (taa.comp:566-702, 773-784, 932-938, 959)
+
+def seg_case(H, W, rng, flags, **weights):
+    """A frame with polygons of constant depth and colour, their material ids and normals, and a
+    previous frame equal to it; only `flags` set, indicator weights from `weights`."""
+    from taastar.workloads import segmentation_inputs
+    n = 12
+    cx, cy = rng.uniform(0, W, n), rng.uniform(0, H, n)
+    yy, xx = np.mgrid[0:H, 0:W]
+    lab = np.argmin((xx[..., None] - cx) ** 2 + (yy[..., None] - cy) ** 2, axis=-1)  # Voronoi cells
+    dep = rng.uniform(0.2, 0.9, n)[lab].astype(f32)
+    col = np.concatenate([rng.uniform(0.05, 2.0, (n, 3))[lab], np.ones((H, W, 1))], -1).astype(f32)
+    mat, uvn = segmentation_inputs(dep)
+    p = params(mRayTraceAugment=1, mRayTraceAugmentFlags=flags, mRayTraceHistoryCount=8,
+               mRayTraceAugment_WNrm=weights.get("nrm", 0.0), mRayTraceAugment_WDpt=weights.get("dpt", 0.0),
+               mRayTraceAugment_WMId=weights.get("mid", 0.0), mRayTraceAugment_WLum=weights.get("lum", 0.0),
+               mRayTraceAugment_Thresh=weights.get("thresh", 1e30), mRejectionAlpha=1.0, mAlpha=0.1)
+    v = np.zeros((H, W, 4), f32)
+    v[..., 3] = 1.0
+    hist = np.concatenate([rng.uniform(0.05, 2.0, (H, W, 3)), np.zeros((H, W, 1))], -1)
+    return dict(p=p, col=col, dep=dep, vel=v, hist=hist, mat=mat, prev_mat=mat.copy(), uvn=uvn,
+                prev_mask=np.zeros((H, W), np.uint32))
+
+
+def run_seg(oracle, c, U=None, dtype=np.float64, mask_init=None):
+    U = U if U is not None else uniforms(c["p"])
+    return oracle.resolve_seg(U, c["col"], c["dep"], c["vel"], np.asarray(c["hist"], dtype), c["dep"], c["mat"],
+                              c["prev_mat"], c["uvn"], c["prev_mask"], mask_init=mask_init, dtype=dtype, threads=4)
+
+
+def seg_total(o):
+    return o["inspect"][..., INSPECT["SEG_TOTAL"]].copy().view(f32).astype(np.float64)
+
+
+def test_seg_sincos_is_within_two_ulp(oracle):
+    a = np.concatenate([np.linspace(-10, 10, 400001), np.linspace(-200, 200, 40001)]).astype(f32)
+    s, c = oracle.seg_sincos(a)
+    a64 = a.astype(np.float64)
+    for got, ref in ((s, np.sin(a64)), (c, np.cos(a64))):
+        ulp = np.spacing(np.abs(ref).astype(f32)).astype(np.float64)
+        assert (np.abs(got - ref) / np.maximum(ulp, 2.0 ** -149)).max() < 2.0
+
+
+def test_seg_depth_and_luminance_indicators_are_sobel_magnitudes(oracle):
+    """taa.comp:651-660, 673-682: |Sobel| of linear depth and of YCoCg luma with clamped edges,
+    against scipy's Sobel operator."""
+    from scipy import ndimage
+    rng = np.random.default_rng(80)
+    H, W = 40, 56
+    for flag, key in ((taastar.RTFLAG["DPT"], "dpt"), (taastar.RTFLAG["LUM"], "lum")):
+        c = seg_case(H, W, rng, flag, **{key: 1.0})
+        o = run_seg(oracle, c)
+        if key == "dpt":
+            d = c["dep"].astype(np.float64)
+            img = 0.1 * 100.0 / (100.0 + d * (0.1 - 100.0))
+        else:
+            rgb = c["col"][..., :3].astype(np.float64)
+            img = 0.25 * rgb[..., 0] + 0.5 * rgb[..., 1] + 0.25 * rgb[..., 2]
+        ref = np.hypot(ndimage.sobel(img, 0, mode="nearest"), ndimage.sobel(img, 1, mode="nearest"))
+        np.testing.assert_allclose(seg_total(o), ref, rtol=1e-6, atol=1e-6)
+        assert ref.max() > 0.05
+        assert np.all(o["segmask"] == 0)   # threshold not reached
+
+
+def test_seg_normal_and_material_indicators(oracle):
+    """taa.comp:640-648: 1 - max(0, min of the 4-neighbour normal dots); :665-670: 1 where a
+    4-neighbour has another material id."""
+    rng = np.random.default_rng(81)
+    H, W = 36, 50
+    c = seg_case(H, W, rng, taastar.RTFLAG["NRM"], nrm=1.0)
+    o = run_seg(oracle, c)
+    z, w = c["uvn"][..., 2].astype(np.float64), c["uvn"][..., 3].astype(np.float64)
+    n = np.stack([np.cos(z) * np.cos(w), np.sin(z) * np.cos(w), np.sin(w)], -1)
+    pad = np.pad(n, ((1, 1), (1, 1), (0, 0)), mode="edge")
+    dots = [np.sum(n * pad[1 + dy:1 + dy + H, 1 + dx:1 + dx + W], -1) for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1))]
+    ref = 1.0 - np.maximum(0.0, np.min(dots, 0))
+    np.testing.assert_allclose(seg_total(o), ref, atol=1e-6)
+    assert ref.max() > 0.1
+    c = seg_case(H, W, rng, taastar.RTFLAG["MID"], mid=1.0)
+    o = run_seg(oracle, c)
+    m = np.pad(c["mat"], 1, mode="edge")
+    edge = np.zeros((H, W), bool)
+    for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+        edge |= m[1 + dy:1 + dy + H, 1 + dx:1 + dx + W] != c["mat"]
+    np.testing.assert_array_equal(seg_total(o), edge.astype(np.float64))
+    assert edge.any() and not edge.all()
+
+
+def test_seg_threshold_rejects_and_marks_for_ray_tracing(oracle):
+    """total >= threshold -> category 2 with the history count in bits 16-31; such texels take
+    mRejectionAlpha (taa.comp:778-781); debug mode 8 shows the category (:932-938)."""
+    rng = np.random.default_rng(82)
+    c = seg_case(32, 48, rng, taastar.RTFLAG["MID"] | taastar.RTFLAG["CNT"], mid=0.25, thresh=0.25)
+    c["p"].mDebugMode, c["p"].mDebugScale = 8, 1.0
+    for i in range(4):
+        c["p"].mDebugMask[i] = 1.0
+    o = run_seg(oracle, c)
+    edge = seg_total(o) >= 0.25
+    np.testing.assert_array_equal(o["segmask"], np.where(edge, 2 | (8 << 16), 0).astype(np.uint32))
+    flags = o["inspect"][..., INSPECT["FLAGS"]]
+    assert np.array_equal((flags & taastar.FLAG_REJECTED) != 0, edge)
+    blend = o["inspect"][..., INSPECT["BLEND"]].copy().view(f32)
+    assert np.all(blend[edge] == 1.0) and np.all(blend[~edge] == f32(0.1))
+    np.testing.assert_array_equal(o["debug"][edge], np.broadcast_to([1.0, 1.0, 0.0, 0.0], (edge.sum(), 4)))
+    np.testing.assert_array_equal(o["debug"][~edge], np.broadcast_to([0.0, 0.0, 1.0, 0.0], ((~edge).sum(), 4)))
+
+
+def test_seg_decision_paths(oracle):
+    """The early decisions of calc_segmentation_value: the fixed 100-texel border (category 1),
+    ray-trace-all (2), history outside the screen (1), disocclusion by a moving object at the
+    reprojected texel (2), and the history count kept from the previous mask (2, count - 1)."""
+    R = taastar.RTFLAG
+    rng = np.random.default_rng(83)
+    H, W = 220, 240
+    c = seg_case(H, W, rng, R["FXD"])
+    o = run_seg(oracle, c)
+    yy, xx = np.mgrid[0:H, 0:W]
+    border = (xx < 100) | (yy < 100) | (xx >= W - 100) | (yy >= H - 100)
+    np.testing.assert_array_equal(o["segmask"], border.astype(np.uint32))
+    c = seg_case(24, 40, rng, R["ALL"] | R["OUT"])
+    assert np.all(run_seg(oracle, c)["segmask"] == 2)
+    # a velocity of 6 texels to the right: the first 6 columns' history lies outside -> 1
+    c = seg_case(24, 40, rng, R["OUT"])
+    c["vel"][..., 0] = 6.0 / 40
+    m = run_seg(oracle, c)["segmask"]
+    assert np.all(m[:, :6] == 1) and np.all(m[:, 6:] == 0)
+    # disocclusion: the previous frame's texel under the history position belongs to a moving
+    # object (bit 31) with another id; static previous ids never trigger
+    c = seg_case(24, 40, rng, R["DIS"] | R["CNT"])
+    c["vel"][..., 1] = -3.0 / 24                      # history 3 rows below
+    prev = c["mat"] & np.uint32(0x7FFFFFFF)            # no moving objects ...
+    prev[10:14, :] = np.uint32(0x80000000 | 77)         # ... but these rows
+    prev[16:18, :] = np.uint32(78)
+    c["prev_mat"] = prev
+    m = run_seg(oracle, c)["segmask"]
+    hit = np.zeros((24, 40), bool)
+    hit[7:11, :] = True                               # y + 3 in rows 10..13 (rows 21..23 map outside: no test)
+    np.testing.assert_array_equal(m, np.where(hit, 2 | (8 << 16), 0).astype(np.uint32))
+    # history count: the previous mask's count decrements while the texel is not marked again
+    c = seg_case(24, 40, rng, R["CNT"])
+    c["prev_mask"][:, :10] = 2 | (3 << 16)
+    c["prev_mask"][:, 10:20] = 2 | (1 << 16)
+    m = run_seg(oracle, c)["segmask"]
+    assert np.all(m[:, :10] == 2 | (2 << 16)) and np.all(m[:, 10:20] == 2) and np.all(m[:, 20:] == 0)
+
+
+def test_seg_passthrough_leaves_the_mask(oracle):
+    rng = np.random.default_rng(84)
+    c = seg_case(16, 20, rng, taastar.RTFLAG["ALL"])
+    c["p"].mPassThrough = 1
+    init = np.full((16, 20), 12345, np.uint32)
+    assert np.all(run_seg(oracle, c, mask_init=init)["segmask"] == 12345)
+
+
+def test_seg_fp32_tracks_fp64(oracle):
+    rng = np.random.default_rng(85)
+    R = taastar.RTFLAG
+    c = seg_case(48, 64, rng, R["NRM"] | R["DPT"] | R["MID"] | R["LUM"], nrm=0.5, dpt=0.015, mid=0.25, lum=0.5,
+                 thresh=1e30)
+    a = seg_total(run_seg(oracle, c, dtype=np.float64))
+    b = seg_total(run_seg(oracle, c, dtype=np.float32))
+    np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-5)
