@@ -224,6 +224,101 @@ def time_velocity(W, H, vfmt_name, objects, steps, warmup):
     return r
 
 
+def time_stream(launch, steps, warmup):
+    """Average and median ms of `launch(stream)` over K back-to-back calls on a private stream (HIP
+    events between launches on that stream), after `warmup` untimed calls."""
+    import torch
+    stream = torch.cuda.Stream(device=torch.device("cuda:0"))
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            launch(stream)
+    stream.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    with torch.cuda.stream(stream):
+        ev[0].record(stream)
+        for i in range(steps):
+            launch(stream)
+            ev[i + 1].record(stream)
+    stream.synchronize()
+    per = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+    return ev[0].elapsed_time(ev[steps]) / steps, per[steps // 2]
+
+
+def time_edge_aa(W, H, steps, warmup):
+    """The edge-AA fallback over a W x H RGBA16F frame whose mask puts about 10% of the texels (in
+    8x8 blocks) in category 1."""
+    import numpy as np
+    import torch
+    import taastar
+    from taastar.workloads import encode_color, scene_for, workload
+
+    dev = torch.device("cuda:0")
+    col = np.clip(scene_for(workload("c3"), width=W, height=H).render(2)[0], 0.0, 1.0)
+    src = torch.from_numpy(encode_color(col, taastar.FMT_RGBA16F)).to(dev)
+    rng = np.random.default_rng(4)
+    blocks = (rng.uniform(0, 1, (H // 8 + 1, W // 8 + 1)) < 0.1).astype(np.int32)
+    mask = torch.from_numpy(np.ascontiguousarray(np.repeat(np.repeat(blocks, 8, 0), 8, 1)[:H, :W])).to(dev)
+    dst = torch.empty_like(src)
+    k = taastar.EdgeAaConsts()
+    taastar.check(taastar.lib().taa_edge_aa_defaults(W, H, C.byref(k)))
+    L = taastar.lib()
+    args = (C.c_void_p(src.data_ptr()), C.c_void_p(mask.data_ptr()), C.c_void_p(dst.data_ptr()), W, H, W, W,
+            taastar.FMT_RGBA16F, C.byref(k))
+    avg, med = time_stream(lambda st: L.taa_edge_aa_apply(*args, C.c_void_p(st.cuda_stream)), steps, warmup)
+    bpp = 8 + 4 + 8  # colour in, mask, colour out
+    r = dict(width=W, height=H, format="rgba16f", category1_fraction=round(float(blocks.mean()), 3), steps=steps,
+             gpix_per_s=round(W * H / (avg * 1e-3) / 1e9, 3), avg_ms=round(avg, 5), median_ms=round(med, 5),
+             bytes_per_pixel=bpp, achieved_gbs=round(bpp * W * H / (avg * 1e-3) / 1e9, 1),
+             hbm_frac=round(bpp * W * H / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+    del src, mask, dst
+    torch.cuda.empty_cache()
+    return r
+
+
+def time_segmented_resolve(W, H, steps, warmup):
+    """taa_resolve_frame_segmented with the reference's default segmentation flags and weights on a
+    W x H frame (RGBA32F, fused CAS), against the same resolve without the mask."""
+    import numpy as np
+    import torch
+    import taastar
+    from taastar.device import FrameBuffers
+    from taastar.workloads import (cas_for, encode_color, encode_velocity, params_c2, scene_for, segmentation_inputs,
+                                   uniforms_for, workload)
+
+    w = workload("c2")
+    sc = scene_for(w, width=W, height=H)
+    col, dep, vel4, jit = sc.render(1)
+    bufs = FrameBuffers(W, H, w["fmt"], w["vel"])
+    bufs.upload(encode_color(col, w["fmt"]), dep, encode_velocity(vel4, w["vel"]))
+    mat, uvn = segmentation_inputs(dep)
+    dev = torch.device("cuda:0")
+    mat_t = torch.from_numpy(mat.view(np.int32)).to(dev)
+    uvn_t = torch.from_numpy(uvn).to(dev)
+    masks = [torch.zeros(H, W, dtype=torch.int32, device=dev) for _ in range(2)]
+    p = params_c2()
+    p.mRayTraceAugment = 1
+    p.mRayTraceHistoryCount = 8
+    U = uniforms_for(w, 1, jit, params=p)
+    U0 = uniforms_for(w, 1, jit)
+    K = cas_for(w, W, H)
+    sg = taastar.SegmentationImages()
+    sg.material, sg.prev_material, sg.uv_normal = mat_t.data_ptr(), mat_t.data_ptr(), uvn_t.data_ptr()
+    sg.prev_segmask, sg.segmask = masks[0].data_ptr(), masks[1].data_ptr()
+    img = bufs.images()
+    L = taastar.lib()
+    seg = lambda st: L.taa_resolve_frame_segmented(C.byref(U), C.byref(img), C.byref(sg), C.byref(K), 0,
+                                                   C.c_void_p(st.cuda_stream))
+    plain = lambda st: L.taa_resolve_frame(C.byref(U0), C.byref(img), C.byref(K), 0, C.c_void_p(st.cuda_stream))
+    a_seg, m_seg = time_stream(seg, steps, warmup)
+    a_plain, _ = time_stream(plain, steps, warmup)
+    del bufs, mat_t, uvn_t, masks
+    torch.cuda.empty_cache()
+    return dict(width=W, height=H, flags="reference defaults", steps=steps, avg_ms=round(a_seg, 5),
+                median_ms=round(m_seg, 5), gpix_per_s=round(W * H / (a_seg * 1e-3) / 1e9, 3),
+                without_mask_avg_ms=round(a_plain, 5))
+
+
 def tune(W, H, fmt, arith, cas, iters=20):
     from taastar import check, lib
     ctx = C.c_void_p()
@@ -291,6 +386,13 @@ def extras(a):
         x[f"velocity_3840x2160_{vf}_{nobj}objects"] = r
         print(f"# velocity {vf} objects={nobj}: {r['gpix_per_s']:.1f} Gpix/s, {r['hbm_frac']:.3f} of HBM",
               file=sys.stderr, flush=True)
+    r = time_edge_aa(3840, 2160, 200, 20)
+    x["edge_aa_3840x2160_rgba16f"] = r
+    print(f"# edge-AA: {r['gpix_per_s']:.1f} Gpix/s, {r['hbm_frac']:.3f} of HBM", file=sys.stderr, flush=True)
+    r = time_segmented_resolve(1920, 1080, 100, 10)
+    x["resolve_segmented_1920x1080_fp32"] = r
+    print(f"# segmented resolve 1080p: {r['avg_ms']:.4f} ms (without mask {r['without_mask_avg_ms']:.4f} ms)",
+          file=sys.stderr, flush=True)
     for cls in RESOLUTIONS:  # autotune every class the mixed loads use (FP16 and FP32 storage)
         W, H = RESOLUTIONS[cls]
         tune(W, H, F16, A16, True)

@@ -253,6 +253,16 @@ typedef struct taa_segmentation_images {
     int32_t         reserved[3];   /* must be 0 */
 } taa_segmentation_images_t;
 
+/* Push constants of the edge-AA fallback pass: push_constants_for_fxaa (source/taa.hpp:93-98),
+ * byte-identical (20 bytes); defaults subpix 0.75, edge_threshold 0.116, edge_threshold_min 0.0833. */
+typedef struct taa_edge_aa_consts {
+    float rcp_frame[2];       /* 1 / output size (the pass steps in whole texels; kept for layout) */
+    float subpix;             /* amount of sub-texel smoothing, [0, 1]                             */
+    float edge_threshold;     /* local luma contrast, relative to the local maximum, below which a
+                                 texel is kept                                                      */
+    float edge_threshold_min; /* absolute contrast below which a texel is kept (dark regions)     */
+} taa_edge_aa_consts_t;
+
 /* ---- per-frame GPU entry points ---------------------------------------------------------- */
 
 /* One taa.comp dispatch over the whole frame (source/taa.hpp:1008-1027) and, when `cas` is not
@@ -282,6 +292,24 @@ int32_t taa_resolve_frame_segmented(const taa_uniforms_t* uniforms, const taa_fr
  * alias. Border texels are read clamped to the edge. */
 int32_t taa_cas_apply(const void* src, void* dst, int32_t width, int32_t height, int32_t pitch,
                       int32_t format, const taa_cas_consts_t* cas, void* stream);
+
+/* The edge-AA fallback on the texels the segmentation mask puts in category 1 (the prepare and
+ * FXAA passes of source/taa.hpp:1061-1107, antialias_fxaa_prepare.comp, antialias_fxaa.comp).
+ * Every texel of dst gets alpha = luma = dot(rgb, (0.299, 0.587, 0.114)) of src, as the prepare
+ * pass writes; texels whose mask category is not 1 keep src's rgb (antialias_fxaa.comp:55-77).
+ * A category-1 texel whose 4-neighbour luma range reaches max(edge_threshold_min,
+ * edge_threshold * local max) is blended with one 4-neighbour across the edge: vertically when the
+ * vertical luma Sobel gradient is at least the horizontal one, else horizontally, towards the side
+ * with the larger luma step, with weight subpix / 2 * clamp(|weighted 3x3 mean luma - luma| /
+ * range, 0, 1) (mean weights: 2 for the 4-neighbours, 1 for the diagonals).
+ * This is the library's own filter, not NVIDIA's FXAA: results differ from the reference's
+ * pass on category-1 texels. src and dst in `format` with rows of `pitch` texels (0 = width),
+ * segmask R32UI with rows of `seg_pitch` (0 = pitch); src and dst must not alias. */
+int32_t taa_edge_aa_apply(const void* src, const uint32_t* segmask, void* dst, int32_t width, int32_t height,
+                          int32_t pitch, int32_t seg_pitch, int32_t format, const taa_edge_aa_consts_t* consts,
+                          void* stream);
+/* The reference's defaults of push_constants_for_fxaa for an output of width x height. */
+int32_t taa_edge_aa_defaults(int32_t width, int32_t height, taa_edge_aa_consts_t* out);
 
 /* ---- context, sessions, batches ---------------------------------------------------------- */
 

@@ -15,6 +15,10 @@
 //                              decision paths (border, all, outside, moving-object disocclusion,
 //                              history count) on constructed frames; FP32 sin/cos within a few
 //                              ulp of the library functions
+//   edge-AA fallback           closed forms: texels outside category 1 and flat or low-contrast
+//                              texels copied with alpha = luma; a straight step edge blends each
+//                              side by subpix/6 towards the other; transposing or mirroring the
+//                              image transposes or mirrors the result
 //   velocity producer          jitter cancels on a still camera; closed-form shifts of a
 //                              translated perspective camera and of a moved object; with zero
 //                              jitter uv - v.xy and depth - v.z equal the resolve's matrix path
@@ -104,6 +108,27 @@ int cas_frame(const T* src, T* dst, int W, int H, uint32_t const1_x, int threads
 }
 
 template <class T>
+int edge_aa_frame(const T* src, const uint32_t* mask, T* dst, int W, int H, T subpix, T thr, T thr_min, int threads) {
+    if (!src || !mask || !dst || W <= 0 || H <= 0) return -1;
+#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            auto at = [&](int dx, int dy) {
+                const int xx = oracle::iclamp(x + dx, 0, W - 1), yy = oracle::iclamp(y + dy, 0, H - 1);
+                const T* p = src + 4 * ((size_t)yy * W + xx);
+                return oracle::v3<T>{p[0], p[1], p[2]};
+            };
+            const size_t i = (size_t)y * W + x;
+            oracle::v4<T> o = oracle::edge_aa_texel<T>(at, (mask[i] & 3u) == 1u, subpix, thr, thr_min);
+            dst[4 * i + 0] = o.x;
+            dst[4 * i + 1] = o.y;
+            dst[4 * i + 2] = o.z;
+            dst[4 * i + 3] = o.w;
+        }
+    return 0;
+}
+
+template <class T>
 int velocity_frame(const taa_reproject_t& R, const float* depth, const uint32_t* ids, int W, int H, T* out,
                    int threads) {
 #pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
@@ -161,6 +186,17 @@ int oracle_resolve_seg_f64(const taa_uniforms_t* U, int W, int H, const float* c
 // sin / cos of the segmentation normals in FP32 (the evaluation the kernels repeat), elementwise.
 void oracle_seg_sincos_f32(const float* a, float* s, float* c, int n) {
     for (int i = 0; i < n; ++i) oracle::seg_sincos(a[i], s[i], c[i]);
+}
+
+// Edge-AA fallback over a W x H RGBA image with its R32UI mask (category = mask & 3).
+int oracle_edge_aa_f32(const float* src, const uint32_t* mask, float* dst, int W, int H, float subpix, float thr,
+                       float thr_min, int threads) {
+    return edge_aa_frame<float>(src, mask, dst, W, H, subpix, thr, thr_min, threads);
+}
+
+int oracle_edge_aa_f64(const double* src, const uint32_t* mask, double* dst, int W, int H, double subpix, double thr,
+                       double thr_min, int threads) {
+    return edge_aa_frame<double>(src, mask, dst, W, H, subpix, thr, thr_min, threads);
 }
 
 // Velocity image (xyzw per texel) of a W x H depth image; ids may be NULL (all static). The object

@@ -865,6 +865,36 @@ inline float halton(int i, int b) {
     return r;
 }
 
+// ---- edge-AA fallback (the library's own filter for category-1 texels) ------------------------
+// luma of the prepare pass (antialias_fxaa_prepare.comp:21-23)
+template <class T> inline T aa_luma(v3<T> c) { return c.x * T(0.299) + c.y * T(0.587) + c.z * T(0.114); }
+
+// One texel of taa_edge_aa_apply (include/taastar/taa_abi.h states the filter). `at(dx, dy)` gives
+// the clamped neighbour's rgb; `category1` says whether the mask selects the texel.
+template <class T, class At>
+inline v4<T> edge_aa_texel(At at, bool category1, T subpix, T edge_threshold, T edge_threshold_min) {
+    const v3<T> cC = at(0, 0);
+    const T lC = aa_luma(cC);
+    if (!category1) return {cC.x, cC.y, cC.z, lC};
+    const T lN = aa_luma(at(0, -1)), lS = aa_luma(at(0, 1)), lW = aa_luma(at(-1, 0)), lE = aa_luma(at(1, 0));
+    const T lNW = aa_luma(at(-1, -1)), lNE = aa_luma(at(1, -1)), lSW = aa_luma(at(-1, 1)), lSE = aa_luma(at(1, 1));
+    const T lmin = gmin(lC, gmin(gmin(lN, lS), gmin(lW, lE)));
+    const T lmax = gmax(lC, gmax(gmax(lN, lS), gmax(lW, lE)));
+    const T range = lmax - lmin;
+    if (range < gmax(edge_threshold_min, lmax * edge_threshold)) return {cC.x, cC.y, cC.z, lC};
+    const T gx = (lNE + T(2) * lE + lSE) - (lNW + T(2) * lW + lSW);
+    const T gy = (lSW + T(2) * lS + lSE) - (lNW + T(2) * lN + lNE);
+    int dx = 0, dy = 0;
+    if (std::fabs(gy) >= std::fabs(gx)) dy = std::fabs(lN - lC) >= std::fabs(lS - lC) ? -1 : 1;
+    else dx = std::fabs(lW - lC) >= std::fabs(lE - lC) ? -1 : 1;
+    const T avg = (((lN + lS) + (lW + lE)) * T(2) + ((lNW + lNE) + (lSW + lSE))) / T(12);
+    const T s = gclamp(std::fabs(avg - lC) / range, T(0), T(1));
+    const T blend = subpix * T(0.5) * s;
+    const v3<T> cX = at(dx, dy);
+    return {cC.x * (T(1) - blend) + cX.x * blend, cC.y * (T(1) - blend) + cX.y * blend,
+            cC.z * (T(1) - blend) + cX.z * blend, lC};
+}
+
 // ---- velocity producer ----------------------------------------------------------------------
 // The motion vector of one texel as the geometry pass writes it (fwd_geometry.frag:288-295),
 // with the current clip position taken from the texel centre and its depth instead of from the

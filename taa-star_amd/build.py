@@ -32,6 +32,7 @@ SOURCES = [
     ("taa_kernels.hip", "hip"),
     ("taa_fast.hip", "hip"),
     ("taa_velocity.hip", "hip"),
+    ("taa_edge_aa.hip", "hip"),
     ("taa_api.cpp", "cxx"),
     ("taa_host.cpp", "cxx"),
     ("taa_scene.cpp", "cxx"),

@@ -36,6 +36,7 @@ static_assert(offsetof(taa_uniforms_t, mResetHistory) == 528, "layout");
 static_assert(offsetof(taa_uniforms_t, mCamNearPlane) == 532, "layout");
 static_assert(offsetof(taa_uniforms_t, pad1) == 540, "layout");
 static_assert(sizeof(taa_cas_consts_t) == 32, "push_constants_for_cas must be 32 bytes (source/taa.hpp:88-91)");
+static_assert(sizeof(taa_edge_aa_consts_t) == 20, "push_constants_for_fxaa is 20 bytes (source/taa.hpp:93-98)");
 static_assert(sizeof(taastar::BatchEntry) % 16 == 0, "BatchEntry alignment");
 
 namespace {
@@ -353,6 +354,37 @@ int32_t taa_cas_apply(const void* src, void* dst, int32_t width, int32_t height,
     if (!fits_offsets(pitch, height, 16)) return fail(TAA_ERR_INVALID_ARG, "image exceeds 4 GiB");
     int e = taastar::launch_cas(src, dst, width, height, pitch, format, cas->const1[0], stream);
     if (e) return hip_fail((hipError_t)e, "cas launch");
+    return ok();
+}
+
+// the prepare + FXAA passes of taa.hpp:1061-1107, with the library's own edge filter
+int32_t taa_edge_aa_apply(const void* src, const uint32_t* segmask, void* dst, int32_t width, int32_t height,
+                          int32_t pitch, int32_t seg_pitch, int32_t format, const taa_edge_aa_consts_t* k, void* stream) {
+    if (!src || !segmask || !dst || !k) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
+    if (src == dst) return fail(TAA_ERR_INVALID_ARG, "src and dst alias");
+    if (static_cast<const void*>(segmask) == dst) return fail(TAA_ERR_INVALID_ARG, "segmask and dst alias");
+    if (width <= 0 || height <= 0) return fail(TAA_ERR_INVALID_ARG, "bad size %dx%d", width, height);
+    if (!valid_color_fmt(format)) return fail(TAA_ERR_INVALID_ARG, "bad format %d", format);
+    if (pitch == 0) pitch = width;
+    if (seg_pitch == 0) seg_pitch = pitch;
+    if (pitch < width || seg_pitch < width) return fail(TAA_ERR_INVALID_ARG, "pitch < width");
+    if (!fits_offsets(pitch, height, 16) || !fits_offsets(seg_pitch, height, 4))
+        return fail(TAA_ERR_INVALID_ARG, "image exceeds 4 GiB");
+    int e = taastar::launch_edge_aa(src, segmask, dst, width, height, pitch, seg_pitch, format, k->subpix, k->edge_threshold,
+                                    k->edge_threshold_min, stream);
+    if (e) return hip_fail((hipError_t)e, "edge-AA launch");
+    return ok();
+}
+
+int32_t taa_edge_aa_defaults(int32_t width, int32_t height, taa_edge_aa_consts_t* out) {
+    if (!out) return fail(TAA_ERR_INVALID_ARG, "out is NULL");
+    if (width <= 0 || height <= 0) return fail(TAA_ERR_INVALID_ARG, "bad size %dx%d", width, height);
+    // source/taa.hpp:94-97, 953
+    out->rcp_frame[0] = 1.0f / (float)width;
+    out->rcp_frame[1] = 1.0f / (float)height;
+    out->subpix = 0.75f;
+    out->edge_threshold = 0.116f;
+    out->edge_threshold_min = 0.0833f;
     return ok();
 }
 

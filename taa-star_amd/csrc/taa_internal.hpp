@@ -90,6 +90,10 @@ int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape,
 int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16,
                       int grid, void* stream);
 
+// Edge-AA fallback on category-1 texels of a segmentation mask (taa_edge_aa.hip).
+int launch_edge_aa(const void* src, const uint32_t* mask, void* dst, int W, int H, int pitch, int seg_pitch, int fmt,
+                   float subpix, float thr, float thr_min, void* stream);
+
 // Velocity producer (taa_velocity.hip): one launch over width x height; ids may be NULL.
 int launch_velocity(const taa_reproject_t& r, const float* depth, const uint32_t* ids, void* velocity, int fmt, int W,
                     int H, int pitch, void* stream);

@@ -11,12 +11,13 @@ from ._abi import (ARITH_FP16, ARITH_FP32, FLAG_DEPTH_REJECT, FLAG_GHOST_REJECT,
                    FLAG_REJECTED, FLAG_USED_VELOCITY, FMT_RGBA16F, FMT_RGBA32F, INSPECT, INSPECT_WORDS, LIB_PATH,
                    PROTOTYPES, VEL_RG16F, VEL_RG32F, VEL_RGBA16F, VEL_RGBA32F, BatchItem, CasConsts, FrameImages,
                    JitterConfig, Params, SceneConfig, ServeReport, ServerConfig, ServerStats, ServeSimConfig,
-                   RTFLAG, SEG_EDGE_AA, SEG_RAY_TRACE, SEG_TAA, Completion, Reproject, SegmentationImages, SessionIO,
+                   RTFLAG, SEG_EDGE_AA, SEG_RAY_TRACE, SEG_TAA, Completion, EdgeAaConsts, Reproject, SegmentationImages,
+                   SessionIO,
                    TaaError, TileShape, Uniforms, check, lib)
 
 __all__ = [
     "lib", "check", "TaaError", "Params", "Uniforms", "CasConsts", "FrameImages", "JitterConfig", "SessionIO",
-    "BatchItem", "Reproject", "SegmentationImages", "RTFLAG", "SEG_TAA", "SEG_EDGE_AA", "SEG_RAY_TRACE", "TileShape", "SceneConfig", "ServerConfig", "ServerStats", "ServeSimConfig", "ServeReport",
+    "BatchItem", "EdgeAaConsts", "Reproject", "SegmentationImages", "RTFLAG", "SEG_TAA", "SEG_EDGE_AA", "SEG_RAY_TRACE", "TileShape", "SceneConfig", "ServerConfig", "ServerStats", "ServeSimConfig", "ServeReport",
     "Completion", "default_params", "cas_setup", "halton", "jitter_offset", "Scene",
     "FMT_RGBA32F", "FMT_RGBA16F", "VEL_RG32F", "VEL_RG16F", "VEL_RGBA32F", "VEL_RGBA16F", "ARITH_FP32",
     "ARITH_FP16", "INSPECT", "INSPECT_WORDS", "LIB_PATH", "PROTOTYPES",

@@ -154,6 +154,15 @@ assert C.sizeof(CasConsts) == 32
 
 
 
+class EdgeAaConsts(C.Structure):
+    """push_constants_for_fxaa (source/taa.hpp:93-98), 20 bytes."""
+    _fields_ = [("rcp_frame", C.c_float * 2), ("subpix", C.c_float), ("edge_threshold", C.c_float),
+                ("edge_threshold_min", C.c_float)]
+
+
+assert C.sizeof(EdgeAaConsts) == 20
+
+
 class SegmentationImages(C.Structure):
     """taa_segmentation_images_t: the segmentation mask's extra images (taa.comp:34-37)."""
     _fields_ = [
@@ -189,6 +198,8 @@ PROTOTYPES = {
     "taa_resolve_frame": (I32, [P(Uniforms), P(FrameImages), P(CasConsts), I32, VP]),
     "taa_resolve_frame_segmented": (I32, [P(Uniforms), P(FrameImages), P(SegmentationImages), P(CasConsts), I32, VP]),
     "taa_cas_apply": (I32, [VP, VP, I32, I32, I32, I32, P(CasConsts), VP]),
+    "taa_edge_aa_apply": (I32, [VP, VP, VP, I32, I32, I32, I32, I32, P(EdgeAaConsts), VP]),
+    "taa_edge_aa_defaults": (I32, [I32, I32, P(EdgeAaConsts)]),
     "taa_velocity_from_depth": (I32, [P(Reproject), VP, VP, VP, I32, I32, I32, I32, VP]),
     "taa_ctx_create": (I32, [I32, U64, P(VP)]),
     "taa_ctx_destroy": (I32, [VP]),

@@ -158,3 +158,18 @@ def seg_sincos(a):
     L.oracle_seg_sincos_f32.argtypes = [fp, fp, fp, C.c_int]
     L.oracle_seg_sincos_f32(_p(a, C.c_float), _p(s, C.c_float), _p(c, C.c_float), a.size)
     return s, c
+
+
+def edge_aa(src4, mask, subpix=0.75, thr=0.116, thr_min=0.0833, dtype=np.float32, threads=0):
+    """The edge-AA fallback over an RGBA image (H, W, 4) with its uint32 mask."""
+    H, W = mask.shape
+    threads = threads or os.cpu_count() or 1
+    src = np.ascontiguousarray(src4, dtype)
+    m = np.ascontiguousarray(mask, np.uint32)
+    dst = np.empty_like(src)
+    T = C.c_float if dtype == np.float32 else C.c_double
+    L = lib()
+    fn = L.oracle_edge_aa_f32 if dtype == np.float32 else L.oracle_edge_aa_f64
+    fn.argtypes = [C.POINTER(T), C.POINTER(C.c_uint32), C.POINTER(T), C.c_int, C.c_int, T, T, T, C.c_int]
+    assert fn(_p(src, T), _p(m, C.c_uint32), _p(dst, T), W, H, subpix, thr, thr_min, threads) == 0
+    return dst

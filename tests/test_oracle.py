@@ -916,3 +916,73 @@ def test_seg_fp32_tracks_fp64(oracle):
     a = seg_total(run_seg(oracle, c, dtype=np.float64))
     b = seg_total(run_seg(oracle, c, dtype=np.float32))
     np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-5)
+
+
+# ---------------------------------------------------------------------------------------------
+# edge-AA fallback on category-1 texels
+
+def luma64(rgb):
+    rgb = rgb.astype(np.float64)
+    return 0.299 * rgb[..., 0] + 0.587 * rgb[..., 1] + 0.114 * rgb[..., 2]
+
+
+def test_edge_aa_copies_outside_category_1_and_flat_regions(oracle):
+    """Texels of categories 0 and 2, and category-1 texels without enough contrast, keep their rgb;
+    every texel's alpha is its luma (antialias_fxaa_prepare.comp:21-23, antialias_fxaa.comp:55-77)."""
+    rng = np.random.default_rng(90)
+    H, W = 20, 30
+    src = np.concatenate([rng.uniform(0, 1, (H, W, 3)), np.zeros((H, W, 1))], -1)
+    mask = rng.choice(np.array([0, 2, 2 | (5 << 16)], np.uint32), (H, W))
+    o = oracle.edge_aa(src, mask, dtype=np.float64)
+    np.testing.assert_array_equal(o[..., :3], src[..., :3])
+    np.testing.assert_allclose(o[..., 3], luma64(src), rtol=1e-15)
+    flat = np.concatenate([np.full((H, W, 3), 0.4), np.zeros((H, W, 1))], -1)
+    flat[5, 7, :3] = 0.42                       # contrast 0.02 < max(0.0833, 0.116 * 0.42)
+    o = oracle.edge_aa(flat, np.ones((H, W), np.uint32), dtype=np.float64)
+    np.testing.assert_array_equal(o[..., :3], flat[..., :3])
+
+
+def test_edge_aa_step_edge_closed_form(oracle):
+    """A vertical step edge (luma 0 | 1), all texels in category 1: the two columns at the edge
+    move towards each other by subpix / 6 (weighted mean 1/3 or 2/3 of the range), all other
+    texels stay; with mirroring and transposition the result mirrors and transposes."""
+    H, W, e = 12, 16, 7
+    img = np.zeros((H, W, 4))
+    img[:, e + 1:, :3] = 1.0
+    mask = np.ones((H, W), np.uint32)
+    for subpix in (0.75, 1.0, 0.3):
+        o = oracle.edge_aa(img, mask, subpix=subpix, dtype=np.float64)
+        ref = img[..., :3].copy()
+        ref[:, e] = subpix / 6
+        ref[:, e + 1] = 1 - subpix / 6
+        np.testing.assert_allclose(o[..., :3], ref, atol=1e-12)
+        om = oracle.edge_aa(img[:, ::-1], mask, subpix=subpix, dtype=np.float64)
+        np.testing.assert_allclose(om, o[:, ::-1], atol=1e-12)
+        ot = oracle.edge_aa(np.ascontiguousarray(img.transpose(1, 0, 2)), mask.T.copy(), subpix=subpix, dtype=np.float64)
+        np.testing.assert_allclose(ot, o.transpose(1, 0, 2), atol=1e-12)
+    # only masked texels change: mask the edge's left column only
+    m2 = np.zeros((H, W), np.uint32)
+    m2[:, e] = 1
+    o = oracle.edge_aa(img, m2, dtype=np.float64)
+    assert np.allclose(o[:, e, :3], 0.125) and np.array_equal(o[:, e + 1, :3], img[:, e + 1, :3])
+
+
+def test_edge_aa_thresholds(oracle):
+    """A step of contrast c on a background b: filtered iff c >= max(thr_min, thr * (b + c))."""
+    H, W, e = 8, 10, 4
+    for b, c, expect in ((0.5, 0.05, False), (0.5, 0.09, True), (1.0, 0.12, False), (1.0, 0.14, True),
+                         (0.01, 0.08, False), (0.01, 0.09, True)):
+        img = np.full((H, W, 4), b)
+        img[:, e + 1:, :3] = b + c
+        o = oracle.edge_aa(img, np.ones((H, W), np.uint32), thr=0.116, thr_min=0.0833, dtype=np.float64)
+        changed = not np.array_equal(o[..., :3], img[..., :3])
+        assert changed == expect, (b, c)
+
+
+def test_edge_aa_fp32_tracks_fp64(oracle):
+    rng = np.random.default_rng(91)
+    src = np.concatenate([rng.uniform(0, 1, (40, 50, 3)), np.ones((40, 50, 1))], -1).astype(f32)
+    mask = rng.integers(0, 3, (40, 50)).astype(np.uint32)
+    a = oracle.edge_aa(src, mask, dtype=np.float64)
+    b = oracle.edge_aa(src, mask, dtype=np.float32)
+    np.testing.assert_allclose(b, a, atol=2e-6)
