@@ -37,6 +37,7 @@ SOURCES = [
     ("taa_host.cpp", "cxx"),
     ("taa_scene.cpp", "cxx"),
     ("taa_serve.cpp", "cxx"),
+    ("taa_settings.cpp", "cxx"),
 ]
 
 

@@ -911,3 +911,7 @@ int32_t taa_autotune(taa_ctx_t* c, int32_t w, int32_t h, int32_t fmt, int32_t ar
 }
 
 }  // extern "C"
+
+namespace taastar {
+int set_last_error(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace taastar

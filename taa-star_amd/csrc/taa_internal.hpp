@@ -94,6 +94,9 @@ int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tile
 int launch_edge_aa(const void* src, const uint32_t* mask, void* dst, int W, int H, int pitch, int seg_pitch, int fmt,
                    float subpix, float thr, float thr_min, void* stream);
 
+// Record `msg` as the calling thread's taa_last_error() and return `code` (taa_api.cpp).
+int set_last_error(int code, const char* msg);
+
 // Velocity producer (taa_velocity.hip): one launch over width x height; ids may be NULL.
 int launch_velocity(const taa_reproject_t& r, const float* depth, const uint32_t* ids, void* velocity, int fmt, int W,
                     int H, int pitch, void* stream);

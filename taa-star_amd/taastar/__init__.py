@@ -12,12 +12,12 @@ from ._abi import (ARITH_FP16, ARITH_FP32, FLAG_DEPTH_REJECT, FLAG_GHOST_REJECT,
                    PROTOTYPES, VEL_RG16F, VEL_RG32F, VEL_RGBA16F, VEL_RGBA32F, BatchItem, CasConsts, FrameImages,
                    JitterConfig, Params, SceneConfig, ServeReport, ServerConfig, ServerStats, ServeSimConfig,
                    RTFLAG, SEG_EDGE_AA, SEG_RAY_TRACE, SEG_TAA, Completion, EdgeAaConsts, Reproject, SegmentationImages,
-                   SessionIO,
+                   SessionIO, Settings,
                    TaaError, TileShape, Uniforms, check, lib)
 
 __all__ = [
     "lib", "check", "TaaError", "Params", "Uniforms", "CasConsts", "FrameImages", "JitterConfig", "SessionIO",
-    "BatchItem", "EdgeAaConsts", "Reproject", "SegmentationImages", "RTFLAG", "SEG_TAA", "SEG_EDGE_AA", "SEG_RAY_TRACE", "TileShape", "SceneConfig", "ServerConfig", "ServerStats", "ServeSimConfig", "ServeReport",
+    "BatchItem", "Settings", "read_settings", "write_settings", "EdgeAaConsts", "Reproject", "SegmentationImages", "RTFLAG", "SEG_TAA", "SEG_EDGE_AA", "SEG_RAY_TRACE", "TileShape", "SceneConfig", "ServerConfig", "ServerStats", "ServeSimConfig", "ServeReport",
     "Completion", "default_params", "cas_setup", "halton", "jitter_offset", "Scene",
     "FMT_RGBA32F", "FMT_RGBA16F", "VEL_RG32F", "VEL_RG16F", "VEL_RGBA32F", "VEL_RGBA16F", "ARITH_FP32",
     "ARITH_FP16", "INSPECT", "INSPECT_WORDS", "LIB_PATH", "PROTOTYPES",
@@ -106,3 +106,25 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+def read_settings(text, base=None):
+    """Apply the TAA sections of an INI text (the reference's settings file) to `base` (default:
+    the reference's defaults); returns the Settings."""
+    s = Settings()
+    if base is None:
+        check(lib().taa_settings_defaults(C.byref(s)))
+    else:
+        C.memmove(C.byref(s), C.byref(base), C.sizeof(Settings))
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    check(lib().taa_settings_read_ini(raw, len(raw), C.byref(s)))
+    return s
+
+
+def write_settings(settings):
+    """The TAA sections of a settings file, as the reference writes them."""
+    n = C.c_size_t()
+    lib().taa_settings_write_ini(C.byref(settings), None, 0, C.byref(n))
+    buf = C.create_string_buffer(n.value + 1)
+    check(lib().taa_settings_write_ini(C.byref(settings), buf, n.value + 1, C.byref(n)))
+    return buf.value.decode()

@@ -1,4 +1,4 @@
-"""ctypes mirror of include/taastar/taa_abi.h, taa_velocity.h and taa_scene.h (struct layouts are asserted)."""
+"""ctypes mirror of include/taastar/taa_abi.h, taa_velocity.h, taa_settings.h and taa_scene.h (struct layouts are asserted)."""
 import ctypes as C
 import os
 
@@ -163,6 +163,22 @@ class EdgeAaConsts(C.Structure):
 assert C.sizeof(EdgeAaConsts) == 20
 
 
+TAA_SETTINGS_MAX_SAMPLE_OFFSETS = 256
+
+
+class Settings(C.Structure):
+    """taa_settings_t (include/taastar/taa_settings.h): the TAA sections of the settings file."""
+    _fields_ = [
+        ("param", Params * 2), ("taa_enabled", C.c_int32), ("sample_distribution", C.c_int32),
+        ("sharpener", C.c_int32), ("sharpen_factor", C.c_float), ("split_screen", C.c_int32), ("split_x", C.c_int32),
+        ("fixed_jitter_index", C.c_int32), ("jitter_extra_scale", C.c_float), ("jitter_slow_motion", C.c_int32),
+        ("jitter_rotate_degrees", C.c_float), ("reset_history_on_change", C.c_int32),
+        ("num_sample_offsets", C.c_int32), ("sample_offsets", (C.c_float * 2) * TAA_SETTINGS_MAX_SAMPLE_OFFSETS),
+        ("postprocess_enabled", C.c_int32), ("zoom", C.c_uint32), ("show_zoom_box", C.c_uint32),
+        ("zoom_src_ltwh", C.c_int32 * 4), ("zoom_dst_ltwh", C.c_int32 * 4),
+    ]
+
+
 class SegmentationImages(C.Structure):
     """taa_segmentation_images_t: the segmentation mask's extra images (taa.comp:34-37)."""
     _fields_ = [
@@ -198,6 +214,10 @@ PROTOTYPES = {
     "taa_resolve_frame": (I32, [P(Uniforms), P(FrameImages), P(CasConsts), I32, VP]),
     "taa_resolve_frame_segmented": (I32, [P(Uniforms), P(FrameImages), P(SegmentationImages), P(CasConsts), I32, VP]),
     "taa_cas_apply": (I32, [VP, VP, I32, I32, I32, I32, P(CasConsts), VP]),
+    "taa_settings_defaults": (I32, [P(Settings)]),
+    "taa_settings_read_ini": (I32, [C.c_char_p, C.c_size_t, P(Settings)]),
+    "taa_settings_write_ini": (I32, [P(Settings), C.c_char_p, C.c_size_t, P(C.c_size_t)]),
+    "taa_settings_apply": (I32, [P(Settings), P(Uniforms), P(JitterConfig)]),
     "taa_edge_aa_apply": (I32, [VP, VP, VP, I32, I32, I32, I32, I32, P(EdgeAaConsts), VP]),
     "taa_edge_aa_defaults": (I32, [I32, I32, P(EdgeAaConsts)]),
     "taa_velocity_from_depth": (I32, [P(Reproject), VP, VP, VP, I32, I32, I32, I32, VP]),
