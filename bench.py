@@ -89,9 +89,20 @@ def cpu_baseline(w, frames_host, seconds):
         el = time.perf_counter() - t0
         if n >= 2 and el >= seconds:
             break
-    return dict(value=round(n * W * H / el / 1e6, 3), unit="Mpix/s", cores=threads, kind="port",
+    value = n * W * H / el / 1e6
+    # one thread: a band of rows of the same frame (resolve only), about a second of work
+    rows = max(8, min(H, int(round(value * 1e6 / threads / W))))
+    col_s, dep, vel_s, jit = frames_host[0]
+    U = uniforms_for(w, 1, jit)
+    t1 = time.perf_counter()
+    _oracle.resolve(U, decode(col_s[:rows]), dep[:rows], velocity_as_read(vel_s[:rows]), hist[:rows], dep[:rows],
+                    threads=1, debug=False, inspect=False)
+    one = rows * W / (time.perf_counter() - t1) / 1e6
+    return dict(value=round(value, 3), unit="Mpix/s", cores=threads, kind="port",
+                value_1_thread=round(one, 3),
                 sample=f"{n} full {W}x{H} frames of the same workload (resolve + CAS, FP32), {el:.1f} s wall, "
-                       f"OpenMP over rows on {threads} threads of {cpu_model()}")
+                       f"OpenMP over rows on {threads} threads of {cpu_model()}; value_1_thread: the resolve of "
+                       f"{rows} rows of one frame on one thread")
 
 
 def time_resolve(name, arith_name, steps, warmup, frames, autotune=False):
@@ -441,6 +452,27 @@ def extras(a):
     return x
 
 
+def measured_copy_gbs(mib=1024, reps=20):
+    """Device-to-device copy rate (bytes read + written per second) of a 1 GiB buffer: the practical
+    HBM ceiling on this device, reported beside the 8 TB/s peak."""
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * n * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(gbs, 1)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -490,6 +522,7 @@ def main():
             "bytes_per_pixel": r["bytes_per_pixel"],
         },
     }
+    out["roofline"]["measured_copy_gbs"] = measured_copy_gbs()
     if a.copy_calib > 0:
         n = a.copy_calib * (1 << 20)
         src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
