@@ -326,16 +326,27 @@ int32_t taa_ctx_usage(const taa_ctx_t* ctx, uint64_t* used_bytes, uint64_t* budg
 /* Device bytes a session of this shape keeps resident (history ping-pong, depth ring,
  * colour and velocity input slots). */
 uint64_t taa_session_bytes(int32_t width, int32_t height, int32_t color_format, int32_t velocity_format);
+/* ... of a session whose inputs are rendered at input_width x input_height (TAAU). */
+uint64_t taa_session_bytes_scaled(int32_t width, int32_t height, int32_t input_width, int32_t input_height,
+                                  int32_t color_format, int32_t velocity_format);
 
 /* A session is one stream of frames: it owns what taa<CF> owns per frame in flight
  * (source/taa.hpp:1364-1369) -- the history ping-pong -- plus resident input slots and the
  * previous depth. TAA_ERR_BUDGET when the context's HBM budget cannot hold it. */
 int32_t taa_session_open(taa_ctx_t* ctx, int32_t width, int32_t height, int32_t color_format,
                          int32_t velocity_format, taa_session_t** out);
+/* A session whose frames are rendered at input_width x input_height and resolved into a
+ * width x height history and result (TAAU: taa.hpp:292 mInputResolution / mOutputResolution):
+ * the colour, depth and velocity slots have the input size; every resolve takes the upsampling
+ * path of taa_resolve_frame. */
+int32_t taa_session_open_scaled(taa_ctx_t* ctx, int32_t width, int32_t height, int32_t input_width,
+                                int32_t input_height, int32_t color_format, int32_t velocity_format,
+                                taa_session_t** out);
 int32_t taa_session_close(taa_session_t* s);
 
 /* Device pointers of the slots the next frame's inputs are written into, and of the history
- * the last resolve produced. */
+ * the last resolve produced. width / height / pitch describe the input slots (the input size of
+ * a scaled session); the history has the session's output size. */
 typedef struct taa_session_io {
     void*       color;     /* write: next frame colour (color_format)        */
     float*      depth;     /* write: next frame depth  (D32F)                */
@@ -349,7 +360,7 @@ int32_t taa_session_io(taa_session_t* s, taa_session_io_t* out);
 /* Resolve the frame in the session's input slots. The first frame of a session, and the first
  * after taa_session_reset, runs with mResetHistory forced on (the reference instead skips TAA
  * on its very first frame, source/taa.hpp:989-990). */
-/* Sessions and batches resolve at one size: mUpsampling there is TAA_ERR_UNSUPPORTED. */
+/* A scaled session, or mUpsampling, takes the upsampling path (as taa_resolve_frame does). */
 int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* uniforms,
                             const taa_cas_consts_t* cas, void* result_screen, int32_t arith,
                             void* stream);
@@ -364,7 +375,9 @@ typedef struct taa_batch_item {
 } taa_batch_item_t;
 
 /* Resolve frames of many sessions (any mix of resolutions, one colour format, one arith) in
- * ONE kernel launch over the tiles of all frames. Sessions must be distinct. Stream-ordered. */
+ * ONE kernel launch over the tiles of all frames. Sessions must be distinct. Stream-ordered.
+ * Upsampling frames (scaled sessions, mUpsampling) are resolved by their own launches on the same
+ * stream after the batched one; a batch plan refuses them (TAA_ERR_UNSUPPORTED). */
 int32_t taa_batch_submit(taa_ctx_t* ctx, int32_t n, const taa_batch_item_t* items,
                          int32_t arith, void* stream);
 

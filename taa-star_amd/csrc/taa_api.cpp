@@ -251,6 +251,7 @@ struct taa_ctx {
 struct taa_session {
     taa_ctx* ctx = nullptr;
     int W = 0, H = 0, pitch = 0, cf = 0, vf = 0;
+    int Wl = 0, Hl = 0;  // input (render) size of the colour, depth and velocity slots
     uint64_t bytes = 0;
     void* base = nullptr;
     void* hist[2] = {nullptr, nullptr};
@@ -286,10 +287,15 @@ void session_desc(taa_session* s, void* screen, const taa_cas_consts_t* cas, taa
     fd.x_end = s->W;
     fd.cas_on = cas ? 1 : 0;
     fd.cas_peak_bits = cas ? cas->const1[0] : 0u;
-    fd.Wl = s->W;
-    fd.Hl = s->H;
-    fd.pitch_l = s->pitch;
-    fd.up = 0;
+    fd.Wl = s->Wl;
+    fd.Hl = s->Hl;
+    fd.pitch_l = s->Wl;
+    fd.up = (s->Wl != s->W || s->Hl != s->H) ? 1 : 0;
+}
+
+// upsampling frames are not batched: they run their own launch (exact FP32 upsampling kernel)
+bool session_upsamples(const taa_session* s, const taa_uniforms_t* u) {
+    return s->Wl != s->W || s->Hl != s->H || u->mUpsampling;
 }
 
 void session_advance(taa_session* s) {
@@ -464,19 +470,29 @@ int32_t taa_ctx_usage(const taa_ctx_t* c, uint64_t* used, uint64_t* budget) {
     return ok();
 }
 
-uint64_t taa_session_bytes(int32_t w, int32_t h, int32_t cf, int32_t vf) {
-    if (w <= 0 || h <= 0 || !valid_color_fmt(cf) || !valid_vel_fmt(vf)) return 0;
-    const size_t px = (size_t)w * (size_t)h;
-    return 3 * align256(px * color_bpp(cf)) + 2 * align256(px * 4) + align256(px * vel_bpp(vf));
+uint64_t taa_session_bytes_scaled(int32_t w, int32_t h, int32_t wl, int32_t hl, int32_t cf, int32_t vf) {
+    if (w <= 0 || h <= 0 || wl <= 0 || hl <= 0 || !valid_color_fmt(cf) || !valid_vel_fmt(vf)) return 0;
+    const size_t px = (size_t)w * (size_t)h, pl = (size_t)wl * (size_t)hl;
+    return 2 * align256(px * color_bpp(cf)) + align256(pl * color_bpp(cf)) + 2 * align256(pl * 4) +
+           align256(pl * vel_bpp(vf));
 }
 
+uint64_t taa_session_bytes(int32_t w, int32_t h, int32_t cf, int32_t vf) { return taa_session_bytes_scaled(w, h, w, h, cf, vf); }
+
 int32_t taa_session_open(taa_ctx_t* c, int32_t w, int32_t h, int32_t cf, int32_t vf, taa_session_t** out) {
+    return taa_session_open_scaled(c, w, h, w, h, cf, vf, out);
+}
+
+int32_t taa_session_open_scaled(taa_ctx_t* c, int32_t w, int32_t h, int32_t wl, int32_t hl, int32_t cf, int32_t vf,
+                                taa_session_t** out) {
     if (!c || !out) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
     *out = nullptr;
     if (w <= 0 || h <= 0 || w > 32768 || h > 32768) return fail(TAA_ERR_INVALID_ARG, "bad size %dx%d", w, h);
+    if (wl <= 0 || hl <= 0 || wl > 32768 || hl > 32768) return fail(TAA_ERR_INVALID_ARG, "bad input size %dx%d", wl, hl);
     if (!valid_color_fmt(cf) || !valid_vel_fmt(vf)) return fail(TAA_ERR_INVALID_ARG, "bad format");
-    if (!fits_offsets(w, h, 16)) return fail(TAA_ERR_INVALID_ARG, "session of %dx%d: an image would exceed 4 GiB", w, h);
-    const uint64_t bytes = taa_session_bytes(w, h, cf, vf);
+    if (!fits_offsets(w, h, 16) || !fits_offsets(wl, hl, 16))
+        return fail(TAA_ERR_INVALID_ARG, "session of %dx%d: an image would exceed 4 GiB", w, h);
+    const uint64_t bytes = taa_session_bytes_scaled(w, h, wl, hl, cf, vf);
     // admission: reserve the bytes against the budget first
     uint64_t cur = c->used.load();
     do {
@@ -500,16 +516,17 @@ int32_t taa_session_open(taa_ctx_t* c, int32_t w, int32_t h, int32_t cf, int32_t
         delete s;
         return hip_fail(e, "hipMemset");
     }
-    const size_t px = (size_t)w * (size_t)h;
+    const size_t px = (size_t)w * (size_t)h, pl = (size_t)wl * (size_t)hl;
     char* p = static_cast<char*>(s->base);
     s->hist[0] = p; p += align256(px * color_bpp(cf));
     s->hist[1] = p; p += align256(px * color_bpp(cf));
-    s->color = p; p += align256(px * color_bpp(cf));
-    s->depth[0] = reinterpret_cast<float*>(p); p += align256(px * 4);
-    s->depth[1] = reinterpret_cast<float*>(p); p += align256(px * 4);
+    s->color = p; p += align256(pl * color_bpp(cf));
+    s->depth[0] = reinterpret_cast<float*>(p); p += align256(pl * 4);
+    s->depth[1] = reinterpret_cast<float*>(p); p += align256(pl * 4);
     s->vel = p;
     s->ctx = c;
     s->W = w; s->H = h; s->pitch = w; s->cf = cf; s->vf = vf;
+    s->Wl = wl; s->Hl = hl;
     s->bytes = bytes;
     *out = s;
     return ok();
@@ -532,7 +549,7 @@ int32_t taa_session_io(taa_session_t* s, taa_session_io_t* out) {
     out->depth = s->depth[k];
     out->velocity = s->vel;
     out->history = s->hist[k ^ 1];
-    out->width = s->W; out->height = s->H; out->pitch = s->pitch;
+    out->width = s->Wl; out->height = s->Hl; out->pitch = s->Wl;  // the input slots
     out->color_format = s->cf; out->velocity_format = s->vf;
     out->frames_resolved = s->frames;
     return ok();
@@ -547,7 +564,6 @@ int32_t taa_session_reset(taa_session_t* s) {
 int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* u, const taa_cas_consts_t* cas, void* screen,
                             int32_t arith, void* stream) {
     if (!s || !u || !screen) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
-    if (u->mUpsampling) return fail(TAA_ERR_UNSUPPORTED, "mUpsampling in a session: use taa_resolve_frame with an input size");
     taa_uniforms_t U = *u;
     if (s->force_reset) U.mResetHistory = 1;
     const int k = (int)(s->frames & 1);
@@ -561,6 +577,7 @@ int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* u, const taa
     img.velocity = s->vel;
     img.result_history = s->hist[k];
     img.width = s->W; img.height = s->H; img.pitch = s->pitch;
+    img.input_width = s->Wl; img.input_height = s->Hl; img.input_pitch = s->Wl;
     img.color_format = s->cf; img.velocity_format = s->vf;
     int rc = resolve_impl(&U, &img, cas, arith, stream);
     if (rc) return rc;
@@ -589,7 +606,8 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
             if (items[j].session == it.session) return fail(TAA_ERR_INVALID_ARG, "item %d: session repeated", i);
         int rc = check_uniforms(it.uniforms);
         if (rc) return rc;
-        if (it.uniforms->mUpsampling) return fail(TAA_ERR_UNSUPPORTED, "item %d: mUpsampling in a batch", i);
+        if (session_upsamples(it.session, it.uniforms))
+            return fail(TAA_ERR_UNSUPPORTED, "item %d: an upsampling frame in a batch plan", i);
         if (it.uniforms->splitScreen) return fail(TAA_ERR_UNSUPPORTED, "item %d: split screen in a batch", i);
         const int px = it.session->W * it.session->H;
         if (px > best_px) {
@@ -620,6 +638,34 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
 int32_t taa_batch_submit(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items, int32_t arith, void* stream) {
     if (!c || !items) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
     if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith");
+    // upsampling frames (scaled sessions, mUpsampling) take their own launches after the batch
+    int n_up = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!items[i].session || !items[i].uniforms || !items[i].result_screen)
+            return fail(TAA_ERR_INVALID_ARG, "item %d: NULL field", i);
+        if (session_upsamples(items[i].session, items[i].uniforms)) ++n_up;
+    }
+    if (n_up) {
+        std::vector<taa_batch_item_t> rest;
+        rest.reserve(n - n_up);
+        for (int i = 0; i < n; ++i)
+            if (!session_upsamples(items[i].session, items[i].uniforms)) rest.push_back(items[i]);
+            else if (items[i].session->ctx != c) return fail(TAA_ERR_INVALID_ARG, "item %d: session of another context", i);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j)
+                if (items[j].session == items[i].session) return fail(TAA_ERR_INVALID_ARG, "item %d: session repeated", i);
+        if (!rest.empty()) {
+            int rc = taa_batch_submit(c, (int32_t)rest.size(), rest.data(), arith, stream);
+            if (rc) return rc;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (!session_upsamples(items[i].session, items[i].uniforms)) continue;
+            int rc = taa_session_resolve(items[i].session, items[i].uniforms, items[i].cas, items[i].result_screen, arith,
+                                         stream);
+            if (rc) return rc;
+        }
+        return ok();
+    }
     std::lock_guard<std::mutex> lk(c->mu);
     taa_ctx::Slot& slot = c->slots[c->next_slot];
     if (slot.used) {

@@ -226,6 +226,8 @@ PROTOTYPES = {
     "taa_ctx_usage": (I32, [VP, P(U64), P(U64)]),
     "taa_session_bytes": (U64, [I32, I32, I32, I32]),
     "taa_session_open": (I32, [VP, I32, I32, I32, I32, P(VP)]),
+    "taa_session_open_scaled": (I32, [VP, I32, I32, I32, I32, I32, I32, P(VP)]),
+    "taa_session_bytes_scaled": (U64, [I32, I32, I32, I32, I32, I32]),
     "taa_session_close": (I32, [VP]),
     "taa_session_io": (I32, [VP, P(SessionIO)]),
     "taa_session_resolve": (I32, [VP, P(Uniforms), P(CasConsts), VP, I32, VP]),

@@ -135,3 +135,96 @@ def test_taau_fp16_request_takes_the_exact_path(oracle):
     up = Upsampler(w, (96, 54), (192, 108), arith=ARITH_FP16)
     for _ in range(3):
         assert_parity(up.step(oracle))
+
+
+def test_scaled_session_equals_resolve_frame(oracle):
+    """A session opened at render size 96x54 -> output 192x108 (taa_session_open_scaled) gives the
+    bits of taa_resolve_frame with the same input size, frame after frame; its input slots have the
+    render size."""
+    from taastar import SessionIO
+    print("case taau scaled session", flush=True)
+    w = workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F)
+    up = Upsampler(w, (96, 54), (192, 108))
+    ctx, s = C.c_void_p(), C.c_void_p()
+    check(lib().taa_ctx_create(0, 0, C.byref(ctx)))
+    check(lib().taa_session_open_scaled(ctx, 192, 108, 96, 54, w["fmt"], w["vel"], C.byref(s)))
+    scr = torch.zeros(108, 192, 4, dtype=torch.float16, device="cuda")
+    for f in range(4):
+        ref = up.step(oracle)  # resolve_frame on the GPU (and the oracle, already checked)
+        io = SessionIO()
+        check(lib().taa_session_io(s, C.byref(io)))
+        assert (io.width, io.height, io.pitch) == (96, 54, 96)
+        hmemcpy = lib().hipMemcpy
+        hmemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        torch.cuda.synchronize()
+        for ptr, src in ((io.color, up.color), (io.depth, up.depth), (io.velocity, up.vel)):
+            nbytes = src.numel() * src.element_size()
+            assert hmemcpy(C.c_void_p(ptr), C.c_void_p(src.data_ptr()), nbytes, 3) == 0  # device to device
+        U = uniforms_for(w, f, up.scene.render(f)[3])
+        U.mUpsampling = 1
+        check(lib().taa_session_resolve(s, C.byref(U), C.byref(up.cas), C.c_void_p(scr.data_ptr()), ARITH_FP16, None))
+        torch.cuda.synchronize()
+        assert torch.equal(scr, up.screen), f"frame {f}: screen"
+    check(lib().taa_session_close(s))
+    check(lib().taa_ctx_destroy(ctx))
+
+
+def test_batch_with_scaled_sessions_equals_alone():
+    """A batch holding two one-size sessions and a scaled one: every result equals the session
+    resolved alone (the scaled frame takes its own launch after the batched one)."""
+    from taastar import BatchItem, SessionIO, Uniforms
+    print("case taau batch mix", flush=True)
+    w = workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F)
+    specs = [((160, 90), (160, 90)), ((120, 68), (240, 136)), ((96, 96), (96, 96))]  # (input, output)
+    scenes = [scene_for(w, width=i[0], height=i[1], seed=500 + k) for k, (i, _) in enumerate(specs)]
+    hmemcpy = lib().hipMemcpy
+    hmemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+
+    def upload(s, sc, f, Wl, Hl):
+        io = SessionIO()
+        check(lib().taa_session_io(s, C.byref(io)))
+        col, dep, vel4, jit = sc.render(f)
+        for ptr, arr in ((io.color, encode_color(col, w["fmt"])), (io.depth, dep), (io.velocity, encode_velocity(vel4, w["vel"]))):
+            a = np.ascontiguousarray(arr)
+            torch.cuda.synchronize()
+            assert hmemcpy(C.c_void_p(ptr), C.c_void_p(a.ctypes.data), a.nbytes, 1) == 0
+        return uniforms_for(w, f, jit)
+
+    def run(batched):
+        ctx = C.c_void_p()
+        check(lib().taa_ctx_create(0, 0, C.byref(ctx)))
+        sess, scrs, Ks = [], [], []
+        for (i, o) in specs:
+            s = C.c_void_p()
+            check(lib().taa_session_open_scaled(ctx, o[0], o[1], i[0], i[1], w["fmt"], w["vel"], C.byref(s)))
+            sess.append(s)
+            scrs.append(torch.zeros(o[1], o[0], 4, dtype=torch.float16, device="cuda"))
+            Ks.append(cas_for(w, o[0], o[1]))
+        outs = []
+        Us = [Uniforms() for _ in specs]
+        for f in range(4):
+            for k, (i, o) in enumerate(specs):
+                U = upload(sess[k], scenes[k], f, *i)
+                U.mUpsampling = int(i != o)
+                C.memmove(C.addressof(Us[k]), C.addressof(U), C.sizeof(Uniforms))
+            if batched:
+                items = (BatchItem * len(specs))()
+                for k in range(len(specs)):
+                    items[k].session, items[k].uniforms = sess[k], C.pointer(Us[k])
+                    items[k].cas, items[k].result_screen = C.pointer(Ks[k]), scrs[k].data_ptr()
+                check(lib().taa_batch_submit(ctx, len(specs), items, ARITH_FP16, None))
+            else:
+                for k in range(len(specs)):
+                    check(lib().taa_session_resolve(sess[k], C.byref(Us[k]), C.byref(Ks[k]),
+                                                    C.c_void_p(scrs[k].data_ptr()), ARITH_FP16, None))
+            torch.cuda.synchronize()
+            outs.append([t.cpu().numpy().copy() for t in scrs])
+        for s in sess:
+            check(lib().taa_session_close(s))
+        check(lib().taa_ctx_destroy(ctx))
+        return outs
+
+    a, b = run(False), run(True)
+    for f in range(4):
+        for k in range(len(specs)):
+            assert np.array_equal(a[f][k], b[f][k]), f"frame {f} session {k}"
