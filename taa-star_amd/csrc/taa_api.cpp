@@ -157,7 +157,7 @@ int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, in
     if (x_end <= x_begin) return TAA_OK;
     const bool cas = fd.cas_on != 0;
     if (sg) {  // segmentation mask: exact FP32 kernel, default tile shape
-        const taastar::TileShapeDef d = shape_def(cas ? taastar::kDefaultShapeCas : taastar::kDefaultShapePlain, cas);
+        const taastar::TileShapeDef d = shape_def(cas ? taastar::kSegShapeCas : taastar::kSegShapePlain, cas);
         fd.x_begin = x_begin;
         fd.x_end = x_end;
         fd.param_idx = param_idx;

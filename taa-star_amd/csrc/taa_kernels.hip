@@ -27,10 +27,11 @@ namespace {
 // ------------------------------------------------------------------------------------------
 // tile geometry and LDS
 
-template <int TW_, int TH_, int PAD_, bool CAS_, bool UP_ = false>
+template <int TW_, int TH_, int PAD_, bool CAS_, bool UP_ = false, bool SEG_ = false>
 struct Geo {
     static constexpr int TW = TW_, TH = TH_, PAD = PAD_;
     static constexpr bool CAS = CAS_, UP = UP_;  // UP: upsampling path, inputs read from global memory
+    static constexpr bool SEG = SEG_;            // SEG: the segmentation mask's inputs are staged too
     static constexpr int NT = 256;
     static constexpr int A = CAS ? 2 : 1;  // input apron
     static constexpr int SW = TW + 2 * A, SH = TH + 2 * A, SP = SW + PAD;
@@ -40,7 +41,19 @@ struct Geo {
         float c0[UP ? 1 : SH * SP], c1[UP ? 1 : SH * SP], c2[UP ? 1 : SH * SP], d[UP ? 1 : SH * SP];
         float r0[CAS ? RH * RP : 1], r1[CAS ? RH * RP : 1], r2[CAS ? RH * RP : 1];
         float ux[SW], uy[SH];  // uv of the tile's (clamped) columns and rows
+        // segmentation inputs of the staged region: unit normal, raw YCoCg luma, material id
+        float nx[SEG ? SH * SP : 1], ny[SEG ? SH * SP : 1], nz[SEG ? SH * SP : 1], lum[SEG ? SH * SP : 1];
+        uint32_t mat[SEG ? SH * SP : 1];
     };
+};
+
+// the staged segmentation planes (row pitch SP, same indexing as the colour planes)
+struct SegPlanes {
+    const float* nx;
+    const float* ny;
+    const float* nz;
+    const float* lum;
+    const uint32_t* mat;
 };
 
 struct PixelResult {
@@ -57,18 +70,16 @@ __device__ __forceinline__ float sobel_length(const float (&t)[8]) {
 }
 
 // calc_segmentation_value (taa.comp:589-702) at texel (x, y), one size. `total` receives the
-// weighted indicator sum when the evaluation gets that far (else 0). The depth Sobel reads the
-// staged depth plane (its apron is clamped to the edge like CLAMP_TO_TEX); the other inputs come
-// from global memory.
-template <bool S16, int SP>
-__device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, const taa_uniforms_t& U,
-                                       const taa_params_t& P, int x, int y, float hu, float hv,
-                                       const float* __restrict__ sd, int l, float& total) {
+// weighted indicator sum when the evaluation gets that far (else 0). Normals, luma, material ids
+// and depth of the 3x3 neighbourhood come from the staged planes (their apron is clamped to the
+// edge like CLAMP_TO_TEX); the previous frame's material id and mask from global memory.
+template <int SP>
+__device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, const SegPlanes& sp,
+                                       const taa_uniforms_t& U, const taa_params_t& P, int x, int y, float hu,
+                                       float hv, const float* __restrict__ sd, int l, float& total) {
     total = 0.0f;
     const uint32_t fl = P.mRayTraceAugmentFlags;
     const int W = fd.W, H = fd.H;
-    auto at_s = [&](int xx, int yy) { return (uint32_t)(clampi(yy, 0, H - 1) * sg.pitch + clampi(xx, 0, W - 1)); };
-    auto at_f = [&](int xx, int yy) { return (uint32_t)(clampi(yy, 0, H - 1) * fd.pitch + clampi(xx, 0, W - 1)); };
     if (fl & TAA_RTFLAG_FXD) {
         const int b = 100;
         if (x < b || y < b || x >= W - b || y >= H - b) return 1u;
@@ -77,7 +88,7 @@ __device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, c
     const bool useHistoryCount = (fl & TAA_RTFLAG_CNT) != 0u;
     const uint32_t newCountValue = useHistoryCount ? (uint32_t)P.mRayTraceHistoryCount << 16 : 0u;
     if ((fl & TAA_RTFLAG_OUT) && (hu < 0.0f || hv < 0.0f || hu >= 1.0f || hv >= 1.0f)) return 1u;
-    const uint32_t matId = gload<uint32_t>(sg.material, at_s(x, y) * 4u);
+    const uint32_t matId = sp.mat[l];
     if (fl & TAA_RTFLAG_DIS) {
         const int tx = to_int(hu * (float)W), ty = to_int(hv * (float)H);  // uv_to_tc(historyUv)
         if (tx >= 0 && ty >= 0 && tx < W && ty < H) {
@@ -88,15 +99,9 @@ __device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, c
     const int ox[8] = {-1, 0, 1, -1, 1, -1, 0, 1}, oy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
     float nrmValue = 0.0f, dptValue = 0.0f, matValue = 0.0f, lumValue = 0.0f;
     if (fl & TAA_RTFLAG_NRM) {
-        auto normal = [&](int xx, int yy) {  // sample_normal (taa.comp:574-579)
-            const float4 q = gload<float4>(sg.uv_normal, at_s(xx, yy) * 16u);
-            float sz, cz, sw, cw;
-            seg_sincos(q.z, sz, cz);
-            seg_sincos(q.w, sw, cw);
-            return f3{cz * cw, sz * cw, sw};
-        };
-        const f3 nC = normal(x, y);
-        auto dn = [&](int dx, int dy) { const f3 n = normal(x + dx, y + dy); return nC.x * n.x + nC.y * n.y + nC.z * n.z; };
+        auto normal = [&](int o) { return f3{sp.nx[l + o], sp.ny[l + o], sp.nz[l + o]}; };  // sample_normal
+        const f3 nC = normal(0);
+        auto dn = [&](int dx, int dy) { const f3 n = normal(dy * SP + dx); return nC.x * n.x + nC.y * n.y + nC.z * n.z; };
         const float dL = dn(-1, 0), dR = dn(1, 0), dT = dn(0, -1), dB = dn(0, 1);
         nrmValue = 1.0f - fmaxf(0.0f, fminf(fminf(fminf(dL, dR), dT), dB));
     }
@@ -107,20 +112,20 @@ __device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, c
         dptValue = sobel_length(t);
     }
     if (fl & TAA_RTFLAG_MID) {
-        auto mid = [&](int xx, int yy) { return gload<uint32_t>(sg.material, at_s(xx, yy) * 4u); };
-        if (matId != mid(x - 1, y) || matId != mid(x + 1, y) || matId != mid(x, y - 1) || matId != mid(x, y + 1)) matValue = 1.0f;
+        if (matId != sp.mat[l - 1] || matId != sp.mat[l + 1] || matId != sp.mat[l - SP] || matId != sp.mat[l + SP])
+            matValue = 1.0f;
     }
     if (fl & TAA_RTFLAG_LUM) {
         float t[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) t[k] = rgb_to_ycocg(xyz(ld_rgba<S16>(fd.color, at_f(x + ox[k], y + oy[k])))).x;
+        for (int k = 0; k < 8; ++k) t[k] = sp.lum[l + oy[k] * SP + ox[k]];
         lumValue = sobel_length(t);
     }
     total = nrmValue * P.mRayTraceAugment_WNrm + dptValue * P.mRayTraceAugment_WDpt + matValue * P.mRayTraceAugment_WMId +
             lumValue * P.mRayTraceAugment_WLum;
     if (total >= P.mRayTraceAugment_Thresh) return 2u | newCountValue;
     if (useHistoryCount) {
-        const uint32_t oldCnt = (gload<uint32_t>(sg.prev_segmask, at_s(x, y) * 4u) & 0xffff0000u) >> 16;
+        const uint32_t oldCnt = (gload<uint32_t>(sg.prev_segmask, (uint32_t)(y * sg.pitch + x) * 4u) & 0xffff0000u) >> 16;
         if (oldCnt > 0u) return 2u | ((oldCnt - 1u) << 16);
     }
     return 0u;
@@ -133,7 +138,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
                                          int x, int y, int l, const float* __restrict__ s0,
                                          const float* __restrict__ s1, const float* __restrict__ s2,
                                          const float* __restrict__ sd, float uvx, float uvy, bool write_aux,
-                                         const SegDesc* sg = nullptr) {
+                                         const SegDesc* sg = nullptr, const SegPlanes* spl = nullptr) {
     // uvx, uvy: tc_to_uv(x, y) = ((x, y) + 0.5) / size (taa.comp:131), tabulated per tile
     const int W = fd.W, H = fd.H;
     const uint32_t gi = (uint32_t)(y * fd.pitch + x);
@@ -315,7 +320,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     float segTotal = 0.0f;
     if constexpr (SEG) {
         if (P.mRayTraceAugment) {
-            segMaskValue = segmentation_value<S16, SP>(fd, *sg, U, P, x, y, hu, hv, sd, l, segTotal);
+            segMaskValue = segmentation_value<SP>(fd, *sg, *spl, U, P, x, y, hu, hv, sd, l, segTotal);
             if ((segMaskValue & 3u) != 0u) {
                 alpha = P.mRejectionAlpha;
                 rejected = true;
@@ -462,7 +467,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
 // ------------------------------------------------------------------------------------------
 // one tile
 
-template <class G, bool S16, bool SEG = false>
+template <class G, bool S16, bool SEG = G::SEG>
 __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm,
                                              const SegDesc* sg = nullptr) {
     const taa_params_t& P = U.param[fd.param_idx];
@@ -478,12 +483,25 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         const int sy = i / G::SW, sx = i - sy * G::SW;
         const int gx = clampi(gx0 + sx, 0, W - 1), gy = clampi(gy0 + sy, 0, H - 1);
         const uint32_t gi = (uint32_t)(gy * pitch + gx);
-        f3 t = stage_convert(P, xyz(ld_rgba<S16>(fd.color, gi)));
+        const f3 raw = xyz(ld_rgba<S16>(fd.color, gi));
+        f3 t = stage_convert(P, raw);
         const int li = sy * G::SP + sx;
         sm.c0[li] = t.x;
         sm.c1[li] = t.y;
         sm.c2[li] = t.z;
         sm.d[li] = ld_f32(fd.depth, gi);
+        if constexpr (G::SEG) {  // sample_normal, sample_luminance and the material id (taa.comp:570-579)
+            const uint32_t si = (uint32_t)(gy * sg->pitch + gx);
+            const float4 q = gload<float4>(sg->uv_normal, si * 16u);
+            float sz, cz, sw, cw;
+            seg_sincos(q.z, sz, cz);
+            seg_sincos(q.w, sw, cw);
+            sm.nx[li] = cz * cw;
+            sm.ny[li] = sz * cw;
+            sm.nz[li] = sw;
+            sm.lum[li] = rgb_to_ycocg(raw).x;
+            sm.mat[li] = gload<uint32_t>(sg->material, si * 4u);
+        }
     }
     if (tid < G::SW) sm.ux[tid] = ((float)clampi(gx0 + tid, 0, W - 1) + 0.5f) / (float)W;
     else if (tid < G::SW + G::SH) sm.uy[tid - G::SW] = ((float)clampi(gy0 + tid - G::SW, 0, H - 1) + 0.5f) / (float)H;
@@ -496,8 +514,9 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         const int cx = clampi(px, 0, W - 1), cy = clampi(py, 0, H - 1);
         const bool owned = rx >= G::RA && rx < G::RA + G::TW && ry >= G::RA && ry < G::RA + G::TH && px < fd.x_end && py < H;
         const int l = (cy - gy0) * G::SP + (cx - gx0);
+        const SegPlanes spl{sm.nx, sm.ny, sm.nz, sm.lum, sm.mat};
         PixelResult r = resolve_pixel_f32<S16, G::SP, G::UP, SEG>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
-                                                                  sm.ux[cx - gx0], sm.uy[cy - gy0], owned, sg);
+                                                                  sm.ux[cx - gx0], sm.uy[cy - gy0], owned, sg, &spl);
         if (owned) {
             const uint32_t gi = (uint32_t)(cy * pitch + cx);
             st_rgba<S16>(fd.history_out, gi, r.history);
@@ -548,7 +567,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3
 // the resolve with the segmentation mask of mRayTraceAugment
 template <int TW, int TH, int PAD, bool CAS, bool S16>
 __global__ __launch_bounds__(256) void k_resolve_seg(const ResolveArgs a, const SegDesc sg) {
-    using G = Geo<TW, TH, PAD, CAS>;
+    using G = Geo<TW, TH, PAD, CAS, false, true>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
     resolve_tile<G, S16, true>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm, &sg);
@@ -660,7 +679,7 @@ int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDe
     a.U = U;
     const int n = fd.tiles_x * fd.tiles_y;
     const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
-    constexpr TileShapeDef C = kShapesCas[kDefaultShapeCas], Pl = kShapesPlain[kDefaultShapePlain];
+    constexpr TileShapeDef C = kShapesCas[kSegShapeCas], Pl = kShapesPlain[kSegShapePlain];
     if (fd.cas_on) {
         if (s16) hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, true>), dim3(n), dim3(256), 0, st, a, sg);
         else hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, false>), dim3(n), dim3(256), 0, st, a, sg);
