@@ -232,21 +232,17 @@ int32_t taa_settings_read_ini(const char* text, size_t len, taa_settings_t* sett
         // mDebugSampleOffsets (taa.hpp:1315-1320): resize to max(1, n) (new entries 0), read n
         int32_t n = 0;
         const Field cnt{"mDebugSampleOffsets.size", Kind::Int, 0};
-        char nb[4];
-        std::memcpy(nb, &n, 4);
-        if (!apply(pr->second, cnt, nb, bad)) return taastar::set_last_error(TAA_ERR_INVALID_ARG, "TAA_Primary.mDebugSampleOffsets.size");
-        std::memcpy(&n, nb, 4);
+        if (!apply(pr->second, cnt, reinterpret_cast<char*>(&n), bad))
+            return taastar::set_last_error(TAA_ERR_INVALID_ARG, "TAA_Primary.mDebugSampleOffsets.size");
         const int32_t keep = n > 1 ? n : 1;
         if (keep > TAA_SETTINGS_MAX_SAMPLE_OFFSETS)
             return taastar::set_last_error(TAA_ERR_CAPACITY, "more than TAA_SETTINGS_MAX_SAMPLE_OFFSETS sample offsets");
         for (int32_t i = s.num_sample_offsets; i < keep; ++i) s.sample_offsets[i][0] = s.sample_offsets[i][1] = 0.0f;
         s.num_sample_offsets = keep;
         for (int32_t i = 0; i < n; ++i) {
-            const std::string k = "mDebugSampleOffsets_" + std::to_string(i);
-            const Field fx{nullptr, Kind::Float, offsetof(taa_settings_t, sample_offsets) + (size_t)i * 8};
-            const Field fy{nullptr, Kind::Float, offsetof(taa_settings_t, sample_offsets) + (size_t)i * 8 + 4};
-            const std::string kx = k + ".x", ky = k + ".y";
-            const Field gx{kx.c_str(), fx.kind, fx.offset}, gy{ky.c_str(), fy.kind, fy.offset};
+            const std::string k = "mDebugSampleOffsets_" + std::to_string(i), kx = k + ".x", ky = k + ".y";
+            const size_t at = offsetof(taa_settings_t, sample_offsets) + (size_t)i * 8;
+            const Field gx{kx.c_str(), Kind::Float, at}, gy{ky.c_str(), Kind::Float, at + 4};
             if (!apply(pr->second, gx, base, bad) || !apply(pr->second, gy, base, bad))
                 return taastar::set_last_error(TAA_ERR_INVALID_ARG, ("TAA_Primary." + bad).c_str());
         }
