@@ -149,12 +149,20 @@ def test_hostile_velocity_and_depth_fp32(oracle, mode):
 
 
 def test_hostile_velocity_fp16(oracle):
+    """The FP16 path on NaN / Inf / huge velocities and NaN depths: texel indices, closest taps and
+    rejection decisions as the transcription's."""
     print("case hostile fp16", flush=True)
     seq = Sequence(workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F), width=123, height=77, cas=True, arith=ARITH_FP16)
     rng = np.random.default_rng(6)
     render = seq.scene.render
-    seq.scene.render = lambda frame, threads=0: (lambda c, d, v4, j: (c, d, hostile_velocity(v4, rng), j))(
-        *render(frame, threads))
+
+    def hostile(frame, threads=0):
+        c, d, v4, jit = render(frame, threads)
+        d = d.copy()
+        d[rng.integers(0, d.shape[0], 20), rng.integers(0, d.shape[1], 20)] = np.nan
+        return c, d, hostile_velocity(v4, rng), jit
+
+    seq.scene.render = hostile
     for _ in range(4):
         assert_fp16_indices(seq.step(oracle))
 
