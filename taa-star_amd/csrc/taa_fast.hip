@@ -91,6 +91,9 @@ __device__ __forceinline__ W4 cr_weights(float f) {
     return r;
 }
 
+// four dwords at an 8-byte-aligned address (two RGBA16F texels): one global_load_dwordx4
+typedef uint32_t u32x4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+
 // one texel of the history as three fp32 channels (fp16 storage feeds mixed FMAs)
 template <bool S16>
 struct Texel {
@@ -122,9 +125,22 @@ __device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, in
             // byte offset of the row's first texel; the four texels follow as immediate offsets
             const uint32_t rb = ((uint32_t)(ky - 1 + j) * (uint32_t)pitch + (uint32_t)(kx - 1)) * TB;
             f3 row = {0.f, 0.f, 0.f};
+            Texel<S16> t4[4];
+            if constexpr (S16) {  // the row's four fp16 texels as two 16-byte loads (8-byte aligned)
+                const u32x4_a8 lo = gload<u32x4_a8>(hist, rb, 0), hi = gload<u32x4_a8>(hist, rb, 16);
+                const uint32_t w8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const h2 rg = __builtin_bit_cast(h2, w8[2 * i]), ba = __builtin_bit_cast(h2, w8[2 * i + 1]);
+                    t4[i] = {(float)rg.x, (float)rg.y, (float)ba.x};
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) t4[i] = load_texel<S16>(hist, rb, i);
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                Texel<S16> t = load_texel<S16>(hist, rb, i);
+                const Texel<S16> t = t4[i];
                 row.x = __builtin_fmaf(t.r, wx.w[i], row.x);
                 row.y = __builtin_fmaf(t.g, wx.w[i], row.y);
                 row.z = __builtin_fmaf(t.b, wx.w[i], row.z);
