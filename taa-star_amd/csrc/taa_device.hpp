@@ -47,6 +47,12 @@ __device__ __forceinline__ void gstore(void* base, uint32_t off, T v, int imm = 
     *(TAA_GLOBAL T*)((TAA_GLOBAL char*)base + off + imm) = v;
 }
 
+// four dwords at an 8-byte-aligned address (two RGBA16F texels): one global_load_dwordx4
+typedef uint32_t u32x4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+__device__ __forceinline__ float4 rgba16_bits(uint32_t lo, uint32_t hi) {
+    return make_float4(h2f(lo & 0xffffu), h2f(lo >> 16), h2f(hi & 0xffffu), h2f(hi >> 16));
+}
+
 // texel i of an RGBA image (element index, not bytes)
 template <bool S16>
 __device__ __forceinline__ float4 ld_rgba(const void* p, uint32_t i) {
@@ -287,16 +293,28 @@ __device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
         ys[i] = (interior ? ky - 1 + i : clampi(ky - 1 + i, 0, H - 1)) * fd.pitch;
     }
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto add = [&](int i, int j, float4 t) {
+        const float a = wx[i], b = wy[j];
+        const float4 s = make_float4(t.x * a * b, t.y * a * b, t.z * a * b, ALPHA ? t.w * a * b : 0.0f);
+        if (i == 0 && j == 0) acc = s;
+        else acc = make_float4(acc.x + s.x, acc.y + s.y, acc.z + s.z, ALPHA ? acc.w + s.w : 0.0f);
+    };
+    if (S16 && interior) {  // each row's four fp16 texels as two 16-byte loads
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t rb = (uint32_t)(ys[j] + xs[0]) * 8u;
+            const u32x4_a8 lo = gload<u32x4_a8>(fd.history, rb, 0), hi = gload<u32x4_a8>(fd.history, rb, 16);
+            add(0, j, rgba16_bits(lo.x, lo.y));
+            add(1, j, rgba16_bits(lo.z, lo.w));
+            add(2, j, rgba16_bits(hi.x, hi.y));
+            add(3, j, rgba16_bits(hi.z, hi.w));
+        }
+        return acc;
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float4 t = ld_rgba<S16>(fd.history, (uint32_t)(ys[j] + xs[i]));
-            float a = wx[i], b = wy[j];
-            float4 s = make_float4(t.x * a * b, t.y * a * b, t.z * a * b, ALPHA ? t.w * a * b : 0.0f);
-            if (i == 0 && j == 0) acc = s;
-            else acc = make_float4(acc.x + s.x, acc.y + s.y, acc.z + s.z, ALPHA ? acc.w + s.w : 0.0f);
-        }
+        for (int i = 0; i < 4; ++i) add(i, j, ld_rgba<S16>(fd.history, (uint32_t)(ys[j] + xs[i])));
     return acc;
 }
 

@@ -91,9 +91,6 @@ __device__ __forceinline__ W4 cr_weights(float f) {
     return r;
 }
 
-// four dwords at an 8-byte-aligned address (two RGBA16F texels): one global_load_dwordx4
-typedef uint32_t u32x4_a8 __attribute__((ext_vector_type(4), aligned(8)));
-
 // one texel of the history as three fp32 channels (fp16 storage feeds mixed FMAs)
 template <bool S16>
 struct Texel {
