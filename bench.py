@@ -25,8 +25,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each divided by what the same
 # counter reads for a device-to-device copy of known size), keyed by (workload, arith)
 TRAFFIC = {
-    # profiles/k_resolve_fast_3840x2160_fp16_cas.md: (124,156.0 KiB x 1.9997 + 129,775.5 KiB x 1.0) x 1024
-    ("c3", "fp16"): 387128672,
+    # profiles/k_resolve_fast_3840x2160_fp16_cas.md: (124,191.39 KiB x 1.99976 + 129,725.10 KiB x 1.0) x 1024
+    ("c3", "fp16"): 387150581,
 }
 
 
