@@ -116,6 +116,12 @@ def test_session_bytes():
     assert lib.taa_session_bytes(w, h, 0, 0) == 3 * a(px * 16) + 2 * a(px * 4) + a(px * 8)
     assert lib.taa_session_bytes(w, h, 1, 1) == 3 * a(px * 8) + 2 * a(px * 4) + a(px * 4)
     assert lib.taa_session_bytes(0, h, 1, 1) == 0 and lib.taa_session_bytes(w, h, 7, 1) == 0
+    # a scaled (TAAU) session: history ping-pong at the output size, input slots and depths at the input size
+    wl, hl = 1280, 720
+    pl = wl * hl
+    assert lib.taa_session_bytes_scaled(w, h, wl, hl, 1, 1) == 2 * a(px * 8) + a(pl * 8) + 2 * a(pl * 4) + a(pl * 4)
+    assert lib.taa_session_bytes_scaled(w, h, w, h, 0, 2) == lib.taa_session_bytes(w, h, 0, 2)
+    assert lib.taa_session_bytes_scaled(w, h, 0, hl, 1, 1) == 0
 
 
 def test_argument_errors_need_no_gpu():
