@@ -332,6 +332,16 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         const float2 v = vel_xy<S16>(sv[l + vo]);
         const float hu = sux[lx] - v.x, hv = suy[ly] - v.y;  // taa.comp:421
 
+        // previous depth under the history position (taa.comp:818), fetched ahead of the history
+        // gather so that the two latencies overlap
+        int dtx = 0, dty = 0;
+        float hd = 0.0f;
+        if (depth_cull) {
+            dtx = to_int(hu * fW);
+            dty = to_int(hv * fH);
+            hd = ld_f32(fd.hdepth, (uint32_t)clampi(dty, 0, H - 1) * pitch + (uint32_t)clampi(dtx, 0, W - 1));
+        }
+
         // ---- Catmull-Rom history (taa.comp:441-514)
         const float tX = hu * fW - 0.5f, tY = hv * fH - 0.5f;
         const float fbx = floorf(tX), fby = floorf(tY);
@@ -345,15 +355,12 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
 
         // ---- rejection (taa.comp:787-823), decisions as in the FP32 path
         bool rejected = false;
-        int flags = TAA_FLAG_USED_VELOCITY, dtx = 0, dty = 0;
+        int flags = TAA_FLAG_USED_VELOCITY;
         if (reject_out && (hu < 0.0f || hv < 0.0f || hu >= 1.0f || hv >= 1.0f)) {
             rejected = true;
             flags |= TAA_FLAG_OUTSIDE;
         }
         if (depth_cull) {
-            dtx = to_int(hu * fW);
-            dty = to_int(hv * fH);
-            const float hd = ld_f32(fd.hdepth, (uint32_t)clampi(dty, 0, H - 1) * pitch + (uint32_t)clampi(dtx, 0, W - 1));
             bool rej;
             if (lin_depth) {
                 const float n = U.mCamNearPlane, f = U.mCamFarPlane;
