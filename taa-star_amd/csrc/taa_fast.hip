@@ -288,6 +288,10 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
             if (rx >= G::RW) { rx -= G::RW; ry += 1; }
         }
         if (tid + it * G::NT >= G::RW * G::RH) break;
+        // raised wave priority from the start of a texel until its history loads are issued: the
+        // chain LDS -> closest depth -> velocity -> history address gets issue slots ahead of
+        // other waves' arithmetic (measured +1.5% at 4K)
+        __builtin_amdgcn_s_setprio(2);
         const int px = x0 - G::RA + rx, py = y0 - G::RA + ry;
         const int cx = clampi(px, 0, W - 1), cy = clampi(py, 0, H - 1);
         const bool owned = rx >= G::RA && rx < G::RA + G::TW && ry >= G::RA && ry < G::RA + G::TH && px < fd.x_end && py < H;
@@ -347,6 +351,7 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         const float fbx = floorf(tX), fby = floorf(tY);
         const int kx = to_int(fbx), ky = to_int(fby);
         const f3 hrgb = history_cr_fast<S16>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby);
+        __builtin_amdgcn_s_setprio(0);
         f3 h = hrgb;
         if (ycocg) {
             const float t = 0.25f * (hrgb.x + hrgb.z);
