@@ -330,6 +330,50 @@ def time_segmented_resolve(W, H, steps, warmup):
                 without_mask_avg_ms=round(a_plain, 5))
 
 
+def time_taau(lo, hi, steps, warmup):
+    """The upsampling resolve (mUpsampling, input size lo, output size hi), RGBA16F storage, the c3
+    feature set and fused CAS: exact FP32 arithmetic (the upsampling kernel)."""
+    import numpy as np
+    import torch
+    import taastar
+    from taastar.device import color_dtype, vel_shape_dtype
+    from taastar.workloads import cas_for, encode_color, encode_velocity, scene_for, uniforms_for, workload
+
+    w = workload("c3")
+    (Wl, Hl), (W, H) = lo, hi
+    col, dep, vel4, jit = scene_for(w, width=Wl, height=Hl).render(1)
+    cd = color_dtype(w["fmt"])
+    vc, vd = vel_shape_dtype(w["vel"])
+    dev = torch.device("cuda:0")
+    color = torch.from_numpy(encode_color(col, w["fmt"])).to(dev)
+    depth = torch.from_numpy(dep).to(dev)
+    vel = torch.from_numpy(np.ascontiguousarray(encode_velocity(vel4, w["vel"]))).to(dev)
+    prev_depth = depth.clone()
+    hist = [torch.zeros(H, W, 4, dtype=cd, device=dev) for _ in range(2)]
+    screen = torch.zeros(H, W, 4, dtype=cd, device=dev)
+    U = uniforms_for(w, 1, jit)
+    U.mUpsampling = 1
+    K = cas_for(w, W, H)
+    img = taastar.FrameImages()
+    img.color, img.depth, img.velocity, img.history_depth = (color.data_ptr(), depth.data_ptr(), vel.data_ptr(),
+                                                            prev_depth.data_ptr())
+    img.history, img.result_history, img.result_screen = hist[0].data_ptr(), hist[1].data_ptr(), screen.data_ptr()
+    img.width, img.height, img.pitch = W, H, W
+    img.input_width, img.input_height, img.input_pitch = Wl, Hl, Wl
+    img.color_format, img.velocity_format = w["fmt"], w["vel"]
+    L = taastar.lib()
+    avg, med = time_stream(lambda st: L.taa_resolve_frame(C.byref(U), C.byref(img), C.byref(K), taastar.ARITH_FP32,
+                                                          C.c_void_p(st.cuda_stream)), steps, warmup)
+    # inputs once at the input size (colour 8, depth 4, velocity 4, previous depth 4), history in
+    # and out and the screen at the output size (8 each)
+    nbytes = Wl * Hl * 20 + W * H * 24
+    del color, depth, vel, prev_depth, hist, screen
+    torch.cuda.empty_cache()
+    return dict(input=f"{Wl}x{Hl}", output=f"{W}x{H}", steps=steps, avg_ms=round(avg, 5), median_ms=round(med, 5),
+                gpix_per_s=round(W * H / (avg * 1e-3) / 1e9, 3), bytes_per_launch=nbytes,
+                achieved_gbs=round(nbytes / (avg * 1e-3) / 1e9, 1), hbm_frac=round(nbytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+
+
 def tune(W, H, fmt, arith, cas, iters=20):
     from taastar import check, lib
     ctx = C.c_void_p()
@@ -404,6 +448,11 @@ def extras(a):
     x["resolve_segmented_1920x1080_fp32"] = r
     print(f"# segmented resolve 1080p: {r['avg_ms']:.4f} ms (without mask {r['without_mask_avg_ms']:.4f} ms)",
           file=sys.stderr, flush=True)
+    for lo in ((1920, 1080), (2560, 1440)):
+        r = time_taau(lo, (3840, 2160), 100, 10)
+        x[f"taau_{r['input']}_to_{r['output']}_rgba16f_fp32arith"] = r
+        print(f"# TAAU {r['input']} -> {r['output']}: {r['gpix_per_s']:.2f} Gpix/s ({r['avg_ms']:.4f} ms)",
+              file=sys.stderr, flush=True)
     for cls in RESOLUTIONS:  # autotune every class the mixed loads use (FP16 and FP32 storage)
         W, H = RESOLUTIONS[cls]
         tune(W, H, F16, A16, True)

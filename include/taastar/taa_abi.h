@@ -133,7 +133,8 @@ typedef struct taa_cas_consts {
 
 /* The images one resolve reads and writes: the descriptor set of taa.comp
  * (shaders/taa.comp:24-38, bound at source/taa.hpp:1009-1025). Inputs are borrowed; the
- * caller owns every buffer. history and result_history must not alias. */
+ * caller owns every buffer. No output (result_screen, result_history, debug, inspect) may be the
+ * same buffer as an input or as another output: TAA_ERR_INVALID_ARG. */
 typedef struct taa_frame_images {
     const void*  color;          /* set0 b1 uCurrentFrame    (color_format)            */
     const float* depth;          /* set0 b2 uCurrentDepth    D32F                       */
@@ -366,6 +367,8 @@ int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* uniforms,
                             void* stream);
 int32_t taa_session_reset(taa_session_t* s);
 
+#define TAA_BATCH_MAX_ITEMS 1024
+
 /* One frame of one session in a batch. */
 typedef struct taa_batch_item {
     taa_session_t*          session;
@@ -375,7 +378,10 @@ typedef struct taa_batch_item {
 } taa_batch_item_t;
 
 /* Resolve frames of many sessions (any mix of resolutions, one colour format, one arith) in
- * ONE kernel launch over the tiles of all frames. Sessions must be distinct. Stream-ordered.
+ * ONE kernel launch over the tiles of all frames. At most TAA_BATCH_MAX_ITEMS items
+ * (TAA_ERR_CAPACITY beyond). Sessions must be distinct, and so must the
+ * result_screen pointers; no result_screen may lie inside a batched session's buffers
+ * (TAA_ERR_INVALID_ARG). Stream-ordered.
  * Upsampling frames (scaled sessions, mUpsampling) are resolved by their own launches on the same
  * stream after the batched one; a batch plan refuses them (TAA_ERR_UNSUPPORTED). */
 int32_t taa_batch_submit(taa_ctx_t* ctx, int32_t n, const taa_batch_item_t* items,

@@ -109,10 +109,19 @@ int make_desc(const taa_frame_images_t* img, const taa_cas_consts_t* cas, taasta
     if (!img->color || !img->depth || !img->history || !img->history_depth || !img->result_screen || !img->velocity ||
         !img->result_history)
         return fail(TAA_ERR_INVALID_ARG, "a required image pointer is NULL");
+    // every workgroup reads its apron and its history footprint from other tiles' texels while
+    // other workgroups write theirs, so no output may share a buffer with an input or another output
     if (img->history == img->result_history) return fail(TAA_ERR_INVALID_ARG, "history and result_history alias");
-    if (img->result_screen == img->result_history) return fail(TAA_ERR_INVALID_ARG, "result_screen and result_history alias");
-    if (cas && (img->result_screen == img->color || img->result_screen == img->history))
-        return fail(TAA_ERR_INVALID_ARG, "result_screen aliases an input");
+    const void* const ins[] = {img->color, img->depth, img->history, img->history_depth, img->velocity};
+    const void* const outs[] = {img->result_screen, img->result_history, img->debug, img->inspect};
+    const char* const out_names[] = {"result_screen", "result_history", "debug", "inspect"};
+    for (int o = 0; o < 4; ++o) {
+        if (!outs[o]) continue;
+        for (const void* in : ins)
+            if (outs[o] == in) return fail(TAA_ERR_INVALID_ARG, "%s aliases an input image", out_names[o]);
+        for (int p = o + 1; p < 4; ++p)
+            if (outs[o] == outs[p]) return fail(TAA_ERR_INVALID_ARG, "%s and %s alias", out_names[o], out_names[p]);
+    }
     std::memset(&fd, 0, sizeof(fd));
     fd.color = img->color;
     fd.depth = img->depth;
@@ -237,7 +246,7 @@ struct taa_ctx {
     std::mutex mu;
     // batch staging ring: pinned host mirror + device copy of entries and uniforms
     static constexpr int kSlots = 16;
-    static constexpr int kCap = 1024;
+    static constexpr int kCap = TAA_BATCH_MAX_ITEMS;
     struct Slot {
         void* host = nullptr;
         void* dev = nullptr;
@@ -587,23 +596,57 @@ int32_t taa_session_resolve(taa_session_t* s, const taa_uniforms_t* u, const taa
 
 // ---- batches ------------------------------------------------------------------------------
 
+// The items' buffers: no NULL field, sessions of this context and distinct, screens distinct and
+// outside every session block of the call (a screen there would be written while that session's
+// images are read).
+static int check_item_buffers(taa_ctx_t* c, int n, const taa_batch_item_t* items) {
+    for (int i = 0; i < n; ++i)
+        if (!items[i].session || !items[i].uniforms || !items[i].result_screen)
+            return fail(TAA_ERR_INVALID_ARG, "item %d: NULL field", i);
+    for (int i = 0; i < n; ++i) {
+        const taa_batch_item_t& it = items[i];
+        if (it.session->ctx != c) return fail(TAA_ERR_INVALID_ARG, "item %d: session of another context", i);
+        for (int j = 0; j < i; ++j) {
+            if (items[j].session == it.session) return fail(TAA_ERR_INVALID_ARG, "item %d: session repeated", i);
+            if (items[j].result_screen == it.result_screen)
+                return fail(TAA_ERR_INVALID_ARG, "items %d and %d: result_screen alias", j, i);
+        }
+        const uintptr_t p = reinterpret_cast<uintptr_t>(it.result_screen);
+        for (int j = 0; j < n; ++j) {
+            const uintptr_t b = reinterpret_cast<uintptr_t>(items[j].session->base);
+            if (p >= b && p < b + items[j].session->bytes)
+                return fail(TAA_ERR_INVALID_ARG, "item %d: result_screen lies inside the buffers of item %d's session", i, j);
+        }
+    }
+    return TAA_OK;
+}
+
+// What the items of one batched launch must satisfy whatever their uniforms hold: checked by
+// every submit and once when a plan is created.
+static int check_items(taa_ctx_t* c, int n, const taa_batch_item_t* items) {
+    if (n <= 0 || n > taa_ctx::kCap) return fail(TAA_ERR_CAPACITY, "batch of %d items (capacity %d)", n, taa_ctx::kCap);
+    if (int rc = check_item_buffers(c, n, items)) return rc;
+    const bool cas = items[0].cas != nullptr;
+    const bool s16 = items[0].session->cf == TAA_FMT_RGBA16F;
+    for (int i = 0; i < n; ++i) {
+        if ((items[i].cas != nullptr) != cas) return fail(TAA_ERR_INVALID_ARG, "item %d: CAS on/off must match the batch", i);
+        if ((items[i].session->cf == TAA_FMT_RGBA16F) != s16)
+            return fail(TAA_ERR_INVALID_ARG, "item %d: colour format must match", i);
+    }
+    return TAA_OK;
+}
+
 static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int arith, void* host, void* dev, int& shape,
                        bool& cas, bool& s16, int& total_tiles, bool& fast) {
-    if (n <= 0 || n > taa_ctx::kCap) return fail(TAA_ERR_CAPACITY, "batch of %d items (capacity %d)", n, taa_ctx::kCap);
     (void)dev;
+    if (int rc = check_items(c, n, items)) return rc;
     auto* ents = static_cast<taastar::BatchEntry*>(host);
     cas = items[0].cas != nullptr;
-    s16 = items[0].session && items[0].session->cf == TAA_FMT_RGBA16F;
+    s16 = items[0].session->cf == TAA_FMT_RGBA16F;
     // one tile shape per launch: the one tuned for the largest frame class in the batch
     int best_px = -1;
     for (int i = 0; i < n; ++i) {
         const taa_batch_item_t& it = items[i];
-        if (!it.session || !it.uniforms || !it.result_screen) return fail(TAA_ERR_INVALID_ARG, "item %d: NULL field", i);
-        if (it.session->ctx != c) return fail(TAA_ERR_INVALID_ARG, "item %d: session of another context", i);
-        if ((it.cas != nullptr) != cas) return fail(TAA_ERR_INVALID_ARG, "item %d: CAS on/off must match the batch", i);
-        if ((it.session->cf == TAA_FMT_RGBA16F) != s16) return fail(TAA_ERR_INVALID_ARG, "item %d: colour format must match", i);
-        for (int j = 0; j < i; ++j)
-            if (items[j].session == it.session) return fail(TAA_ERR_INVALID_ARG, "item %d: session repeated", i);
         int rc = check_uniforms(it.uniforms);
         if (rc) return rc;
         if (session_upsamples(it.session, it.uniforms))
@@ -638,22 +681,17 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
 int32_t taa_batch_submit(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items, int32_t arith, void* stream) {
     if (!c || !items) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
     if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith");
+    if (n <= 0 || n > taa_ctx::kCap) return fail(TAA_ERR_CAPACITY, "batch of %d items (capacity %d)", n, taa_ctx::kCap);
+    if (int rc = check_item_buffers(c, n, items)) return rc;
     // upsampling frames (scaled sessions, mUpsampling) take their own launches after the batch
     int n_up = 0;
-    for (int i = 0; i < n; ++i) {
-        if (!items[i].session || !items[i].uniforms || !items[i].result_screen)
-            return fail(TAA_ERR_INVALID_ARG, "item %d: NULL field", i);
+    for (int i = 0; i < n; ++i)
         if (session_upsamples(items[i].session, items[i].uniforms)) ++n_up;
-    }
     if (n_up) {
         std::vector<taa_batch_item_t> rest;
         rest.reserve(n - n_up);
         for (int i = 0; i < n; ++i)
             if (!session_upsamples(items[i].session, items[i].uniforms)) rest.push_back(items[i]);
-            else if (items[i].session->ctx != c) return fail(TAA_ERR_INVALID_ARG, "item %d: session of another context", i);
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < i; ++j)
-                if (items[j].session == items[i].session) return fail(TAA_ERR_INVALID_ARG, "item %d: session repeated", i);
         if (!rest.empty()) {
             int rc = taa_batch_submit(c, (int32_t)rest.size(), rest.data(), arith, stream);
             if (rc) return rc;
@@ -755,10 +793,7 @@ int32_t taa_batch_plan_create(taa_ctx_t* c, int32_t n, const taa_batch_item_t* i
     if (!c || !items || !out) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
     *out = nullptr;
     if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith");
-    if (n <= 0 || n > taa_ctx::kCap) return fail(TAA_ERR_CAPACITY, "batch of %d items (capacity %d)", n, taa_ctx::kCap);
-    for (int i = 0; i < n; ++i)
-        if (!items[i].session || !items[i].uniforms || !items[i].result_screen)
-            return fail(TAA_ERR_INVALID_ARG, "item %d: NULL field", i);
+    if (int rc = check_items(c, n, items)) return rc;
     taa_batch_plan* p = new (std::nothrow) taa_batch_plan();
     if (!p) return fail(TAA_ERR_OUT_OF_MEMORY, "host allocation");
     p->ctx = c;

@@ -154,6 +154,19 @@ def test_argument_errors_need_no_gpu():
     assert lib.taa_resolve_frame(C.byref(U), C.byref(big), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
     assert b"reserved" in lib.taa_last_error()
     assert lib.taa_session_close(None) == _abi.TAA_ERR_INVALID_ARG
+    # outputs may share a buffer with neither an input nor another output (other workgroups read
+    # their aprons and history footprints while a tile is written)
+    good = taastar.FrameImages()
+    names = ("color", "depth", "history", "history_depth", "result_screen", "velocity", "result_history", "debug", "inspect")
+    for i, f in enumerate(names):
+        setattr(good, f, 0x10000 * (i + 1))
+    good.width, good.height, good.pitch = 64, 32, 64
+    for out_field, in_field in (("result_screen", "color"), ("result_screen", "history_depth"), ("result_history", "velocity"),
+                                ("debug", "depth"), ("inspect", "history"), ("debug", "result_screen")):
+        bad = taastar.FrameImages.from_buffer_copy(good)
+        setattr(bad, out_field, getattr(good, in_field))
+        assert lib.taa_resolve_frame(C.byref(U), C.byref(bad), None, 0, None) == _abi.TAA_ERR_INVALID_ARG, out_field
+        assert b"alias" in lib.taa_last_error(), (out_field, in_field, lib.taa_last_error())
 
 
 def test_velocity_argument_errors_need_no_gpu():

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 import taastar  # noqa: E402
 from taastar import (ARITH_FP16, ARITH_FP32, FMT_RGBA16F, FMT_RGBA32F, BatchItem, ServerConfig, ServeReport,  # noqa: E402
                      ServeSimConfig, SessionIO, Completion, check, lib)
-from taastar._abi import TAA_ERR_BUDGET  # noqa: E402
+from taastar._abi import TAA_ERR_BUDGET, TAA_ERR_INVALID_ARG  # noqa: E402
 from taastar.workloads import (cas_for, encode_color, encode_velocity, scene_for, uniforms_for, workload)  # noqa: E402
 
 
@@ -82,6 +82,32 @@ def io_of(s):
 
 
 SIZES = [(160, 90), (96, 200), (333, 121)]
+
+
+def test_batch_refuses_aliasing_screens():
+    """Two items writing one screen, or a screen inside a batched session's buffers, are refused
+    before anything is launched."""
+    w = workload("c2", fmt=FMT_RGBA16F, vel=1)
+    ctx = new_ctx()
+    sess = [open_session(ctx, w, 64, 32), open_session(ctx, w, 48, 40)]
+    scr = [torch.zeros(64 * 40 * 8, dtype=torch.uint8, device="cuda") for _ in sess]
+    U = taastar.Uniforms()
+    items = (BatchItem * 2)()
+    for i in range(2):
+        items[i].session = sess[i]
+        items[i].uniforms = C.pointer(U)
+        items[i].cas = None
+        items[i].result_screen = scr[i].data_ptr()
+    for screens in ((scr[0].data_ptr(), scr[0].data_ptr()), (scr[0].data_ptr(), io_of(sess[0]).color),
+                    (io_of(sess[1]).velocity, scr[1].data_ptr())):
+        items[0].result_screen, items[1].result_screen = screens
+        assert lib().taa_batch_submit(ctx, 2, items, ARITH_FP16, None) == TAA_ERR_INVALID_ARG
+        assert b"result_screen" in lib().taa_last_error()
+        plan = C.c_void_p()
+        assert lib().taa_batch_plan_create(ctx, 2, items, ARITH_FP16, C.byref(plan)) == TAA_ERR_INVALID_ARG
+    for s in sess:
+        check(lib().taa_session_close(s))
+    check(lib().taa_ctx_destroy(ctx))
 
 
 @pytest.mark.parametrize("fmt,arith", [(FMT_RGBA32F, ARITH_FP32), (FMT_RGBA16F, ARITH_FP16), (FMT_RGBA16F, ARITH_FP32)])
