@@ -193,75 +193,103 @@ __device__ __forceinline__ void seg_sincos(float a, float& s, float& c) {
     }
 }
 
-// Bilinear (clamp to edge, exact weights) over an RGBA image -- the sampler of taa.hpp:274 at a
-// position that is not a texel centre.
-template <bool S16>
-__device__ float4 bilinear_rgba(const void* img, int W, int H, int pitch, float u, float v) {
-    float px = u * (float)W - 0.5f, py = v * (float)H - 0.5f;
-    float fx = floorf(px), fy = floorf(py);
-    float tx = px - fx, ty = py - fy;
-    int x0 = clampi(to_int(fx), -1, W), y0 = clampi(to_int(fy), -1, H);
-    int xa = clampi(x0, 0, W - 1), xb = clampi(x0 + 1, 0, W - 1);
-    int ya = clampi(y0, 0, H - 1), yb = clampi(y0 + 1, 0, H - 1);
-    float4 a = ld_rgba<S16>(img, (uint32_t)(ya * pitch + xa)), b = ld_rgba<S16>(img, (uint32_t)(ya * pitch + xb));
-    float4 c = ld_rgba<S16>(img, (uint32_t)(yb * pitch + xa)), d = ld_rgba<S16>(img, (uint32_t)(yb * pitch + xb));
-    float ux = 1.0f - tx, uy = 1.0f - ty;
+// Bilinear (clamp to edge, exact weights) -- the sampler of taa.hpp:274 at a position that is
+// not a texel centre -- with the texel fetch `at(x, y)` (clamped coordinates) supplied by the
+// caller, so that a staged copy and the image in global memory give the same bits.
+// One axis of a bilinear sample: the two texels (clamped) and their weights t (second) and
+// u = 1 - t (first). The axes are independent, so per-column and per-row tables of it give the
+// same bits as evaluating it per sample.
+struct BilinearAxis {
+    int a, b;
+    float t, u;
+};
+__device__ __forceinline__ BilinearAxis bilinear_axis(int W, float u) {
+    const float px = u * (float)W - 0.5f;
+    const float fx = floorf(px);
+    BilinearAxis r;
+    r.t = px - fx;
+    const int x0 = clampi(to_int(fx), -1, W);
+    r.a = clampi(x0, 0, W - 1);
+    r.b = clampi(x0 + 1, 0, W - 1);
+    r.u = 1.0f - r.t;
+    return r;
+}
+struct BilinearTaps {
+    int xa, xb, ya, yb;
+    float tx, ty, ux, uy;
+};
+__device__ __forceinline__ BilinearTaps bilinear_taps(const BilinearAxis& x, const BilinearAxis& y) {
+    return BilinearTaps{x.a, x.b, y.a, y.b, x.t, y.t, x.u, y.u};
+}
+__device__ __forceinline__ BilinearTaps bilinear_taps(int W, int H, float u, float v) {
+    return bilinear_taps(bilinear_axis(W, u), bilinear_axis(H, v));
+}
+template <class F>
+__device__ __forceinline__ float4 bilinear4_at(const BilinearTaps& t, F&& at) {
+    const float tx = t.tx, ty = t.ty, ux = t.ux, uy = t.uy;
+    float4 a = at(t.xa, t.ya), b = at(t.xb, t.ya);
+    float4 c = at(t.xa, t.yb), d = at(t.xb, t.yb);
     float4 top = make_float4(a.x * ux + b.x * tx, a.y * ux + b.y * tx, a.z * ux + b.z * tx, a.w * ux + b.w * tx);
     float4 bot = make_float4(c.x * ux + d.x * tx, c.y * ux + d.y * tx, c.z * ux + d.z * tx, c.w * ux + d.w * tx);
     return make_float4(top.x * uy + bot.x * ty, top.y * uy + bot.y * ty, top.z * uy + bot.z * ty, top.w * uy + bot.w * ty);
+}
+template <class F>
+__device__ __forceinline__ float4 bilinear4_at(int W, int H, float u, float v, F&& at) {
+    return bilinear4_at(bilinear_taps(W, H, u, v), at);
+}
+template <class F>
+__device__ __forceinline__ float bilinear1_at(const BilinearTaps& t, F&& at) {
+    float a = at(t.xa, t.ya), b = at(t.xb, t.ya);
+    float c = at(t.xa, t.yb), d = at(t.xb, t.yb);
+    return (a * t.ux + b * t.tx) * t.uy + (c * t.ux + d * t.tx) * t.ty;
+}
+template <class F>
+__device__ __forceinline__ float bilinear1_at(int W, int H, float u, float v, F&& at) {
+    return bilinear1_at(bilinear_taps(W, H, u, v), at);
+}
+
+// over an RGBA image in global memory
+template <bool S16>
+__device__ float4 bilinear_rgba(const void* img, int W, int H, int pitch, float u, float v) {
+    return bilinear4_at(W, H, u, v, [&](int x, int y) { return ld_rgba<S16>(img, (uint32_t)(y * pitch + x)); });
 }
 
 // Bilinear samples of the velocity (format fmt) and of a depth image at a non-centre uv, with the
 // weights and the order of operations of bilinear_rgba: texture() of the upsampling path, where
 // the output texel's uv is not a centre of the input-sized depth and velocity images.
 __device__ float4 bilinear_vel(const void* img, int fmt, int W, int H, int pitch, float u, float v) {
-    float px = u * (float)W - 0.5f, py = v * (float)H - 0.5f;
-    float fx = floorf(px), fy = floorf(py);
-    float tx = px - fx, ty = py - fy;
-    int x0 = clampi(to_int(fx), -1, W), y0 = clampi(to_int(fy), -1, H);
-    int xa = clampi(x0, 0, W - 1), xb = clampi(x0 + 1, 0, W - 1);
-    int ya = clampi(y0, 0, H - 1), yb = clampi(y0 + 1, 0, H - 1);
-    float4 a = ld_vel(img, fmt, (uint32_t)(ya * pitch + xa)), b = ld_vel(img, fmt, (uint32_t)(ya * pitch + xb));
-    float4 c = ld_vel(img, fmt, (uint32_t)(yb * pitch + xa)), d = ld_vel(img, fmt, (uint32_t)(yb * pitch + xb));
-    float ux = 1.0f - tx, uy = 1.0f - ty;
-    float4 top = make_float4(a.x * ux + b.x * tx, a.y * ux + b.y * tx, a.z * ux + b.z * tx, a.w * ux + b.w * tx);
-    float4 bot = make_float4(c.x * ux + d.x * tx, c.y * ux + d.y * tx, c.z * ux + d.z * tx, c.w * ux + d.w * tx);
-    return make_float4(top.x * uy + bot.x * ty, top.y * uy + bot.y * ty, top.z * uy + bot.z * ty, top.w * uy + bot.w * ty);
+    return bilinear4_at(W, H, u, v, [&](int x, int y) { return ld_vel(img, fmt, (uint32_t)(y * pitch + x)); });
 }
 __device__ float bilinear_f32(const float* img, int W, int H, int pitch, float u, float v) {
-    float px = u * (float)W - 0.5f, py = v * (float)H - 0.5f;
-    float fx = floorf(px), fy = floorf(py);
-    float tx = px - fx, ty = py - fy;
-    int x0 = clampi(to_int(fx), -1, W), y0 = clampi(to_int(fy), -1, H);
-    int xa = clampi(x0, 0, W - 1), xb = clampi(x0 + 1, 0, W - 1);
-    int ya = clampi(y0, 0, H - 1), yb = clampi(y0 + 1, 0, H - 1);
-    float a = ld_f32(img, (uint32_t)(ya * pitch + xa)), b = ld_f32(img, (uint32_t)(ya * pitch + xb));
-    float c = ld_f32(img, (uint32_t)(yb * pitch + xa)), d = ld_f32(img, (uint32_t)(yb * pitch + xb));
-    float ux = 1.0f - tx, uy = 1.0f - ty;
-    return (a * ux + b * tx) * uy + (c * ux + d * tx) * ty;
+    return bilinear1_at(W, H, u, v, [&](int x, int y) { return ld_f32(img, (uint32_t)(y * pitch + x)); });
 }
 
-// getCurrentUpsampledColor (taa.comp:222-257): the input sample whose jittered position lands in
-// output texel (tcx, tcy) after scaling, fetched at its texel; beta = 0 when there is none.
-template <bool S16>
-__device__ f3 current_upsampled(const FrameDesc& fd, const taa_params_t& P, int tcx, int tcy, float jux, float juy,
-                                float& beta) {
-    const float lx = (float)fd.Wl, ly = (float)fd.Hl, hx = (float)fd.W, hy = (float)fd.H;
-    const float scx = hx / lx, scy = hy / ly;
-    const float tjx = jux * lx * -1.0f, tjy = juy * ly * -1.0f;
-    const float a1 = 0.999999f;
-    const float ox[4] = {0.0f, a1, 0.0f, a1}, oy[4] = {0.0f, 0.0f, a1, a1};
+// getCurrentUpsampledColor (taa.comp:222-257), one axis: the jittered input sample position s
+// (input texels) for output texel tc and corner offset o, and whether it lands in tc after scaling
+// by h / l. l, h: input and output size; tj = jitter * l * -1 (input texels).
+__device__ __forceinline__ float upsample_axis(float l, float h, float tj, int tc, float o, bool& hit) {
+    const float sc = h / l;
+    const float s = floorf(l * ((float)tc + o) / h) + 0.5f + tj;
+    hit = to_int(s * sc) == tc;
+    return s;
+}
+constexpr float kUpsampleCorner = 0.999999f;  // the second corner offset of the search (taa.comp:236-239)
+
+// The sample of the four corners (0,0), (c,0), (0,c), (c,c) -- the last that lands in the output
+// texel on both axes -- fetched at its texel through `color_at(x, y)` (converted colour, clamped
+// coordinates); beta = 0 when there is none. sx/hx: upsample_axis at offsets 0 and c for the
+// column, sy/hy for the row.
+template <class F>
+__device__ f3 current_upsampled(const float (&sx)[2], const bool (&hx)[2], const float (&sy)[2], const bool (&hy)[2],
+                                int Wl, int Hl, float& beta, F&& color_at) {
     float fx = -1.0f, fy = -1.0f;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const float sx = floorf(lx * ((float)tcx + ox[k]) / hx) + 0.5f + tjx;
-        const float sy = floorf(ly * ((float)tcy + oy[k]) / hy) + 0.5f + tjy;
-        if (to_int(sx * scx) == tcx && to_int(sy * scy) == tcy) { fx = sx; fy = sy; }
-    }
+    for (int k = 0; k < 4; ++k)
+        if (hx[k & 1] && hy[k >> 1]) { fx = sx[k & 1]; fy = sy[k >> 1]; }
     if (fx >= 0.0f) {
         beta = 1.0f;
-        const int ix = clampi(to_int(floorf(fx)), 0, fd.Wl - 1), iy = clampi(to_int(floorf(fy)), 0, fd.Hl - 1);
-        return stage_convert(P, xyz(ld_rgba<S16>(fd.color, (uint32_t)(iy * fd.pitch_l + ix))));
+        const int ix = clampi(to_int(floorf(fx)), 0, Wl - 1), iy = clampi(to_int(floorf(fy)), 0, Hl - 1);
+        return color_at(ix, iy);
     }
     beta = 0.0f;
     return f3{0.0f, 0.0f, 0.0f};

@@ -30,13 +30,17 @@ namespace {
 template <int TW_, int TH_, int PAD_, bool CAS_, bool UP_ = false, bool SEG_ = false>
 struct Geo {
     static constexpr int TW = TW_, TH = TH_, PAD = PAD_;
-    static constexpr bool CAS = CAS_, UP = UP_;  // UP: upsampling path, inputs read from global memory
+    static constexpr bool CAS = CAS_, UP = UP_;  // UP: upsampling path (input size != output size)
     static constexpr bool SEG = SEG_;            // SEG: the segmentation mask's inputs are staged too
     static constexpr int NT = 256;
     static constexpr int A = CAS ? 2 : 1;  // input apron
     static constexpr int SW = TW + 2 * A, SH = TH + 2 * A, SP = SW + PAD;
     static constexpr int RA = CAS ? 1 : 0;  // resolve apron (CAS needs the resolved cross)
     static constexpr int RW = TW + 2 * RA, RH = TH + 2 * RA, RP = RW + PAD;
+    // upsampling path: the input texels under the resolved region (+3 on every side) are staged
+    // when they fit a UC x UC square (every scale >= 1 at these tile sizes); fetches outside it
+    // read global memory
+    static constexpr int UC = UP ? 24 : 1;
     struct Smem {
         float c0[UP ? 1 : SH * SP], c1[UP ? 1 : SH * SP], c2[UP ? 1 : SH * SP], d[UP ? 1 : SH * SP];
         float r0[CAS ? RH * RP : 1], r1[CAS ? RH * RP : 1], r2[CAS ? RH * RP : 1];
@@ -44,7 +48,36 @@ struct Geo {
         // segmentation inputs of the staged region: unit normal, raw YCoCg luma, material id
         float nx[SEG ? SH * SP : 1], ny[SEG ? SH * SP : 1], nz[SEG ? SH * SP : 1], lum[SEG ? SH * SP : 1];
         uint32_t mat[SEG ? SH * SP : 1];
+        // upsampling path: converted colour, depth and velocity (as ld_vel reads it) of the staged square
+        float4 uvel[UC * UC];
+        float uc0[UC * UC], uc1[UC * UC], uc2[UC * UC], ud[UC * UC];
+        // upsampling path, per resolved column (row): the bilinear axis of the depth and velocity
+        // samples at uv + (d - 1) input texels, d = 0..2, and the current-sample search at the
+        // corner offsets 0 and c (position, hit)
+        BilinearAxis uax[UP ? 3 * RW : 1], uay[UP ? 3 * RH : 1];
+        float usx[UP ? 2 * RW : 1], usy[UP ? 2 * RH : 1];
+        int uhx[UP ? 2 * RW : 1], uhy[UP ? 2 * RH : 1];
     };
+};
+
+// The input-sized images of the upsampling path as staged for one tile: input texel (x, y)
+// (clamped coordinates) is entry (y - y0) * n + (x - x0) when 0 <= x - x0 < w and 0 <= y - y0 < h;
+// w = 0 when nothing is staged.
+struct UpIn {
+    const float* c0;
+    const float* c1;
+    const float* c2;
+    const float* d;
+    const float4* v;
+    int x0, y0, w, h, n;
+    // the per-column and per-row tables (column j of the resolve region at [d * rw + j])
+    const BilinearAxis* ax;
+    const BilinearAxis* ay;
+    const float* sx;
+    const float* sy;
+    const int* hx;
+    const int* hy;
+    int rw, rh;
 };
 
 // the staged segmentation planes (row pitch SP, same indexing as the colour planes)
@@ -133,12 +166,13 @@ __device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, c
 
 // One output texel of taa.comp main (taa.comp:707-960), FP32 arithmetic. `l` indexes the texel
 // in the staged LDS planes (row pitch SP). SEG: the segmentation mask's images are in `sg`.
-template <bool S16, int SP, bool UP, bool SEG = false>
+template <bool S16, int SP, bool UP, bool SEG = false, bool STG = false>
 __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
                                          int x, int y, int l, const float* __restrict__ s0,
                                          const float* __restrict__ s1, const float* __restrict__ s2,
                                          const float* __restrict__ sd, float uvx, float uvy, bool write_aux,
-                                         const SegDesc* sg = nullptr, const SegPlanes* spl = nullptr) {
+                                         const SegDesc* sg = nullptr, const SegPlanes* spl = nullptr,
+                                         const UpIn* up = nullptr, int rx = 0, int ry = 0) {
     // uvx, uvy: tc_to_uv(x, y) = ((x, y) + 0.5) / size (taa.comp:131), tabulated per tile
     const int W = fd.W, H = fd.H;
     const uint32_t gi = (uint32_t)(y * fd.pitch + x);
@@ -151,6 +185,31 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         yl = to_int(uvy * (float)Hl);
     }
     auto lo = [&](int xx, int yy) { return (uint32_t)(clampi(yy, 0, Hl - 1) * pl + clampi(xx, 0, Wl - 1)); };
+    // upsampling path: input texel (xx, yy) (clamped coordinates) from the staged square (STG: the
+    // caller has checked that every fetch of this texel lies in it), else from global memory --
+    // the same values either way
+    auto staged = [&](int xx, int yy) { return (yy - up->y0) * up->n + (xx - up->x0); };
+    auto color_at = [&](int xx, int yy) -> f3 {
+        if constexpr (STG) {
+            const int k = staged(xx, yy);
+            return f3{up->c0[k], up->c1[k], up->c2[k]};
+        } else {
+            return stage_convert(P, xyz(ld_rgba<S16>(fd.color, (uint32_t)(yy * pl + xx))));
+        }
+    };
+    auto depth_at = [&](int xx, int yy) -> float {
+        if constexpr (STG) return up->d[staged(xx, yy)];
+        else return ld_f32(fd.depth, (uint32_t)(yy * pl + xx));
+    };
+    auto vel_at = [&](int xx, int yy) -> float4 {
+        if constexpr (STG) return up->v[staged(xx, yy)];
+        else return ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(yy * pl + xx));
+    };
+    // the bilinear taps of a depth or velocity sample at uv + (dx, dy) input texels, dx, dy in -1..1
+    // (SAMPLE_TEX at uv + toU * offset), from the tile's tables
+    auto up_taps = [&](int dx, int dy) {
+        return bilinear_taps(up->ax[(dx + 1) * up->rw + rx], up->ay[(dy + 1) * up->rh + ry]);
+    };
     const float toUx = 1.0f / (float)Wl, toUy = 1.0f / (float)Hl;  // 1 / textureSize(depth, velocity)
 
     if (P.mPassThrough || U.mBypassHistoryUpdate) {  // taa.comp:726-737
@@ -177,7 +236,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             if constexpr (UP) {
-                c[k] = stage_convert(P, xyz(ld_rgba<S16>(fd.color, lo(xl + ox[k], yl + oy[k]))));
+                c[k] = color_at(clampi(xl + ox[k], 0, Wl - 1), clampi(yl + oy[k], 0, Hl - 1));
             } else {
                 int li = l + oy[k] * SP + ox[k];
                 c[k] = {s0[li], s1[li], s2[li]};
@@ -231,7 +290,10 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     bool upsampled = false;
     if constexpr (UP) {
         if (U.mUpsampling) {
-            currentColor = current_upsampled<S16>(fd, P, x, y, jux, juy, beta);
+            const float sx[2] = {up->sx[rx], up->sx[up->rw + rx]}, sy[2] = {up->sy[ry], up->sy[up->rh + ry]};
+            const bool hx[2] = {up->hx[rx] != 0, up->hx[up->rw + rx] != 0};
+            const bool hy[2] = {up->hy[ry] != 0, up->hy[up->rh + ry] != 0};
+            currentColor = current_upsampled(sx, hx, sy, hy, Wl, Hl, beta, color_at);
             upsampled = true;
         }
     }
@@ -240,7 +302,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         currentColor = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, Wl, Hl, pl, u, v)));
     }
     float depth;  // taa.comp:756
-    if constexpr (UP) depth = ld_f32(fd.depth, lo(xl, yl));
+    if constexpr (UP) depth = depth_at(clampi(xl, 0, Wl - 1), clampi(yl, 0, Hl - 1));
     else depth = sd[l];
 
     // ---- getHistoryPosition (taa.comp:391-438)
@@ -248,9 +310,8 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     float hu, hv, expectedHistoryDepth;
     {
         // SAMPLE_TEX(uCurrentVelocity, uv): a texel fetch at one size, bilinear when upsampling
-        auto vel_at_uv = [&](float u, float v) { return bilinear_vel(fd.velocity, fd.vel_fmt, Wl, Hl, pl, u, v); };
         float4 vs;
-        if constexpr (UP) vs = vel_at_uv(uvx, uvy);
+        if constexpr (UP) vs = bilinear4_at(up_taps(0, 0), vel_at);
         else vs = ld_vel(fd.velocity, fd.vel_fmt, gi);
         bool canUseVelocity = !(P.mUseVelocityVectors == 0 || (P.mUseVelocityVectors == 1 && vs.w < 0.5f));
         if (canUseVelocity) {
@@ -258,7 +319,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
             if (P.mVelocitySampleMode == 1) {
                 // taa.comp:398-412: the longest vector is tracked but the last tap is assigned
                 float4 last;
-                if constexpr (UP) last = vel_at_uv(uvx + toUx, uvy + toUy);
+                if constexpr (UP) last = bilinear4_at(up_taps(1, 1), vel_at);
                 else last = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1)));
                 vs.x = last.x;
                 vs.y = last.y;
@@ -266,12 +327,12 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
                 // findClosestUvAndZ_3x3 (taa.comp:371-389): scan order row-major, strict <
                 int best = 0;
                 if constexpr (UP) {  // depth taps at the output uv, interpolated
-                    float dC = bilinear_f32(fd.depth, Wl, Hl, pl, uvx + toUx * -1.0f, uvy + toUy * -1.0f);
+                    float dC = bilinear1_at(up_taps(-1, -1), depth_at);
                     for (int k = 1; k < 9; ++k) {
-                        float d = bilinear_f32(fd.depth, Wl, Hl, pl, uvx + toUx * (float)(k % 3 - 1), uvy + toUy * (float)(k / 3 - 1));
+                        float d = bilinear1_at(up_taps(k % 3 - 1, k / 3 - 1), depth_at);
                         if (d < dC) { dC = d; best = k; }
                     }
-                    vs = vel_at_uv(uvx + toUx * (float)(best % 3 - 1), uvy + toUy * (float)(best / 3 - 1));
+                    vs = bilinear4_at(up_taps(best % 3 - 1, best / 3 - 1), vel_at);
                 } else {
                     float dC = sd[l - SP - 1];
 #pragma unroll
@@ -356,7 +417,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         const float eps = 1e-5f;
         auto mov = [&](int dx, int dy) {
             float4 v;
-            if constexpr (UP) v = bilinear_vel(fd.velocity, fd.vel_fmt, Wl, Hl, pl, uvx + toUx * (float)dx, uvy + toUy * (float)dy);
+            if constexpr (UP) v = bilinear4_at(up_taps(dx, dy), vel_at);
             else v = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1)));
             return (fabsf(v.x) > eps || fabsf(v.y) > eps) && (fabsf(v.w) >= 0.5f);
         };
@@ -514,6 +575,72 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
             sm.mat[li] = gload<uint32_t>(sg->material, si * 4u);
         }
     }
+    // upsampling path: stage the input texels under the resolved region. xl = int(uv * input size)
+    // grows with the output column (uv as tabulated below), so the region's first and last
+    // columns bound it. With an input no larger than the output, every fetch of a resolved texel
+    // lies within 2 input texels of its (xl, yl): the neighbourhood at +-1, the bilinear taps of
+    // depth and velocity at uv + (-1..1) input texels (floor(uv * size - 0.5) + {-1, 0, 1, 2} around
+    // xl, the float error of uv * size being far below half a texel), the upsampled sample inside
+    // the output texel's footprint (+-1); the square keeps 3. An input larger than the output
+    // stages nothing and every fetch reads global memory.
+    UpIn ui{sm.uc0, sm.uc1, sm.uc2, sm.ud, sm.uvel, 0, 0, 0, 0, G::UC,
+            sm.uax, sm.uay, sm.usx, sm.usy, sm.uhx, sm.uhy, G::RW, G::RH};
+    if constexpr (G::UP) {
+        const int Wl = fd.Wl, Hl = fd.Hl, pl = fd.pitch_l;
+        // per-column and per-row tables of the resolve region (clamped texel, uv as tabulated
+        // below): the bilinear axes of SAMPLE_TEX at uv + (d - 1) / input size, and one axis of
+        // getCurrentUpsampledColor's search at both corner offsets
+        {
+            const float toUx = 1.0f / (float)Wl, toUy = 1.0f / (float)Hl;
+            const float jux = U.mJitterNdc[0] * 0.5f * P.mUnjitterFactor, juy = U.mJitterNdc[1] * 0.5f * P.mUnjitterFactor;
+            const float lx = (float)Wl, ly = (float)Hl, hxs = (float)W, hys = (float)H;
+            const float tjx = jux * lx * -1.0f, tjy = juy * ly * -1.0f;
+            for (int i = tid; i < 5 * (G::RW + G::RH); i += G::NT) {
+                const bool col = i < 5 * G::RW;
+                const int n = col ? G::RW : G::RH, e = col ? i : i - 5 * G::RW;
+                const int q = e / n, j = e - q * n;  // q = 0..2: bilinear offset, 3..4: search corner
+                const int c = col ? clampi(x0 - G::RA + j, 0, W - 1) : clampi(y0 - G::RA + j, 0, H - 1);
+                const float uv = col ? ((float)c + 0.5f) / (float)W : ((float)c + 0.5f) / (float)H;
+                if (q < 3) {
+                    const BilinearAxis ba = col ? bilinear_axis(Wl, uv + toUx * (float)(q - 1))
+                                                : bilinear_axis(Hl, uv + toUy * (float)(q - 1));
+                    (col ? sm.uax : sm.uay)[q * n + j] = ba;
+                } else {
+                    bool hit;
+                    const float o = q == 3 ? 0.0f : kUpsampleCorner;
+                    const float sp = col ? upsample_axis(lx, hxs, tjx, c, o, hit) : upsample_axis(ly, hys, tjy, c, o, hit);
+                    (col ? sm.usx : sm.usy)[(q - 3) * n + j] = sp;
+                    (col ? sm.uhx : sm.uhy)[(q - 3) * n + j] = hit ? 1 : 0;
+                }
+            }
+        }
+        const int cx_lo = clampi(x0 - G::RA, 0, W - 1), cx_hi = clampi(x0 + G::TW + G::RA - 1, 0, W - 1);
+        const int cy_lo = clampi(y0 - G::RA, 0, H - 1), cy_hi = clampi(y0 + G::TH + G::RA - 1, 0, H - 1);
+        const int xl_lo = to_int(((float)cx_lo + 0.5f) / (float)W * (float)Wl);
+        const int xl_hi = to_int(((float)cx_hi + 0.5f) / (float)W * (float)Wl);
+        const int yl_lo = to_int(((float)cy_lo + 0.5f) / (float)H * (float)Hl);
+        const int yl_hi = to_int(((float)cy_hi + 0.5f) / (float)H * (float)Hl);
+        const int w = xl_hi - xl_lo + 7, h = yl_hi - yl_lo + 7;
+        // staged only for inputs no larger than the output (scale >= 1), where the bounds below hold
+        if (Wl <= W && Hl <= H && w <= G::UC && h <= G::UC) {
+            ui.x0 = xl_lo - 3;
+            ui.y0 = yl_lo - 3;
+            ui.w = w;
+            ui.h = h;
+            for (int i = tid; i < w * h; i += G::NT) {
+                const int sy = i / w, sx = i - sy * w;
+                const int gx = clampi(ui.x0 + sx, 0, Wl - 1), gy = clampi(ui.y0 + sy, 0, Hl - 1);
+                const uint32_t gi = (uint32_t)(gy * pl + gx);
+                const f3 t = stage_convert(P, xyz(ld_rgba<S16>(fd.color, gi)));
+                const int k = sy * G::UC + sx;
+                sm.uc0[k] = t.x;
+                sm.uc1[k] = t.y;
+                sm.uc2[k] = t.z;
+                sm.ud[k] = ld_f32(fd.depth, gi);
+                sm.uvel[k] = ld_vel(fd.velocity, fd.vel_fmt, gi);
+            }
+        }
+    }
     if (tid < G::SW) sm.ux[tid] = ((float)clampi(gx0 + tid, 0, W - 1) + 0.5f) / (float)W;
     else if (tid < G::SW + G::SH) sm.uy[tid - G::SW] = ((float)clampi(gy0 + tid - G::SW, 0, H - 1) + 0.5f) / (float)H;
     __syncthreads();
@@ -526,8 +653,24 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         const bool owned = rx >= G::RA && rx < G::RA + G::TW && ry >= G::RA && ry < G::RA + G::TH && px < fd.x_end && py < H;
         const int l = (cy - gy0) * G::SP + (cx - gx0);
         const SegPlanes spl{sm.nx, sm.ny, sm.nz, sm.lum, sm.mat};
-        PixelResult r = resolve_pixel_f32<S16, G::SP, G::UP, SEG>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
-                                                                  sm.ux[cx - gx0], sm.uy[cy - gy0], owned, sg, &spl);
+        const float uvx = sm.ux[cx - gx0], uvy = sm.uy[cy - gy0];
+        PixelResult r;
+        if constexpr (G::UP) {
+            // every input fetch of this texel lies within 2 input texels of (xl, yl) = int(uv * input
+            // size) (bilinear taps at uv +- one texel, the neighbourhood, the upsampled sample):
+            // inside the staged square the staged variant reads only LDS
+            const int xl = to_int(uvx * (float)fd.Wl), yl = to_int(uvy * (float)fd.Hl);
+            const bool in = ui.w > 0 && xl - 2 >= ui.x0 && xl + 2 < ui.x0 + ui.w && yl - 2 >= ui.y0 && yl + 2 < ui.y0 + ui.h;
+            if (in)
+                r = resolve_pixel_f32<S16, G::SP, true, false, true>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
+                                                                     owned, sg, &spl, &ui, rx, ry);
+            else
+                r = resolve_pixel_f32<S16, G::SP, true, false, false>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
+                                                                      owned, sg, &spl, &ui, rx, ry);
+        } else {
+            r = resolve_pixel_f32<S16, G::SP, false, SEG>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy, owned, sg,
+                                                          &spl);
+        }
         if (owned) {
             const uint32_t gi = (uint32_t)(cy * pitch + cx);
             st_rgba<S16>(fd.history_out, gi, r.history);
@@ -573,8 +716,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3
     resolve_tile<G, S16>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
 }
 
-// Upsampling path (input size != output size, or mUpsampling): exact FP32 arithmetic, inputs read
-// from global memory around each output texel's input position, fused CAS as above.
 // the resolve with the segmentation mask of mRayTraceAugment
 template <int TW, int TH, int PAD, bool CAS, bool S16>
 __global__ __launch_bounds__(256) void k_resolve_seg(const ResolveArgs a, const SegDesc sg) {
@@ -584,6 +725,9 @@ __global__ __launch_bounds__(256) void k_resolve_seg(const ResolveArgs a, const 
     resolve_tile<G, S16, true>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm, &sg);
 }
 
+// Upsampling path (input size != output size, or mUpsampling): exact FP32 arithmetic; the input
+// texels under the tile are staged in LDS (converted colour, depth, velocity), the rest of the
+// resolve as above, fused CAS from LDS.
 template <int TW, int TH, int PAD, bool CAS, bool S16>
 __global__ __launch_bounds__(256) void k_resolve_up(const ResolveArgs a) {
     using G = Geo<TW, TH, PAD, CAS, true>;

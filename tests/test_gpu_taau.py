@@ -89,6 +89,9 @@ CASES = {
     "2x_nearest": dict(lo=(96, 54), hi=(192, 108), upsampling=False),
     "1.33x_ragged_nocas": dict(lo=(75, 41), hi=(101, 57), upsampling=True, cas=False),
     "1x_upsampling_flag": dict(lo=(160, 90), hi=(160, 90), upsampling=True),
+    # an input larger than the output: the tile's input texels do not fit the staged square and
+    # every fetch reads global memory
+    "0.5x_supersampled": dict(lo=(192, 108), hi=(96, 54), upsampling=True),
 }
 
 
