@@ -68,6 +68,9 @@ constexpr int kDefaultShapeCas = 2;
 // default flags: 116 us against 127 us for 30x14 and 155 us for 30x30)
 constexpr int kSegShapeCas = 0;
 constexpr int kSegShapePlain = kDefaultShapePlain;
+// tile shapes of the upsampling resolve (its LDS holds the staged input rectangle)
+constexpr int kUpShapeCas = 0;
+constexpr int kUpShapePlain = kDefaultShapePlain;
 
 inline int tiles_for(int extent, int t) { return (extent + t - 1) / t; }
 

@@ -38,9 +38,8 @@ struct Geo {
     static constexpr int RA = CAS ? 1 : 0;  // resolve apron (CAS needs the resolved cross)
     static constexpr int RW = TW + 2 * RA, RH = TH + 2 * RA, RP = RW + PAD;
     // upsampling path: the input texels under the resolved region (+3 on every side) are staged
-    // when they fit a UC x UC square (every scale >= 1 at these tile sizes); fetches outside it
-    // read global memory
-    static constexpr int UC = UP ? 24 : 1;
+    // in a UCW x UCH rectangle, which holds them for every scale >= 1 (at most RW + 6 columns)
+    static constexpr int UCW = UP ? RW + 8 : 1, UCH = UP ? RH + 8 : 1;
     struct Smem {
         float c0[UP ? 1 : SH * SP], c1[UP ? 1 : SH * SP], c2[UP ? 1 : SH * SP], d[UP ? 1 : SH * SP];
         float r0[CAS ? RH * RP : 1], r1[CAS ? RH * RP : 1], r2[CAS ? RH * RP : 1];
@@ -48,9 +47,9 @@ struct Geo {
         // segmentation inputs of the staged region: unit normal, raw YCoCg luma, material id
         float nx[SEG ? SH * SP : 1], ny[SEG ? SH * SP : 1], nz[SEG ? SH * SP : 1], lum[SEG ? SH * SP : 1];
         uint32_t mat[SEG ? SH * SP : 1];
-        // upsampling path: converted colour, depth and velocity (as ld_vel reads it) of the staged square
-        float4 uvel[UC * UC];
-        float uc0[UC * UC], uc1[UC * UC], uc2[UC * UC], ud[UC * UC];
+        // upsampling path: converted colour, depth and velocity (as ld_vel reads it) of the staged rectangle
+        float4 uvel[UCW * UCH];
+        float uc0[UCW * UCH], uc1[UCW * UCH], uc2[UCW * UCH], ud[UCW * UCH];
         // upsampling path, per resolved column (row): the bilinear axis of the depth and velocity
         // samples at uv + (d - 1) input texels, d = 0..2, and the current-sample search at the
         // corner offsets 0 and c (position, hit)
@@ -185,7 +184,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         yl = to_int(uvy * (float)Hl);
     }
     auto lo = [&](int xx, int yy) { return (uint32_t)(clampi(yy, 0, Hl - 1) * pl + clampi(xx, 0, Wl - 1)); };
-    // upsampling path: input texel (xx, yy) (clamped coordinates) from the staged square (STG: the
+    // upsampling path: input texel (xx, yy) (clamped coordinates) from the staged rectangle (STG: the
     // caller has checked that every fetch of this texel lies in it), else from global memory --
     // the same values either way
     auto staged = [&](int xx, int yy) { return (yy - up->y0) * up->n + (xx - up->x0); };
@@ -581,9 +580,9 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
     // lies within 2 input texels of its (xl, yl): the neighbourhood at +-1, the bilinear taps of
     // depth and velocity at uv + (-1..1) input texels (floor(uv * size - 0.5) + {-1, 0, 1, 2} around
     // xl, the float error of uv * size being far below half a texel), the upsampled sample inside
-    // the output texel's footprint (+-1); the square keeps 3. An input larger than the output
+    // the output texel's footprint (+-1); the rectangle keeps 3. An input larger than the output
     // stages nothing and every fetch reads global memory.
-    UpIn ui{sm.uc0, sm.uc1, sm.uc2, sm.ud, sm.uvel, 0, 0, 0, 0, G::UC,
+    UpIn ui{sm.uc0, sm.uc1, sm.uc2, sm.ud, sm.uvel, 0, 0, 0, 0, G::UCW,
             sm.uax, sm.uay, sm.usx, sm.usy, sm.uhx, sm.uhy, G::RW, G::RH};
     if constexpr (G::UP) {
         const int Wl = fd.Wl, Hl = fd.Hl, pl = fd.pitch_l;
@@ -622,7 +621,7 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         const int yl_hi = to_int(((float)cy_hi + 0.5f) / (float)H * (float)Hl);
         const int w = xl_hi - xl_lo + 7, h = yl_hi - yl_lo + 7;
         // staged only for inputs no larger than the output (scale >= 1), where the bounds below hold
-        if (Wl <= W && Hl <= H && w <= G::UC && h <= G::UC) {
+        if (Wl <= W && Hl <= H && w <= G::UCW && h <= G::UCH) {
             ui.x0 = xl_lo - 3;
             ui.y0 = yl_lo - 3;
             ui.w = w;
@@ -632,7 +631,7 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
                 const int gx = clampi(ui.x0 + sx, 0, Wl - 1), gy = clampi(ui.y0 + sy, 0, Hl - 1);
                 const uint32_t gi = (uint32_t)(gy * pl + gx);
                 const f3 t = stage_convert(P, xyz(ld_rgba<S16>(fd.color, gi)));
-                const int k = sy * G::UC + sx;
+                const int k = sy * G::UCW + sx;
                 sm.uc0[k] = t.x;
                 sm.uc1[k] = t.y;
                 sm.uc2[k] = t.z;
@@ -658,7 +657,7 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         if constexpr (G::UP) {
             // every input fetch of this texel lies within 2 input texels of (xl, yl) = int(uv * input
             // size) (bilinear taps at uv +- one texel, the neighbourhood, the upsampled sample):
-            // inside the staged square the staged variant reads only LDS
+            // inside the staged rectangle the staged variant reads only LDS
             const int xl = to_int(uvx * (float)fd.Wl), yl = to_int(uvy * (float)fd.Hl);
             const bool in = ui.w > 0 && xl - 2 >= ui.x0 && xl + 2 < ui.x0 + ui.w && yl - 2 >= ui.y0 && yl + 2 < ui.y0 + ui.h;
             if (in)
@@ -816,7 +815,7 @@ int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, void* stream
     a.U = U;
     const int n = fd.tiles_x * fd.tiles_y;
     const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
-    constexpr TileShapeDef C = kShapesCas[0], Pl = kShapesPlain[0];
+    constexpr TileShapeDef C = kShapesCas[kUpShapeCas], Pl = kShapesPlain[kUpShapePlain];
     if (fd.cas_on) {
         if (s16) hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, true>), dim3(n), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, false>), dim3(n), dim3(256), 0, st, a);
