@@ -382,8 +382,8 @@ typedef struct taa_batch_item {
  * (TAA_ERR_CAPACITY beyond). Sessions must be distinct, and so must the
  * result_screen pointers; no result_screen may lie inside a batched session's buffers
  * (TAA_ERR_INVALID_ARG). Stream-ordered.
- * Upsampling frames (scaled sessions, mUpsampling) are resolved by their own launches on the same
- * stream after the batched one; a batch plan refuses them (TAA_ERR_UNSUPPORTED). */
+ * Upsampling frames (scaled sessions, mUpsampling) go through a second batched launch of the exact
+ * FP32 upsampling resolve in the same call (and in the same captured graph of a plan). */
 int32_t taa_batch_submit(taa_ctx_t* ctx, int32_t n, const taa_batch_item_t* items,
                          int32_t arith, void* stream);
 

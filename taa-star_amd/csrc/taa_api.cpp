@@ -636,91 +636,94 @@ static int check_items(taa_ctx_t* c, int n, const taa_batch_item_t* items) {
     return TAA_OK;
 }
 
-static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int arith, void* host, void* dev, int& shape,
-                       bool& cas, bool& s16, int& total_tiles, bool& fast) {
-    (void)dev;
+// The descriptor table of one batched call: one-size frames first (entries [0, n_main), the
+// main batched kernel), then the upsampling frames (entries [n_main, n_main + n_up), the
+// batched upsampling kernel); each part numbers its tiles from 0.
+struct BatchLayout {
+    int shape = 0;  // tile shape of the main launch
+    bool cas = false, s16 = false, fast = false;
+    int n_main = 0, total_main = 0;
+    int n_up = 0, total_up = 0;
+};
+
+static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int arith, void* host, BatchLayout& L) {
     if (int rc = check_items(c, n, items)) return rc;
     auto* ents = static_cast<taastar::BatchEntry*>(host);
-    cas = items[0].cas != nullptr;
-    s16 = items[0].session->cf == TAA_FMT_RGBA16F;
-    // one tile shape per launch: the one tuned for the largest frame class in the batch
+    L = BatchLayout{};
+    L.cas = items[0].cas != nullptr;
+    L.s16 = items[0].session->cf == TAA_FMT_RGBA16F;
+    // one tile shape per main launch: the one tuned for the largest one-size frame class
     int best_px = -1;
     for (int i = 0; i < n; ++i) {
         const taa_batch_item_t& it = items[i];
         int rc = check_uniforms(it.uniforms);
         if (rc) return rc;
-        if (session_upsamples(it.session, it.uniforms))
-            return fail(TAA_ERR_UNSUPPORTED, "item %d: an upsampling frame in a batch plan", i);
         if (it.uniforms->splitScreen) return fail(TAA_ERR_UNSUPPORTED, "item %d: split screen in a batch", i);
+        if (session_upsamples(it.session, it.uniforms)) {
+            ++L.n_up;
+            continue;
+        }
+        ++L.n_main;
         const int px = it.session->W * it.session->H;
         if (px > best_px) {
             best_px = px;
-            shape = selected_shape(it.session->W, it.session->H, it.session->cf, arith, cas);
+            L.shape = selected_shape(it.session->W, it.session->H, it.session->cf, arith, L.cas);
         }
     }
-    const taastar::TileShapeDef sd = shape_def(shape, cas);
-    int tiles = 0;
+    const taastar::TileShapeDef sd = shape_def(L.shape, L.cas);
+    const taastar::TileShapeDef su = shape_def(L.cas ? taastar::kUpShapeCas : taastar::kUpShapePlain, L.cas);
+    int k_main = 0, k_up = L.n_main;
     for (int i = 0; i < n; ++i) {
         taa_session* s = items[i].session;
-        taastar::BatchEntry& e = ents[i];
+        const bool up = session_upsamples(s, items[i].uniforms);
+        taastar::BatchEntry& e = ents[up ? k_up++ : k_main++];
         std::memset(&e, 0, sizeof(e));
         session_desc(s, items[i].result_screen, items[i].cas, e.fd);
-        e.fd.tiles_x = taastar::tiles_for(s->W, sd.tw);
-        e.fd.tiles_y = taastar::tiles_for(s->H, sd.th);
+        const taastar::TileShapeDef& t = up ? su : sd;
+        if (up) e.fd.up = 1;  // also a one-size frame with mUpsampling
+        e.fd.tiles_x = taastar::tiles_for(s->W, t.tw);
+        e.fd.tiles_y = taastar::tiles_for(s->H, t.th);
         e.U = *items[i].uniforms;
         if (s->force_reset) e.U.mResetHistory = 1;
-        e.tile_begin = tiles;
-        tiles += e.fd.tiles_x * e.fd.tiles_y;
+        int& total = up ? L.total_up : L.total_main;
+        e.tile_begin = total;
+        total += e.fd.tiles_x * e.fd.tiles_y;
     }
-    total_tiles = tiles;
-    fast = arith == TAA_ARITH_FP16;
-    for (int i = 0; i < n && fast; ++i) fast = taastar::fast_path_supported(ents[i].fd, ents[i].U, 0);
+    L.fast = arith == TAA_ARITH_FP16 && L.n_main > 0;
+    for (int i = 0; i < L.n_main && L.fast; ++i) L.fast = taastar::fast_path_supported(ents[i].fd, ents[i].U, 0);
     return TAA_OK;
+}
+
+// The launches of one batched call on `stream` (descriptors already on the device at d_ents).
+static int launch_layout(const taastar::BatchEntry* d_ents, const BatchLayout& L, int arith, void* stream) {
+    if (L.n_main) {
+        const int grid = taastar::batch_grid(L.total_main);
+        int le = L.fast ? taastar::launch_batch_fast(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, grid, stream)
+                        : taastar::launch_batch(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, arith, grid, stream);
+        if (le) return le;
+    }
+    if (L.n_up)
+        return taastar::launch_batch_up(d_ents + L.n_main, L.n_up, L.total_up, L.cas, L.s16, taastar::batch_grid(L.total_up),
+                                        stream);
+    return 0;
 }
 
 int32_t taa_batch_submit(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items, int32_t arith, void* stream) {
     if (!c || !items) return fail(TAA_ERR_INVALID_ARG, "NULL argument");
     if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith");
-    if (n <= 0 || n > taa_ctx::kCap) return fail(TAA_ERR_CAPACITY, "batch of %d items (capacity %d)", n, taa_ctx::kCap);
-    if (int rc = check_item_buffers(c, n, items)) return rc;
-    // upsampling frames (scaled sessions, mUpsampling) take their own launches after the batch
-    int n_up = 0;
-    for (int i = 0; i < n; ++i)
-        if (session_upsamples(items[i].session, items[i].uniforms)) ++n_up;
-    if (n_up) {
-        std::vector<taa_batch_item_t> rest;
-        rest.reserve(n - n_up);
-        for (int i = 0; i < n; ++i)
-            if (!session_upsamples(items[i].session, items[i].uniforms)) rest.push_back(items[i]);
-        if (!rest.empty()) {
-            int rc = taa_batch_submit(c, (int32_t)rest.size(), rest.data(), arith, stream);
-            if (rc) return rc;
-        }
-        for (int i = 0; i < n; ++i) {
-            if (!session_upsamples(items[i].session, items[i].uniforms)) continue;
-            int rc = taa_session_resolve(items[i].session, items[i].uniforms, items[i].cas, items[i].result_screen, arith,
-                                         stream);
-            if (rc) return rc;
-        }
-        return ok();
-    }
     std::lock_guard<std::mutex> lk(c->mu);
     taa_ctx::Slot& slot = c->slots[c->next_slot];
     if (slot.used) {
         hipError_t e = hipEventSynchronize(slot.done);  // the slot's previous batch has been consumed
         if (e != hipSuccess) return hip_fail(e, "staging slot wait");
     }
-    int shape = 0, total = 0;
-    bool cas = false, s16 = false, fast = false;
-    int rc = build_batch(c, n, items, arith, slot.host, slot.dev, shape, cas, s16, total, fast);
+    BatchLayout L;
+    int rc = build_batch(c, n, items, arith, slot.host, L);
     if (rc) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e = hipMemcpyAsync(slot.dev, slot.host, sizeof(taastar::BatchEntry) * (size_t)n, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "descriptor upload");
-    const auto* d_ents = static_cast<const taastar::BatchEntry*>(slot.dev);
-    const int grid = taastar::batch_grid(total);
-    int le = fast ? taastar::launch_batch_fast(d_ents, n, total, shape, cas, s16, grid, stream)
-                  : taastar::launch_batch(d_ents, n, total, shape, cas, s16, arith, grid, stream);
+    int le = launch_layout(static_cast<const taastar::BatchEntry*>(slot.dev), L, arith, stream);
     if (le) return hip_fail((hipError_t)le, "batch launch");
     if ((e = hipEventRecord(slot.done, st)) != hipSuccess) return hip_fail(e, "event record");
     slot.used = true;
@@ -738,13 +741,14 @@ struct taa_batch_plan {
     int n = 0, arith = 0;
     std::vector<taa_batch_item_t> items;
     // two descriptor buffers alternate between launches; each holds one captured graph per kernel
-    // family (FP32 kernel, FP16 kernel), captured the first time that family is needed
+    // family of the one-size frames (FP32 kernel, FP16 kernel), captured the first time that family
+    // is needed and again when the launch layout (tile shape, tile counts, split between one-size
+    // and upsampling frames) changes
     struct Buf {
         void* host = nullptr;
         void* dev = nullptr;
         hipGraphExec_t exec[2] = {nullptr, nullptr};
-        int shape[2] = {-1, -1};
-        int total[2] = {0, 0};
+        BatchLayout layout[2];
         hipEvent_t done = nullptr;
         bool used = false;
     } buf[2];
@@ -766,22 +770,28 @@ void plan_free(taa_batch_plan* p) {
     delete p;
 }
 
-// Capture descriptor upload + persistent kernel for one buffer and kernel family.
-int plan_capture(taa_batch_plan* p, taa_batch_plan::Buf& b, bool fast, int shape, bool cas, bool s16, int total) {
-    const int grid = taastar::batch_grid(total);
+bool same_launches(const BatchLayout& a, const BatchLayout& b) {
+    return a.shape == b.shape && a.fast == b.fast && a.n_main == b.n_main && a.total_main == b.total_main &&
+           a.n_up == b.n_up && a.total_up == b.total_up;
+}
+
+// Capture descriptor upload + the batched launches of a layout for one buffer.
+int plan_capture(taa_batch_plan* p, taa_batch_plan::Buf& b, const BatchLayout& L) {
+    const int fam = L.fast ? 1 : 0;
     hipGraph_t g = nullptr;
     hipError_t e;
     if ((e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
     hipMemcpyAsync(b.dev, b.host, sizeof(taastar::BatchEntry) * (size_t)p->n, hipMemcpyHostToDevice, p->cap);
-    const auto* d_ents = static_cast<const taastar::BatchEntry*>(b.dev);
-    if (fast) taastar::launch_batch_fast(d_ents, p->n, total, shape, cas, s16, grid, p->cap);
-    else taastar::launch_batch(d_ents, p->n, total, shape, cas, s16, p->arith, grid, p->cap);
+    const int le = launch_layout(static_cast<const taastar::BatchEntry*>(b.dev), L, p->arith, p->cap);
     if ((e = hipStreamEndCapture(p->cap, &g)) != hipSuccess) return hip_fail(e, "end capture");
-    e = hipGraphInstantiate(&b.exec[fast ? 1 : 0], g, nullptr, nullptr, 0);
+    if (le) {
+        hipGraphDestroy(g);
+        return hip_fail((hipError_t)le, "batch launch (capture)");
+    }
+    e = hipGraphInstantiate(&b.exec[fam], g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(e, "graph instantiate");
-    b.shape[fast ? 1 : 0] = shape;
-    b.total[fast ? 1 : 0] = total;
+    b.layout[fam] = L;
     return TAA_OK;
 }
 
@@ -822,17 +832,16 @@ int32_t taa_batch_plan_launch(taa_batch_plan_t* p, void* stream) {
         hipError_t e = hipEventSynchronize(b.done);  // this buffer's previous replay has finished
         if (e != hipSuccess) return hip_fail(e, "plan buffer wait");
     }
-    int shape = 0, total = 0;
-    bool cas = false, s16 = false, fast = false;
-    int rc = build_batch(p->ctx, p->n, p->items.data(), p->arith, b.host, b.dev, shape, cas, s16, total, fast);
+    BatchLayout L;
+    int rc = build_batch(p->ctx, p->n, p->items.data(), p->arith, b.host, L);
     if (rc) return rc;
-    const int fam = fast ? 1 : 0;
-    if (b.exec[fam] && (b.shape[fam] != shape || b.total[fam] != total)) {  // tile table changed: recapture
+    const int fam = L.fast ? 1 : 0;
+    if (b.exec[fam] && !same_launches(b.layout[fam], L)) {  // launch layout changed: recapture
         hipGraphExecDestroy(b.exec[fam]);
         b.exec[fam] = nullptr;
     }
     if (!b.exec[fam]) {
-        rc = plan_capture(p, b, fast, shape, cas, s16, total);
+        rc = plan_capture(p, b, L);
         if (rc) return rc;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);

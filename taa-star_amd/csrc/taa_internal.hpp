@@ -89,6 +89,9 @@ inline int batch_grid(int total_tiles) { return (total_tiles + 7) & ~7; }
 
 int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas,
                  bool s16, int arith, int grid, void* stream);
+// the upsampling frames of a batch (exact FP32 upsampling resolve, kUpShapeCas / kUpShapePlain tiles)
+int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int grid,
+                    void* stream);
 
 // Throughput kernel (taa_fast.hip): variance clip + Catmull-Rom + simple/closest-depth velocity
 // with RG velocity, fp16-fed mixed arithmetic. fast_path_supported() says whether a parameter

@@ -753,6 +753,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3
     }
 }
 
+// Batched upsampling resolve: the frames of a batch whose input size differs from the output size
+// (or that set mUpsampling), walked like k_resolve_batch, each tile as in k_resolve_up.
+template <int TW, int TH, int PAD, bool CAS, bool S16>
+__global__ __launch_bounds__(256) void k_resolve_up_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
+    using G = Geo<TW, TH, PAD, CAS, true>;
+    __shared__ typename G::Smem sm;
+    const int per = (total_tiles + 7) >> 3;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
+    for (int j = slot; j < per; j += slots) {
+        const int t = xcd * per + j;
+        if (t >= total_tiles) break;
+        const BatchEntry& e = batch_entry(entries, n_entries, t);
+        resolve_tile<G, S16>(e.fd, e.U, t - e.tile_begin, sm);
+        __syncthreads();  // LDS is reused by the next tile
+    }
+}
+
 // sharpen_cas.comp on its own (one thread per pixel, clamped loads)
 template <bool S16>
 __global__ __launch_bounds__(256) void k_cas(const void* __restrict__ src, void* __restrict__ dst, int W, int H, int pitch, uint32_t peak_bits) {
@@ -849,6 +866,20 @@ int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uin
     dim3 grid((W + 15) / 16, (H + 15) / 16);
     if (fmt == TAA_FMT_RGBA16F) hipLaunchKernelGGL(k_cas<true>, grid, dim3(256), 0, st, src, dst, W, H, pitch, peak_bits);
     else hipLaunchKernelGGL(k_cas<false>, grid, dim3(256), 0, st, src, dst, W, H, pitch, peak_bits);
+    return (int)hipGetLastError();
+}
+
+int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int grid,
+                    void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    constexpr TileShapeDef C = kShapesCas[kUpShapeCas], Pl = kShapesPlain[kUpShapePlain];
+    if (cas) {
+        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, true>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        else hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, false>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+    } else {
+        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, true>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        else hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, false>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+    }
     return (int)hipGetLastError();
 }
 

@@ -173,8 +173,10 @@ def test_scaled_session_equals_resolve_frame(oracle):
 
 
 def test_batch_with_scaled_sessions_equals_alone():
-    """A batch holding two one-size sessions and a scaled one: every result equals the session
-    resolved alone (the scaled frame takes its own launch after the batched one)."""
+    """A batch holding two one-size sessions and a scaled one, submitted and through a graph-captured
+    plan: every result equals the session resolved alone. The upsampling frames go through the
+    batched upsampling kernel; the third session sets mUpsampling on odd frames, which moves it
+    between the two launches (and makes the plan recapture)."""
     from taastar import BatchItem, SessionIO, Uniforms
     print("case taau batch mix", flush=True)
     w = workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F)
@@ -193,7 +195,7 @@ def test_batch_with_scaled_sessions_equals_alone():
             assert hmemcpy(C.c_void_p(ptr), C.c_void_p(a.ctypes.data), a.nbytes, 1) == 0
         return uniforms_for(w, f, jit)
 
-    def run(batched):
+    def run(mode):
         ctx = C.c_void_p()
         check(lib().taa_ctx_create(0, 0, C.byref(ctx)))
         sess, scrs, Ks = [], [], []
@@ -205,29 +207,38 @@ def test_batch_with_scaled_sessions_equals_alone():
             Ks.append(cas_for(w, o[0], o[1]))
         outs = []
         Us = [Uniforms() for _ in specs]
-        for f in range(4):
+        items = (BatchItem * len(specs))()
+        for k in range(len(specs)):
+            items[k].session, items[k].uniforms = sess[k], C.pointer(Us[k])
+            items[k].cas, items[k].result_screen = C.pointer(Ks[k]), scrs[k].data_ptr()
+        plan = C.c_void_p()
+        if mode == "plan":
+            check(lib().taa_batch_plan_create(ctx, len(specs), items, ARITH_FP16, C.byref(plan)))
+        for f in range(6):
             for k, (i, o) in enumerate(specs):
                 U = upload(sess[k], scenes[k], f, *i)
-                U.mUpsampling = int(i != o)
+                U.mUpsampling = int(i != o or (k == 2 and f % 2 == 1))
                 C.memmove(C.addressof(Us[k]), C.addressof(U), C.sizeof(Uniforms))
-            if batched:
-                items = (BatchItem * len(specs))()
-                for k in range(len(specs)):
-                    items[k].session, items[k].uniforms = sess[k], C.pointer(Us[k])
-                    items[k].cas, items[k].result_screen = C.pointer(Ks[k]), scrs[k].data_ptr()
+            if mode == "batch":
                 check(lib().taa_batch_submit(ctx, len(specs), items, ARITH_FP16, None))
+            elif mode == "plan":
+                check(lib().taa_batch_plan_launch(plan, None))
             else:
                 for k in range(len(specs)):
                     check(lib().taa_session_resolve(sess[k], C.byref(Us[k]), C.byref(Ks[k]),
                                                     C.c_void_p(scrs[k].data_ptr()), ARITH_FP16, None))
             torch.cuda.synchronize()
             outs.append([t.cpu().numpy().copy() for t in scrs])
+        if mode == "plan":
+            check(lib().taa_batch_plan_destroy(plan))
         for s in sess:
             check(lib().taa_session_close(s))
         check(lib().taa_ctx_destroy(ctx))
         return outs
 
-    a, b = run(False), run(True)
-    for f in range(4):
-        for k in range(len(specs)):
-            assert np.array_equal(a[f][k], b[f][k]), f"frame {f} session {k}"
+    alone = run("alone")
+    for mode in ("batch", "plan"):
+        got = run(mode)
+        for f in range(6):
+            for k in range(len(specs)):
+                assert np.array_equal(alone[f][k], got[f][k]), f"{mode}: frame {f} session {k}"
