@@ -296,9 +296,15 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
             upsampled = true;
         }
     }
-    if (!upsampled && P.mUnjitterCurrentSample) {
-        float u = jux + ((float)xl + 0.5f) * (1.0f / (float)Wl), v = juy + ((float)yl + 0.5f) * (1.0f / (float)Hl);
-        currentColor = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, Wl, Hl, pl, u, v)));
+    if (!upsampled) {
+        if (P.mUnjitterCurrentSample) {
+            float u = jux + ((float)xl + 0.5f) * (1.0f / (float)Wl), v = juy + ((float)yl + 0.5f) * (1.0f / (float)Hl);
+            currentColor = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, Wl, Hl, pl, u, v)));
+        } else if (P.mUnjitterNeighbourhood) {
+            // c[0] is the unjittered neighbourhood centre; getCurrentColor samples the texel itself
+            if constexpr (UP) currentColor = color_at(clampi(xl, 0, Wl - 1), clampi(yl, 0, Hl - 1));
+            else currentColor = {s0[l], s1[l], s2[l]};
+        }
     }
     float depth;  // taa.comp:756
     if constexpr (UP) depth = depth_at(clampi(xl, 0, Wl - 1), clampi(yl, 0, Hl - 1));

@@ -59,6 +59,9 @@ FEATURES = {
     "karis": dict(mToneMapLumaKaris=1, mVarClipGamma=1.25),
     "antighost": dict(mDynamicAntiGhosting=1),
     "unjitter": dict(mUnjitterNeighbourhood=1, mUnjitterCurrentSample=1),
+    # the neighbourhood unjittered, the current sample not: getCurrentColor fetches the texel
+    # itself instead of reusing the neighbourhood's centre (taa.comp:216-220, 750-753)
+    "unjitter_neighbourhood_only": dict(mUnjitterNeighbourhood=1),
     "passthrough": dict(mPassThrough=1),
 }
 

@@ -1,0 +1,112 @@
+"""Seeded random combinations of the Parameters options against the FP32 transcription of taa.comp.
+
+test_gpu_parity.py turns one option on at a time; this file draws whole parameter blocks (every
+option of taa.hpp:30-77 the resolve implements, with random alpha, gamma and rejection alpha,
+either storage format, any velocity format, CAS on or off, raw or linear depth rejection), so
+that the options' interactions are exercised: Karis tone mapping with shaped neighbourhoods and
+slow clipping, movers-only velocity with anti-ghosting, unjitter with B-spline history, ...
+
+Bars: FP32 arithmetic as in test_gpu_parity.py (indices, taps, flags and the neighbourhood box
+bit-exact, history and screen within 1e-5 relative). FP16 arithmetic (the throughput kernel when
+the block is one it implements, else the exact kernel): indices, taps and flags bit-exact against
+the FP32 transcription, screen PSNR > 55 dB against it (peak 1; CAS output in [0, 1]).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from gpu_helpers import INDEX_WORDS, Sequence, assert_parity, psnr  # noqa: E402
+from taastar import (ARITH_FP16, FMT_RGBA16F, FMT_RGBA32F, VEL_RG16F, VEL_RG32F, VEL_RGBA16F,  # noqa: E402
+                     VEL_RGBA32F)
+from taastar.workloads import params_c2, workload  # noqa: E402
+
+
+@pytest.fixture(autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def random_params(rng, fast_only=False):
+    """A parameter block; fast_only keeps to the options of the FP16 throughput kernel."""
+    p = params_c2()
+    b = lambda: int(rng.integers(2))
+    p.mAlpha = float(rng.uniform(0.02, 0.3))
+    p.mVarClipGamma = float(rng.uniform(0.5, 1.5))
+    p.mRejectionAlpha = float(rng.uniform(0.5, 1.0))
+    p.mUseYCoCg = b()
+    p.mDepthCulling = b()
+    p.mLumaWeightingLottes = b()
+    p.mRejectOutside = b()
+    p.mColorClampingOrClipping = int(rng.integers(3))
+    p.mVelocitySampleMode = int(rng.choice([0, 2]))
+    p.mUseVelocityVectors = int(rng.choice([1, 2]))
+    if fast_only:
+        p.mUseVelocityVectors = 2
+        return p
+    p.mVarianceClipping = b()
+    p.mShapedNeighbourhood = b()
+    p.mShrinkChromaAxis = b()
+    p.mColorClampingOrClipping = int(rng.integers(4))
+    p.mInterpolationMode = int(rng.integers(3))
+    p.mUseVelocityVectors = int(rng.integers(3))
+    p.mVelocitySampleMode = int(rng.integers(3))
+    p.mReduceBlendNearClamp = b()
+    p.mVelBasedAlpha = b()
+    p.mToneMapLumaKaris = b()
+    p.mDynamicAntiGhosting = b()
+    p.mUnjitterNeighbourhood = b()
+    p.mUnjitterCurrentSample = b()
+    return p
+
+
+def describe(p):
+    keys = ("mVarianceClipping", "mShapedNeighbourhood", "mUseYCoCg", "mShrinkChromaAxis", "mColorClampingOrClipping",
+            "mInterpolationMode", "mUseVelocityVectors", "mVelocitySampleMode", "mDepthCulling", "mLumaWeightingLottes",
+            "mRejectOutside", "mReduceBlendNearClamp", "mVelBasedAlpha", "mToneMapLumaKaris", "mDynamicAntiGhosting",
+            "mUnjitterNeighbourhood", "mUnjitterCurrentSample")
+    return " ".join(f"{k[1:]}={getattr(p, k)}" for k in keys)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_parameter_block_fp32(oracle, seed):
+    rng = np.random.default_rng(0x7AA5F000 + seed)
+    p = random_params(rng)
+    fmt = int(rng.choice([FMT_RGBA32F, FMT_RGBA16F]))
+    vfmt = int(rng.choice([VEL_RG32F, VEL_RG16F, VEL_RGBA32F, VEL_RGBA16F]))
+    cas = bool(rng.integers(2))
+    lin = float(rng.choice([0.0, 0.01]))
+    print(f"case random fp32 seed={seed} fmt={fmt} vfmt={vfmt} cas={cas} linear={lin} {describe(p)}", flush=True)
+    w = workload("c2", fmt=fmt, vel=vfmt, linear_depth=lin, params=lambda p=p: p)
+    seq = Sequence(w, width=96, height=64, cas=cas, seed=0x5EED0000 + seed)
+    for f in range(3):
+        assert_parity(seq.step(oracle))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_parameter_block_fp16(oracle, seed):
+    rng = np.random.default_rng(0x7AA5F100 + seed)
+    p = random_params(rng, fast_only=True)
+    fmt = int(rng.choice([FMT_RGBA32F, FMT_RGBA16F]))
+    vfmt = VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F
+    cas = bool(rng.integers(2))
+    lin = float(rng.choice([0.0, 0.01]))
+    print(f"case random fp16 seed={seed} fmt={fmt} cas={cas} linear={lin} {describe(p)}", flush=True)
+    w = workload("c2", fmt=fmt, vel=vfmt, linear_depth=lin, params=lambda p=p: p)
+    seq = Sequence(w, width=128, height=80, cas=cas, arith=ARITH_FP16, seed=0x5EED1000 + seed)
+    for f in range(3):
+        out = seq.step(oracle)
+        gi, oi = out["gpu_inspect"], out["oracle_inspect"]
+        for wd in INDEX_WORDS:
+            bad = np.count_nonzero(gi[..., wd] != oi[..., wd])
+            assert bad == 0, f"frame {f}: inspect word {wd}: {bad} texels differ"
+        scr_g, scr_o = out["gpu_screen"][..., :3], out["oracle_screen"][..., :3]
+        if cas:
+            assert psnr(scr_g, scr_o) > 55.0, f"frame {f}: screen PSNR {psnr(scr_g, scr_o):.2f} dB"
+        else:  # unsharpened HDR screen: PSNR of the display-referred (clamped) value
+            assert psnr(np.clip(scr_g, 0, 1), np.clip(scr_o, 0, 1)) > 55.0, f"frame {f}"
+        # the FP16 path follows the same history: keep the oracle on the GPU's stored history
+        seq.hist_o = out["gpu_history"]

@@ -110,6 +110,7 @@ VARIANTS = {
     "simple_velocity": dict(mVelocitySampleMode=0),
     "longest_velocity": dict(mVelocitySampleMode=1),
     "antighost_unjitter": dict(mDynamicAntiGhosting=1, mUnjitterNeighbourhood=1, mUnjitterCurrentSample=1),
+    "unjitter_neighbourhood_nearest": dict(mUnjitterNeighbourhood=1),
     "matrix_reproj_raw_depth": dict(mUseVelocityVectors=0),
     "debug_velocity": dict(mDebugMode=4),
 }
@@ -122,7 +123,7 @@ def test_taau_feature_variants(oracle, name):
     for k, v in VARIANTS[name].items():
         setattr(p, k, v)
     w = workload("c2", vel=VEL_RGBA32F)
-    up = Upsampler(w, (80, 48), (160, 96), params=p)
+    up = Upsampler(w, (80, 48), (160, 96), params=p, upsampling=not name.endswith("_nearest"))
     if name == "matrix_reproj_raw_depth":
         up.w = dict(up.w, linear_depth=0.0)
     for _ in range(4):
