@@ -71,7 +71,7 @@ def describe(p):
     return " ".join(f"{k[1:]}={getattr(p, k)}" for k in keys)
 
 
-@pytest.mark.parametrize("seed", range(16))
+@pytest.mark.parametrize("seed", range(48))
 def test_random_parameter_block_fp32(oracle, seed):
     rng = np.random.default_rng(0x7AA5F000 + seed)
     p = random_params(rng)
@@ -110,3 +110,46 @@ def test_random_parameter_block_fp16(oracle, seed):
             assert psnr(np.clip(scr_g, 0, 1), np.clip(scr_o, 0, 1)) > 55.0, f"frame {f}"
         # the FP16 path follows the same history: keep the oracle on the GPU's stored history
         seq.hist_o = out["gpu_history"]
+
+
+UPSCALES = [((96, 54), (192, 108)), ((128, 72), (192, 108)), ((75, 41), (101, 57)), ((120, 68), (120, 68))]
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_parameter_block_upsampling(oracle, seed):
+    """Random blocks on the upsampling path: a random scale, mUpsampling on or off (off: the
+    nearest input texel), either storage format, CAS on or off."""
+    from test_gpu_taau import Upsampler
+    rng = np.random.default_rng(0x7AA5F200 + seed)
+    p = random_params(rng)
+    lo, hi = UPSCALES[int(rng.integers(len(UPSCALES)))]
+    fmt = int(rng.choice([FMT_RGBA32F, FMT_RGBA16F]))
+    vfmt = int(rng.choice([VEL_RG32F, VEL_RGBA32F]))
+    cas = bool(rng.integers(2))
+    upsampling = bool(rng.integers(2)) or lo == hi
+    lin = float(rng.choice([0.0, 0.01]))
+    print(f"case random up seed={seed} {lo}->{hi} mUpsampling={int(upsampling)} fmt={fmt} vfmt={vfmt} cas={cas} "
+          f"linear={lin} {describe(p)}", flush=True)
+    w = workload("c2", fmt=fmt, vel=vfmt, linear_depth=lin)
+    up = Upsampler(w, lo, hi, cas=cas, upsampling=upsampling, params=p)
+    for f in range(3):
+        assert_parity(up.step(oracle))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_split_screen(oracle, seed):
+    """Split screen with two random blocks (taa.comp:717-718) at a random split column."""
+    from taastar.workloads import uniforms_for
+    rng = np.random.default_rng(0x7AA5F300 + seed)
+    p0, p1 = random_params(rng), random_params(rng)
+    split = int(rng.integers(1, 159))
+    print(f"case random split seed={seed} splitX={split}\n  left  {describe(p0)}\n  right {describe(p1)}", flush=True)
+    w = workload("c2", vel=VEL_RGBA32F)
+    seq = Sequence(w, width=160, height=64, cas=False, seed=0x5EED2000 + seed)
+    for f in range(3):
+        col, dep, vel4, jit = seq.scene.render(seq.frame)
+        U = uniforms_for(w, seq.frame, jit, params=p0)
+        U.param[1] = p1
+        U.splitScreen = 1
+        U.splitX = split
+        assert_parity(seq.step(oracle, uniforms=U))
