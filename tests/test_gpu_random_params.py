@@ -153,3 +153,29 @@ def test_random_split_screen(oracle, seed):
         U.splitScreen = 1
         U.splitX = split
         assert_parity(seq.step(oracle, uniforms=U))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_segmentation(oracle, seed):
+    """The resolve with the segmentation mask: a random subset of the indicator flags
+    (shader_cpu_common.h:31-40), random weights, threshold and history count, on a random block."""
+    from taastar import RTFLAG
+    from test_gpu_segmentation import SegSequence, assert_seg_parity
+    rng = np.random.default_rng(0x7AA5F400 + seed)
+    p = random_params(rng)
+    names = ["NRM", "DPT", "MID", "LUM", "OUT", "DIS", "CNT", "FXD", "ALL"]
+    keep = [n for n in names if rng.uniform() < (0.1 if n in ("ALL", "FXD") else 0.5)]
+    p.mRayTraceAugment = 1
+    p.mRayTraceAugmentFlags = int(sum(RTFLAG[n] for n in keep))
+    p.mRayTraceAugment_WNrm, p.mRayTraceAugment_WDpt, p.mRayTraceAugment_WMId, p.mRayTraceAugment_WLum = (
+        float(v) for v in rng.uniform(0.0, 1.0, 4))
+    p.mRayTraceAugment_Thresh = float(rng.uniform(0.1, 1.0))
+    p.mRayTraceHistoryCount = int(rng.integers(0, 10))
+    p.mDebugMode = 8
+    fmt = int(rng.choice([FMT_RGBA32F, FMT_RGBA16F]))
+    cas = bool(rng.integers(2))
+    print(f"case random seg seed={seed} flags={'|'.join(keep) or '0'} fmt={fmt} cas={cas} {describe(p)}", flush=True)
+    w = workload("c2", fmt=fmt, vel=VEL_RGBA32F)
+    seq = SegSequence(w, 160, 96, cas=cas, params=p)
+    for f in range(3):
+        assert_seg_parity(seq.step(oracle))
