@@ -179,3 +179,53 @@ def test_random_segmentation(oracle, seed):
     seq = SegSequence(w, 160, 96, cas=cas, params=p)
     for f in range(3):
         assert_seg_parity(seq.step(oracle))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_velocity_producer(oracle, seed):
+    """The velocity producer on random cameras: a random frame size, camera move and turn, up to
+    five moved objects (and an id beyond the table), any of the four formats; bit-exact."""
+    from taastar.workloads import camera_case
+    from test_gpu_velocity import bits, produce, stored
+    rng = np.random.default_rng(0x7AA5F500 + seed)
+    W, H = int(rng.integers(1, 300)), int(rng.integers(1, 200))
+    vfmt = int(rng.choice([VEL_RG32F, VEL_RG16F, VEL_RGBA32F, VEL_RGBA16F]))
+    n_obj = int(rng.integers(0, 6))
+    move = tuple(float(v) for v in rng.uniform(-3e-3, 3e-3, 3))
+    turn = float(rng.uniform(-2.0, 2.0))
+    pad = int(rng.integers(0, 9))
+    print(f"case random velocity seed={seed} {W}x{H} pad={pad} vfmt={vfmt} objects={n_obj} move={move} turn={turn:.3f}",
+          flush=True)
+    case = camera_case(W, H, seed=0x5EED3000 + seed, n_objects=n_obj, move=move, turn_deg=turn)
+    got = produce(case, vfmt, pad=pad).cpu().numpy()
+    ref = stored(oracle.velocity(case["reproject"], case["depth"], case["ids"], case["objects"]), vfmt)
+    bad = np.count_nonzero(bits(got[:, :W]) != bits(ref))
+    assert bad == 0, f"{bad} words differ"
+    assert np.all(got[:, W:] == 7.0), "row padding written"
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_edge_aa(oracle, seed):
+    """The edge-AA fallback with random constants (subpix, thresholds), a random mask of all
+    categories, random frame size and row padding, either storage; HDR colour above 1 included."""
+    from taastar.workloads import encode_color
+    from test_gpu_edge_aa import bits, consts, run, stored
+    rng = np.random.default_rng(0x7AA5F600 + seed)
+    W, H = int(rng.integers(1, 260)), int(rng.integers(1, 140))
+    fmt = int(rng.choice([FMT_RGBA32F, FMT_RGBA16F]))
+    pad = int(rng.integers(0, 7))
+    sub, thr, thr_min = float(rng.uniform(0, 1)), float(rng.uniform(0, 0.4)), float(rng.uniform(0, 0.1))
+    hdr = float(rng.choice([1.0, 4.0]))
+    print(f"case random edge-aa seed={seed} {W}x{H} pad={pad} fmt={fmt} subpix={sub:.3f} thr={thr:.3f} "
+          f"thr_min={thr_min:.3f} peak={hdr}", flush=True)
+    col = np.concatenate([rng.uniform(0, hdr, (H, W, 3)), np.ones((H, W, 1))], -1).astype(np.float32)
+    # smooth regions with edges between them, so that some texels pass the contrast thresholds and some do not
+    col[..., :3] = np.where(rng.uniform(size=(H, W, 1)) < 0.7, np.round(col[..., :3] * 2) / 2, col[..., :3])
+    src_s = encode_color(col, fmt)
+    mask = rng.integers(0, 3, (H, W)).astype(np.uint32) | (rng.integers(0, 5, (H, W)).astype(np.uint32) << 16)
+    k = consts(W, H, subpix=sub, edge_threshold=thr, edge_threshold_min=thr_min)
+    got = run(src_s, mask, fmt, k, pad=pad)
+    ref = stored(oracle.edge_aa(src_s.astype(np.float32), mask, k.subpix, k.edge_threshold, k.edge_threshold_min), fmt)
+    bad = np.count_nonzero(bits(got[:, :W]) != bits(ref))
+    assert bad == 0, f"{bad} words differ"
+    assert np.all(got[:, W:] == 7.0), "row padding written"
