@@ -48,6 +48,8 @@ def parse():
                     help="after the timed loop, copy MIB MiB device-to-device (calibrates HBM byte counters)")
     ap.add_argument("--velocity-only", action="store_true",
                     help="time only the velocity producer (4K, RG16F, 3 moving objects) and print its line")
+    ap.add_argument("--taau-only", action="store_true",
+                    help="time only the upsampling resolve (1080p and 1440p inputs to 4K) and print its lines")
     return ap.parse_args()
 
 
@@ -534,6 +536,10 @@ def main():
     torch.cuda.set_device(0)
     if a.velocity_only:
         print(json.dumps(dict(kernel="k_velocity", **time_velocity(3840, 2160, "rgba16f", 3, a.steps, a.warmup))))
+        return
+    if a.taau_only:
+        for lo in ((1920, 1080), (2560, 1440)):
+            print(json.dumps(dict(kernel="k_resolve_up", **time_taau(lo, (3840, 2160), a.steps, a.warmup))), flush=True)
         return
     r = time_resolve(a.workload, a.arith, a.steps, a.warmup, a.frames, autotune=a.autotune)
     w = r["w"]
