@@ -370,19 +370,15 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     const float vux = uvx - hu, vuy = uvy - hv;
     const float pixelSpeed = sqrtf(vux * vux + vuy * vuy);
 
-    // the previous depth under the history position (taa.comp:818); with RGBA32F storage it is
-    // fetched here, ahead of the history gather, so that the two loads overlap (measured: faster
-    // for RGBA32F, slower for RGBA16F, whose gather is fewer and wider loads); the test itself
-    // keeps its place below
+    // the previous depth under the history position (taa.comp:818), fetched here, ahead of the
+    // history gather, so that the two loads overlap; the test itself keeps its place below
     int dtx = 0, dty = 0;
     float historyDepth = 0.0f;
-    auto fetch_history_depth = [&]() {
+    if (P.mDepthCulling) {
         dtx = to_int(hu * (float)Wl);  // uHistoryDepth has the input size (taa.comp:818)
         dty = to_int(hv * (float)Hl);
         historyDepth = ld_f32(fd.hdepth, lo(dtx, dty));
-    };
-    if constexpr (!S16)
-        if (P.mDepthCulling) fetch_history_depth();
+    }
 
     // ---- sample_history_rgba (taa.comp:545-549)
     float4 historyRaw;
@@ -434,7 +430,6 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         writeDynamicMask = movC ? 1.0f : 0.0f;
     }
     if (P.mDepthCulling) {
-        if constexpr (S16) fetch_history_depth();
         bool rej;
         if (U.pad1 > 0.0f) {
             float n = U.mCamNearPlane, f = U.mCamFarPlane;
