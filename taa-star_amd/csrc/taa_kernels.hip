@@ -27,11 +27,15 @@ namespace {
 // ------------------------------------------------------------------------------------------
 // tile geometry and LDS
 
-template <int TW_, int TH_, int PAD_, bool CAS_, bool UP_ = false, bool SEG_ = false>
+template <int TW_, int TH_, int PAD_, bool CAS_, bool UP_ = false, bool SEG_ = false, bool VEL16_ = false>
 struct Geo {
     static constexpr int TW = TW_, TH = TH_, PAD = PAD_;
     static constexpr bool CAS = CAS_, UP = UP_;  // UP: upsampling path (input size != output size)
     static constexpr bool SEG = SEG_;            // SEG: the segmentation mask's inputs are staged too
+    // VEL16: an RG16F velocity image is staged as well (its 32-bit texels); measured with RGBA16F
+    // colour at 4K: 30.6 -> 31.4 Gpix/s. The RG32F equivalent ran slower with RGBA32F colour
+    // (24.3 -> 17.4 at 1080p: the larger region costs the schedule its overlapped gathers).
+    static constexpr bool VEL16 = VEL16_;
     static constexpr int NT = 256;
     static constexpr int A = CAS ? 2 : 1;  // input apron
     static constexpr int SW = TW + 2 * A, SH = TH + 2 * A, SP = SW + PAD;
@@ -47,6 +51,7 @@ struct Geo {
         // segmentation inputs of the staged region: unit normal, raw YCoCg luma, material id
         float nx[SEG ? SH * SP : 1], ny[SEG ? SH * SP : 1], nz[SEG ? SH * SP : 1], lum[SEG ? SH * SP : 1];
         uint32_t mat[SEG ? SH * SP : 1];
+        uint32_t vel[VEL16 ? SH * SP : 1];  // RG16F velocity of the staged region
         // upsampling path: converted colour, depth and velocity (as ld_vel reads it) of the staged rectangle
         float4 uvel[UCW * UCH];
         float uc0[UCW * UCH], uc1[UCW * UCH], uc2[UCW * UCH], ud[UCW * UCH];
@@ -165,13 +170,14 @@ __device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, c
 
 // One output texel of taa.comp main (taa.comp:707-960), FP32 arithmetic. `l` indexes the texel
 // in the staged LDS planes (row pitch SP). SEG: the segmentation mask's images are in `sg`.
-template <bool S16, int SP, bool UP, bool SEG = false, bool STG = false>
+template <bool S16, int SP, bool UP, bool SEG = false, bool STG = false, bool VEL16 = false>
 __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
                                          int x, int y, int l, const float* __restrict__ s0,
                                          const float* __restrict__ s1, const float* __restrict__ s2,
                                          const float* __restrict__ sd, float uvx, float uvy, bool write_aux,
                                          const SegDesc* sg = nullptr, const SegPlanes* spl = nullptr,
-                                         const UpIn* up = nullptr, int rx = 0, int ry = 0) {
+                                         const UpIn* up = nullptr, int rx = 0, int ry = 0,
+                                         const uint32_t* __restrict__ svel = nullptr) {
     // uvx, uvy: tc_to_uv(x, y) = ((x, y) + 0.5) / size (taa.comp:131), tabulated per tile
     const int W = fd.W, H = fd.H;
     const uint32_t gi = (uint32_t)(y * fd.pitch + x);
@@ -203,6 +209,17 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     auto vel_at = [&](int xx, int yy) -> float4 {
         if constexpr (STG) return up->v[staged(xx, yy)];
         else return ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(yy * pl + xx));
+    };
+    // velocity texel (x + dx, y + dy) clamped to the image (global index gidx), dx, dy in -1..1:
+    // from the staged RG16F region when the tile staged it (svel), else from global memory
+    auto vel_tap = [&](int dx, int dy, uint32_t gidx) -> float4 {
+        if constexpr (VEL16) {
+            if (svel) {
+                const uint32_t v = svel[l + dy * SP + dx];
+                return make_float4(h2f(v & 0xffffu), h2f(v >> 16), 0.f, 1.f);  // as ld_vel reads RG16F
+            }
+        }
+        return ld_vel(fd.velocity, fd.vel_fmt, gidx);
     };
     // the bilinear taps of a depth or velocity sample at uv + (dx, dy) input texels, dx, dy in -1..1
     // (SAMPLE_TEX at uv + toU * offset), from the tile's tables
@@ -317,7 +334,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         // SAMPLE_TEX(uCurrentVelocity, uv): a texel fetch at one size, bilinear when upsampling
         float4 vs;
         if constexpr (UP) vs = bilinear4_at(up_taps(0, 0), vel_at);
-        else vs = ld_vel(fd.velocity, fd.vel_fmt, gi);
+        else vs = vel_tap(0, 0, gi);
         bool canUseVelocity = !(P.mUseVelocityVectors == 0 || (P.mUseVelocityVectors == 1 && vs.w < 0.5f));
         if (canUseVelocity) {
             flags |= TAA_FLAG_USED_VELOCITY;
@@ -325,7 +342,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
                 // taa.comp:398-412: the longest vector is tracked but the last tap is assigned
                 float4 last;
                 if constexpr (UP) last = bilinear4_at(up_taps(1, 1), vel_at);
-                else last = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1)));
+                else last = vel_tap(1, 1, (uint32_t)(min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1)));
                 vs.x = last.x;
                 vs.y = last.y;
             } else if (P.mVelocitySampleMode == 2) {
@@ -346,7 +363,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
                         if (d < dC) { dC = d; best = k; }
                     }
                     int vx = clampi(x + best % 3 - 1, 0, W - 1), vy = clampi(y + best / 3 - 1, 0, H - 1);
-                    vs = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(vy * fd.pitch + vx));
+                    vs = vel_tap(best % 3 - 1, best / 3 - 1, (uint32_t)(vy * fd.pitch + vx));
                 }
                 closest = best;
             }
@@ -419,7 +436,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         auto mov = [&](int dx, int dy) {
             float4 v;
             if constexpr (UP) v = bilinear4_at(up_taps(dx, dy), vel_at);
-            else v = ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1)));
+            else v = vel_tap(dx, dy, (uint32_t)(clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1)));
             return (fabsf(v.x) > eps || fabsf(v.y) > eps) && (fabsf(v.w) >= 0.5f);
         };
         bool movL = mov(-1, 0), movR = mov(1, 0), movT = mov(0, -1), movB = mov(0, 1), movC = mov(0, 0);
@@ -551,6 +568,7 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
 
     // 1. stage colour (converted once per texel) and depth, clamped to the edge (one size only:
     // the upsampling path reads its inputs where the output texel maps to)
+    const bool vel_staged = G::VEL16 && fd.vel_fmt == TAA_VEL_RG16F;
     for (int i = tid; i < (G::UP ? 0 : G::SW * G::SH); i += G::NT) {
         const int sy = i / G::SW, sx = i - sy * G::SW;
         const int gx = clampi(gx0 + sx, 0, W - 1), gy = clampi(gy0 + sy, 0, H - 1);
@@ -562,6 +580,8 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
         sm.c1[li] = t.y;
         sm.c2[li] = t.z;
         sm.d[li] = ld_f32(fd.depth, gi);
+        if constexpr (G::VEL16)
+            if (vel_staged) sm.vel[li] = gload<uint32_t>(fd.velocity, gi * 4u);
         if constexpr (G::SEG) {  // sample_normal, sample_luminance and the material id (taa.comp:570-579)
             const uint32_t si = (uint32_t)(gy * sg->pitch + gx);
             const float4 q = gload<float4>(sg->uv_normal, si * 16u);
@@ -668,8 +688,9 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
                 r = resolve_pixel_f32<S16, G::SP, true, false, false>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
                                                                       owned, sg, &spl, &ui, rx, ry);
         } else {
-            r = resolve_pixel_f32<S16, G::SP, false, SEG>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy, owned, sg,
-                                                          &spl);
+            r = resolve_pixel_f32<S16, G::SP, false, SEG, false, G::VEL16>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
+                                                                            uvx, uvy, owned, sg, &spl, nullptr, 0, 0,
+                                                                            vel_staged ? sm.vel : nullptr);
         }
         if (owned) {
             const uint32_t gi = (uint32_t)(cy * pitch + cx);
@@ -710,7 +731,7 @@ struct ResolveArgs {
 
 template <int TW, int TH, int PAD, bool CAS, bool S16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3))) void k_resolve(const ResolveArgs a) {
-    using G = Geo<TW, TH, PAD, CAS>;
+    using G = Geo<TW, TH, PAD, CAS, false, false, S16>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
     resolve_tile<G, S16>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
