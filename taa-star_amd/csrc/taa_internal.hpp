@@ -65,7 +65,7 @@ constexpr int kDefaultShapePlain = 0;
 constexpr int kDefaultShapeCas = 2;
 // tile shapes of the resolve with the segmentation mask: its LDS also holds the mask's inputs,
 // so the small 14x14 CAS tile keeps more workgroups resident (1080p FP32 with the reference's
-// default flags: 116 us against 127 us for 30x14 and 155 us for 30x30)
+// default flags: 102 us against 105 us for 30x14 and 151 us for 30x30)
 constexpr int kSegShapeCas = 0;
 constexpr int kSegShapePlain = kDefaultShapePlain;
 // tile shapes of the upsampling resolve (its LDS holds the staged input rectangle)
