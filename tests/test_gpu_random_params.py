@@ -229,3 +229,85 @@ def test_random_edge_aa(oracle, seed):
     bad = np.count_nonzero(bits(got[:, :W]) != bits(ref))
     assert bad == 0, f"{bad} words differ"
     assert np.all(got[:, W:] == 7.0), "row padding written"
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_batch_equals_alone(seed):
+    """A random batch -- 2 to 6 sessions of random sizes (some scaled: input below the output),
+    one random parameter block per session, CAS on or off for all -- submitted and through a
+    graph-captured plan over three frames: every screen bit-identical to its session resolved
+    alone. FP32 arithmetic with any block, or FP16 with blocks of the throughput kernel."""
+    import ctypes as C
+    from taastar import ARITH_FP32, BatchItem, SessionIO, Uniforms, check, lib
+    from taastar.workloads import cas_for, encode_color, encode_velocity, scene_for, uniforms_for
+    from test_gpu_serve import h2d
+    rng = np.random.default_rng(0x7AA5F700 + seed)
+    fmt = int(rng.choice([FMT_RGBA32F, FMT_RGBA16F]))
+    vfmt = VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F
+    arith = int(rng.choice([ARITH_FP32, ARITH_FP16]))
+    cas = bool(rng.integers(2))
+    n = int(rng.integers(2, 7))
+    specs = []
+    for k in range(n):
+        W, H = int(rng.integers(8, 200)), int(rng.integers(8, 120))
+        lo = (W, H) if rng.uniform() < 0.7 else (max(4, W * 2 // 3), max(4, H * 2 // 3))
+        specs.append((lo, (W, H), random_params(rng, fast_only=arith == ARITH_FP16)))
+    print(f"case random batch seed={seed} fmt={fmt} arith={arith} cas={cas} sizes={[(s[0], s[1]) for s in specs]}",
+          flush=True)
+    w = workload("c2", fmt=fmt, vel=vfmt)
+    scenes = [scene_for(w, width=lo[0], height=lo[1], seed=0x5EED4000 + 97 * seed + k) for k, (lo, _, _) in enumerate(specs)]
+    elt = 8 if fmt == FMT_RGBA16F else 16
+
+    def run(mode):
+        ctx = C.c_void_p()
+        check(lib().taa_ctx_create(0, 0, C.byref(ctx)))
+        sess, scrs, Ks = [], [], []
+        for lo, hi, _ in specs:
+            s = C.c_void_p()
+            check(lib().taa_session_open_scaled(ctx, hi[0], hi[1], lo[0], lo[1], fmt, vfmt, C.byref(s)))
+            sess.append(s)
+            scrs.append(torch.zeros(hi[1] * hi[0] * elt, dtype=torch.uint8, device="cuda"))
+            Ks.append(cas_for(w, hi[0], hi[1]) if cas else None)
+        Us = [Uniforms() for _ in specs]
+        items = (BatchItem * n)()
+        for k in range(n):
+            items[k].session, items[k].uniforms = sess[k], C.pointer(Us[k])
+            items[k].cas = C.pointer(Ks[k]) if cas else None
+            items[k].result_screen = scrs[k].data_ptr()
+        plan = C.c_void_p()
+        if mode == "plan":
+            check(lib().taa_batch_plan_create(ctx, n, items, arith, C.byref(plan)))
+        outs = []
+        for f in range(3):
+            for k, (lo, hi, p) in enumerate(specs):
+                io = SessionIO()
+                check(lib().taa_session_io(sess[k], C.byref(io)))
+                col, dep, vel4, jit = scenes[k].render(f)
+                h2d(io.color, encode_color(col, fmt))
+                h2d(io.depth, dep)
+                h2d(io.velocity, encode_velocity(vel4, vfmt))
+                U = uniforms_for(w, f, jit, params=p)
+                C.memmove(C.addressof(Us[k]), C.addressof(U), C.sizeof(Uniforms))
+            if mode == "batch":
+                check(lib().taa_batch_submit(ctx, n, items, arith, None))
+            elif mode == "plan":
+                check(lib().taa_batch_plan_launch(plan, None))
+            else:
+                for k in range(n):
+                    check(lib().taa_session_resolve(sess[k], C.byref(Us[k]), C.byref(Ks[k]) if cas else None,
+                                                    C.c_void_p(scrs[k].data_ptr()), arith, None))
+            torch.cuda.synchronize()
+            outs.append([t.cpu().numpy().copy() for t in scrs])
+        if mode == "plan":
+            check(lib().taa_batch_plan_destroy(plan))
+        for s in sess:
+            check(lib().taa_session_close(s))
+        check(lib().taa_ctx_destroy(ctx))
+        return outs
+
+    alone = run("alone")
+    for mode in ("batch", "plan"):
+        got = run(mode)
+        for f in range(3):
+            for k in range(n):
+                assert np.array_equal(alone[f][k], got[f][k]), f"{mode}: frame {f} session {k}"
