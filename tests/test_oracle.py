@@ -378,6 +378,38 @@ def test_clip_lands_on_the_box(oracle):
     assert np.array_equal(o["history"][..., :3], np.clip(hist[..., :3], mn, mx))
 
 
+def test_current_sample_unjitter(oracle):
+    """getCurrentColor (taa.comp:216-220) samples the texel itself unless mUnjitterCurrentSample is
+    set, whatever mUnjitterNeighbourhood says; with it, a bilinear clamp-to-edge sample at
+    (p + 0.5) / size + jitter * 0.5 * mUnjitterFactor. On a reset frame the screen is that sample."""
+    rng = np.random.default_rng(29)
+    H, W = 9, 12
+    col, dep, vel, hist = frame(H, W, rng)
+    jit = (0.6 / W, -0.4 / H)
+    for unj_nb, unj_cur in ((1, 0), (0, 0), (1, 1), (0, 1)):
+        U = uniforms(params(mUnjitterNeighbourhood=unj_nb, mUnjitterCurrentSample=unj_cur, mUnjitterFactor=1.0), reset=1)
+        U.mJitterNdc[0], U.mJitterNdc[1] = jit
+        o = run(oracle, U, col, dep, vel, hist)
+        if not unj_cur:
+            assert np.array_equal(o["screen"][..., :3], col[..., :3].astype(np.float64)), (unj_nb, unj_cur)
+            continue
+        # numpy bilinear, clamp to edge, at the unjittered texel centres
+        c = col[..., :3].astype(np.float64)
+        ju, jv = np.float64(f32(jit[0])) * 0.5, np.float64(f32(jit[1])) * 0.5
+        px = (np.arange(W) + 0.5) / W + ju
+        py = (np.arange(H) + 0.5) / H + jv
+        fx, fy = px * W - 0.5, py * H - 0.5
+        x0, y0 = np.floor(fx).astype(int), np.floor(fy).astype(int)
+        tx, ty = fx - x0, fy - y0
+        xa, xb = np.clip(x0, 0, W - 1), np.clip(x0 + 1, 0, W - 1)
+        ya, yb = np.clip(y0, 0, H - 1), np.clip(y0 + 1, 0, H - 1)
+        top = c[ya][:, xa] * (1 - tx)[None, :, None] + c[ya][:, xb] * tx[None, :, None]
+        bot = c[yb][:, xa] * (1 - tx)[None, :, None] + c[yb][:, xb] * tx[None, :, None]
+        ref = top * (1 - ty)[:, None, None] + bot * ty[:, None, None]
+        assert np.abs(o["screen"][..., :3] - ref).max() < 1e-9, (unj_nb, unj_cur)
+        assert not np.allclose(ref, c)  # the jitter really moved the sample
+
+
 def test_blend_closed_forms(oracle):
     rng = np.random.default_rng(13)
     H, W = 10, 13
