@@ -115,7 +115,10 @@ __device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, in
     constexpr uint32_t TB = S16 ? 8u : 16u;  // bytes per texel
     const W4 wx = cr_weights(fx), wy = cr_weights(fy);
     f3 acc = {0.f, 0.f, 0.f};
-    const bool interior = kx >= 1 && kx + 2 <= W - 1 && ky >= 1 && ky + 2 <= H - 1;
+    // the whole 4x4 footprint inside the image. kx, ky come unclamped from to_int(floor(...)) and
+    // saturate at INT_MIN / INT_MAX for huge or infinite history positions, so the test is written
+    // without forming kx + 2 (which would wrap and pass for kx = INT_MAX)
+    const bool interior = kx >= 1 && kx <= W - 3 && ky >= 1 && ky <= H - 3;
     if (interior) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {

@@ -45,7 +45,8 @@ BOX_WORDS = list(range(INSPECT["COLMIN"], INSPECT["COLMIN"] + 3)) + list(range(I
 class Sequence:
     """Frames of one seeded scene, resolved on the GPU and by the FP32 oracle side by side."""
 
-    def __init__(self, w, width=None, height=None, cas=True, arith=0, params=None, seed=None, pan_mode=0, pan=8.0):
+    def __init__(self, w, width=None, height=None, cas=True, arith=0, params=None, seed=None, pan_mode=0, pan=8.0,
+                 debug=True):
         import torch
         from taastar.device import FrameBuffers
         self.w = w
@@ -56,7 +57,9 @@ class Sequence:
         self.cas = cas_for(w, self.W, self.H) if cas else None
         self.arith = arith
         self.params = params
-        self.bufs = FrameBuffers(self.W, self.H, self.fmt, self.vfmt, debug=True, inspect=True)
+        # a bound debug image sends an FP16 request to the exact kernel (the throughput kernel
+        # writes no debug image): debug=False keeps the FP16 kernel on the FP16 path
+        self.bufs = FrameBuffers(self.W, self.H, self.fmt, self.vfmt, debug=debug, inspect=True)
         self.hist_o = np.zeros((self.H, self.W, 4), np.float32)
         self.prev_depth = np.zeros((self.H, self.W), np.float32)
         self.frame = 0
@@ -73,7 +76,7 @@ class Sequence:
         g_hist = decode(self.bufs.hist_in.cpu().numpy())  # hist_in now points at the frame just written
         g_scr = decode(self.bufs.screen.cpu().numpy())
         g_ins = self.bufs.inspect.cpu().numpy()
-        g_dbg = self.bufs.debug.cpu().numpy().astype(np.float32)
+        g_dbg = self.bufs.debug.cpu().numpy().astype(np.float32) if self.bufs.debug is not None else None
         out = dict(gpu_history=g_hist, gpu_screen=g_scr, gpu_inspect=g_ins, gpu_debug=g_dbg, uniforms=U)
         if run_oracle:
             o = oracle.resolve(U, decode(col_s), dep, velocity_as_read(vel_s), self.hist_o, self.prev_depth)

@@ -53,7 +53,8 @@ def test_small_and_boundary_sizes_fp32(oracle, W, H, cas):
 @pytest.mark.parametrize("W,H", SIZES)
 def test_small_and_boundary_sizes_fp16(oracle, W, H, cas):
     print(f"case fp16 {W}x{H} cas={cas}", flush=True)
-    seq = Sequence(workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F), width=W, height=H, cas=cas, arith=ARITH_FP16)
+    seq = Sequence(workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F), width=W, height=H, cas=cas, arith=ARITH_FP16,
+                   debug=False)
     for _ in range(3):
         out = seq.step(oracle)
         assert_fp16_indices(out)
@@ -152,7 +153,8 @@ def test_hostile_velocity_fp16(oracle):
     """The FP16 path on NaN / Inf / huge velocities and NaN depths: texel indices, closest taps and
     rejection decisions as the transcription's."""
     print("case hostile fp16", flush=True)
-    seq = Sequence(workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F), width=123, height=77, cas=True, arith=ARITH_FP16)
+    seq = Sequence(workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F), width=123, height=77, cas=True, arith=ARITH_FP16,
+                   debug=False)
     rng = np.random.default_rng(6)
     render = seq.scene.render
 

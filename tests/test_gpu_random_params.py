@@ -19,8 +19,9 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 from gpu_helpers import INDEX_WORDS, Sequence, assert_parity, psnr  # noqa: E402
-from taastar import (ARITH_FP16, FMT_RGBA16F, FMT_RGBA32F, VEL_RG16F, VEL_RG32F, VEL_RGBA16F,  # noqa: E402
+from taastar import (ARITH_FP16, FMT_RGBA16F, FMT_RGBA32F, INSPECT, VEL_RG16F, VEL_RG32F, VEL_RGBA16F,  # noqa: E402
                      VEL_RGBA32F)
+from taastar import FLAG_RECTIFIED  # noqa: E402
 from taastar.workloads import params_c2, workload  # noqa: E402
 
 
@@ -96,12 +97,15 @@ def test_random_parameter_block_fp16(oracle, seed):
     lin = float(rng.choice([0.0, 0.01]))
     print(f"case random fp16 seed={seed} fmt={fmt} cas={cas} linear={lin} {describe(p)}", flush=True)
     w = workload("c2", fmt=fmt, vel=vfmt, linear_depth=lin, params=lambda p=p: p)
-    seq = Sequence(w, width=128, height=80, cas=cas, arith=ARITH_FP16, seed=0x5EED1000 + seed)
+    seq = Sequence(w, width=128, height=80, cas=cas, arith=ARITH_FP16, seed=0x5EED1000 + seed, debug=False)
     for f in range(3):
         out = seq.step(oracle)
         gi, oi = out["gpu_inspect"], out["oracle_inspect"]
         for wd in INDEX_WORDS:
-            bad = np.count_nonzero(gi[..., wd] != oi[..., wd])
+            g, o = gi[..., wd], oi[..., wd]
+            if wd == INSPECT["FLAGS"]:  # "rectified" is a threshold on arithmetic, not a selection
+                g, o = g & ~FLAG_RECTIFIED, o & ~FLAG_RECTIFIED
+            bad = np.count_nonzero(g != o)
             assert bad == 0, f"frame {f}: inspect word {wd}: {bad} texels differ"
         scr_g, scr_o = out["gpu_screen"][..., :3], out["oracle_screen"][..., :3]
         if cas:
