@@ -315,3 +315,41 @@ def test_random_batch_equals_alone(seed):
         for f in range(3):
             for k in range(n):
                 assert np.array_equal(alone[f][k], got[f][k]), f"{mode}: frame {f} session {k}"
+
+
+@pytest.mark.filterwarnings("ignore:overflow encountered in cast:RuntimeWarning")  # huge velocities stored as fp16
+@pytest.mark.parametrize("seed", range(12))
+def test_random_fp16_sizes_and_hostile_inputs(oracle, seed):
+    """The FP16 kernel at random frame sizes (1 to 150 texels a side) with scattered NaN / Inf /
+    huge velocities and NaN depths: texel indices, closest taps and rejection decisions as the FP32
+    transcription's (int(x) saturates, NaN converts to 0, every fetch clamped to the edge)."""
+    from test_gpu_edges import hostile_velocity
+    rng = np.random.default_rng(0x7AA5F800 + seed)
+    p = random_params(rng, fast_only=True)
+    fmt = int(rng.choice([FMT_RGBA32F, FMT_RGBA16F]))
+    vfmt = VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F
+    W, H = int(rng.integers(1, 151)), int(rng.integers(1, 151))
+    cas = bool(rng.integers(2))
+    print(f"case random fp16 hostile seed={seed} {W}x{H} fmt={fmt} cas={cas} {describe(p)}", flush=True)
+    w = workload("c2", fmt=fmt, vel=vfmt, params=lambda p=p: p)
+    seq = Sequence(w, width=W, height=H, cas=cas, arith=ARITH_FP16, seed=0x5EED5000 + seed, debug=False)
+    render = seq.scene.render
+
+    def hostile(frame, threads=0):
+        c, d, v4, jit = render(frame, threads)
+        d = d.copy()
+        n = max(1, W * H // 40)
+        d[rng.integers(0, H, n), rng.integers(0, W, n)] = np.nan
+        return c, d, hostile_velocity(v4, rng), jit
+
+    seq.scene.render = hostile
+    for f in range(3):
+        out = seq.step(oracle)
+        gi, oi = out["gpu_inspect"], out["oracle_inspect"]
+        for wd in INDEX_WORDS:
+            g, o = gi[..., wd], oi[..., wd]
+            if wd == INSPECT["FLAGS"]:
+                g, o = g & ~FLAG_RECTIFIED, o & ~FLAG_RECTIFIED
+            bad = np.count_nonzero(g != o)
+            assert bad == 0, f"frame {f}: inspect word {wd}: {bad} texels differ"
+        seq.hist_o = out["gpu_history"]
