@@ -53,8 +53,13 @@ enum { TAA_VEL_RG32F = 0, TAA_VEL_RG16F = 1, TAA_VEL_RGBA32F = 2, TAA_VEL_RGBA16
 /* arithmetic of the resolve.
  *   FP32: IEEE fp32 in the operation order of taa.comp (no contraction, correctly rounded
  *         division and square root) -- the parity path.
- *   FP16: colour and history filtering in packed fp16, variance moments and the blend in fp32,
- *         hardware reciprocal / square root -- the throughput path. */
+ *   FP16: fp16 texels entering the history and CAS filters through mixed fp16 x fp32 FMAs (the
+ *         CAS minimum / maximum in packed fp16), moments, rectification and the blend in fp32,
+ *         hardware reciprocal / square root; texel indices and rejection decisions as in FP32 --
+ *         the throughput path. It covers the variance or plain min/max box, Catmull-Rom or
+ *         bilinear history, simple or closest-depth velocity, clamp or clip, reject-outside, raw
+ *         or linear depth test, Lottes weighting and fused CAS, with RG velocity and no debug
+ *         image; any other block is resolved by the FP32 path. */
 enum { TAA_ARITH_FP32 = 0, TAA_ARITH_FP16 = 1 };
 
 /* ---- reference structs ------------------------------------------------------------------- */

@@ -173,6 +173,21 @@ __device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, in
     return acc;
 }
 
+// Bilinear history (sample_history_rgba with mInterpolationMode 0, taa.comp:545-549): the 2x2 texels
+// at (kx, ky) .. (kx + 1, ky + 1), clamped to the edge, weights (1 - f, f); fp32 accumulation.
+template <bool S16>
+__device__ __forceinline__ f3 history_bilinear_fast(const void* hist, int W, int H, int pitch, int kx, int ky, float fx, float fy) {
+    constexpr uint32_t TB = S16 ? 8u : 16u;
+    const uint32_t xa = (uint32_t)clampi(kx, 0, W - 1), xb = (uint32_t)clampi(kx, -1, W - 2) + 1u;
+    const uint32_t ya = (uint32_t)clampi(ky, 0, H - 1) * (uint32_t)pitch, yb = ((uint32_t)clampi(ky, -1, H - 2) + 1u) * (uint32_t)pitch;
+    const Texel<S16> a = load_texel<S16>(hist, (ya + xa) * TB, 0), b = load_texel<S16>(hist, (ya + xb) * TB, 0);
+    const Texel<S16> c = load_texel<S16>(hist, (yb + xa) * TB, 0), d = load_texel<S16>(hist, (yb + xb) * TB, 0);
+    const float ux = 1.0f - fx, uy = 1.0f - fy;
+    const f3 top = {__builtin_fmaf(b.r, fx, a.r * ux), __builtin_fmaf(b.g, fx, a.g * ux), __builtin_fmaf(b.b, fx, a.b * ux)};
+    const f3 bot = {__builtin_fmaf(d.r, fx, c.r * ux), __builtin_fmaf(d.g, fx, c.g * ux), __builtin_fmaf(d.b, fx, c.b * ux)};
+    return {__builtin_fmaf(bot.x, fy, top.x * uy), __builtin_fmaf(bot.y, fy, top.y * uy), __builtin_fmaf(bot.z, fy, top.z * uy)};
+}
+
 // CAS (ffx_cas.h:424-537) with contraction: the same filter as cas_filter(), the four-neighbour
 // sum factored as (b + d + f + h) * w + e.
 __device__ __forceinline__ f3 cas_filter_fma(f3 b, f3 d, f3 e, f3 f, f3 h, float peak) {
@@ -224,7 +239,11 @@ __device__ __forceinline__ uint2 cas_filter_h2(uint2 b, uint2 d, uint2 e, uint2 
     return make_uint2(pack_unit_f2h(r, g), pack_unit_f2h(bl, 1.0f));
 }
 
-template <class G, bool INSP>
+// GEN: the general configuration (variance or plain min/max box, Catmull-Rom or bilinear history,
+// chosen per launch from the parameter block); without it the kernel is the variance box with
+// Catmull-Rom history and no branch on either (the production configuration keeps the schedule
+// measured for it: the general form runs it 15% slower)
+template <class G, bool INSP, bool GEN>
 __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm) {
     constexpr bool S16 = G::S16;
     const taa_params_t& P = U.param[fd.param_idx];
@@ -244,6 +263,8 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
     const bool lin_depth = U.pad1 > 0.0f;
     const bool reset = U.mResetHistory != 0;
     const float gamma = P.mVarClipGamma;
+    const bool var_clip = !GEN || P.mVarianceClipping != 0;   // else the plain min/max box of the nine taps
+    const bool cr_hist = !GEN || P.mInterpolationMode == 2;  // else bilinear history
     const float fW = (float)W, fH = (float)H;
     typedef typename G::VelT VelT;
 
@@ -353,7 +374,8 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         const float tX = hu * fW - 0.5f, tY = hv * fH - 0.5f;
         const float fbx = floorf(tX), fby = floorf(tY);
         const int kx = to_int(fbx), ky = to_int(fby);
-        const f3 hrgb = history_cr_fast<S16>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby);
+        const f3 hrgb = cr_hist ? history_cr_fast<S16>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby)
+                                : history_bilinear_fast<S16>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby);
         __builtin_amdgcn_s_setprio(0);
         f3 h = hrgb;
         if (ycocg) {
@@ -382,20 +404,33 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
             }
         }
 
-        // ---- rectification: clamp to / clip towards the variance box (centre = mean)
-        const f3 mn = {__builtin_fmaf(-gamma, sig.x, mean.x), __builtin_fmaf(-gamma, sig.y, mean.y), __builtin_fmaf(-gamma, sig.z, mean.z)};
-        const f3 mx = {__builtin_fmaf(gamma, sig.x, mean.x), __builtin_fmaf(gamma, sig.y, mean.y), __builtin_fmaf(gamma, sig.z, mean.z)};
+        // ---- rectification: clamp to / clip towards the variance box (centre = mean) or the plain
+        // min/max box of the nine taps (taa.comp:288-292, centre = its midpoint)
+        f3 mn = {__builtin_fmaf(-gamma, sig.x, mean.x), __builtin_fmaf(-gamma, sig.y, mean.y), __builtin_fmaf(-gamma, sig.z, mean.z)};
+        f3 mx = {__builtin_fmaf(gamma, sig.x, mean.x), __builtin_fmaf(gamma, sig.y, mean.y), __builtin_fmaf(gamma, sig.z, mean.z)};
+        if (!var_clip) {
+            mn = cur;
+            mx = cur;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const float4 t = scd[l + (k / 3 - 1) * G::SP + (k % 3 - 1)];
+                mn = vmin(mn, f3{t.x, t.y, t.z});
+                mx = vmax(mx, f3{t.x, t.y, t.z});
+            }
+        }
         if (rect == 2) {
             // clipAabb (taa.comp:323-345): ma = max_i |v_i| / e_i, clip when ma > 1, with one
             // reciprocal: ma = M / D for M = max_i |v_i| * prod_{j != i} e_j and D = e_x e_y e_z
-            const f3 vc = h - mean;
-            const float ex = __builtin_fmaf(gamma, sig.x, 1e-7f), ey = __builtin_fmaf(gamma, sig.y, 1e-7f),
-                        ez = __builtin_fmaf(gamma, sig.z, 1e-7f);
+            const f3 bc = var_clip ? mean : f3{0.5f * (mx.x + mn.x), 0.5f * (mx.y + mn.y), 0.5f * (mx.z + mn.z)};
+            const f3 vc = h - bc;
+            const float ex = var_clip ? __builtin_fmaf(gamma, sig.x, 1e-7f) : __builtin_fmaf(0.5f, mx.x - mn.x, 1e-7f);
+            const float ey = var_clip ? __builtin_fmaf(gamma, sig.y, 1e-7f) : __builtin_fmaf(0.5f, mx.y - mn.y, 1e-7f);
+            const float ez = var_clip ? __builtin_fmaf(gamma, sig.z, 1e-7f) : __builtin_fmaf(0.5f, mx.z - mn.z, 1e-7f);
             const float exy = ex * ey, D = exy * ez;
             const float M = fmaxf(fabsf(vc.x) * (ey * ez), fmaxf(fabsf(vc.y) * (ex * ez), fabsf(vc.z) * exy));
             if (M > D) {
                 const float s = D * rcp_fast(M);
-                h = {__builtin_fmaf(vc.x, s, mean.x), __builtin_fmaf(vc.y, s, mean.y), __builtin_fmaf(vc.z, s, mean.z)};
+                h = {__builtin_fmaf(vc.x, s, bc.x), __builtin_fmaf(vc.y, s, bc.y), __builtin_fmaf(vc.z, s, bc.z)};
             }
         } else if (rect == 1) {
             h = vmin(vmax(h, mn), mx);
@@ -500,15 +535,15 @@ struct FastArgs {
     taa_uniforms_t U;
 };
 
-template <int TW, int TH, bool CAS, bool S16, bool INSP>
+template <int TW, int TH, bool CAS, bool S16, bool INSP, bool GEN>
 __global__ __launch_bounds__(256) void k_resolve_fast(const FastArgs a) {
     using G = FGeo<TW, TH, CAS, S16>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
-    fast_tile<G, INSP>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
+    fast_tile<G, INSP, GEN>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
 }
 
-template <int TW, int TH, bool CAS, bool S16>
+template <int TW, int TH, bool CAS, bool S16, bool GEN>
 __global__ __launch_bounds__(256) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = FGeo<TW, TH, CAS, S16>;
     __shared__ typename G::Smem sm;
@@ -518,12 +553,12 @@ __global__ __launch_bounds__(256) void k_resolve_fast_batch(const BatchEntry* __
         const int t = xcd * per + j;
         if (t >= total_tiles) break;
         const BatchEntry& e = batch_entry(entries, n_entries, t);
-        fast_tile<G, false>(e.fd, e.U, t - e.tile_begin, sm);
+        fast_tile<G, false, GEN>(e.fd, e.U, t - e.tile_begin, sm);
         __syncthreads();
     }
 }
 
-template <bool CAS, bool S16, bool INSP>
+template <bool CAS, bool S16, bool INSP, bool GEN>
 hipError_t launch_fast_t(const FrameDesc& fd, const taa_uniforms_t& U, int shape, hipStream_t st) {
     FastArgs a;
     a.fd = fd;
@@ -531,22 +566,22 @@ hipError_t launch_fast_t(const FrameDesc& fd, const taa_uniforms_t& U, int shape
     const int n = fd.tiles_x * fd.tiles_y;
     constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
     switch (shape) {
-        case 0: hipLaunchKernelGGL((k_resolve_fast<S[0].tw, S[0].th, CAS, S16, INSP>), dim3(n), dim3(256), 0, st, a); break;
-        case 1: hipLaunchKernelGGL((k_resolve_fast<S[1].tw, S[1].th, CAS, S16, INSP>), dim3(n), dim3(256), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((k_resolve_fast<S[2].tw, S[2].th, CAS, S16, INSP>), dim3(n), dim3(256), 0, st, a); break;
-        default: hipLaunchKernelGGL((k_resolve_fast<S[3].tw, S[3].th, CAS, S16, INSP>), dim3(n), dim3(256), 0, st, a); break;
+        case 0: hipLaunchKernelGGL((k_resolve_fast<S[0].tw, S[0].th, CAS, S16, INSP, GEN>), dim3(n), dim3(256), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_resolve_fast<S[1].tw, S[1].th, CAS, S16, INSP, GEN>), dim3(n), dim3(256), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_resolve_fast<S[2].tw, S[2].th, CAS, S16, INSP, GEN>), dim3(n), dim3(256), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_resolve_fast<S[3].tw, S[3].th, CAS, S16, INSP, GEN>), dim3(n), dim3(256), 0, st, a); break;
     }
     return hipGetLastError();
 }
 
-template <bool CAS, bool S16>
+template <bool CAS, bool S16, bool GEN>
 hipError_t launch_fast_batch_t(const BatchEntry* e, int n, int total, int shape, int grid, hipStream_t st) {
     constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
     switch (shape) {
-        case 0: hipLaunchKernelGGL((k_resolve_fast_batch<S[0].tw, S[0].th, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
-        case 1: hipLaunchKernelGGL((k_resolve_fast_batch<S[1].tw, S[1].th, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
-        case 2: hipLaunchKernelGGL((k_resolve_fast_batch<S[2].tw, S[2].th, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
-        default: hipLaunchKernelGGL((k_resolve_fast_batch<S[3].tw, S[3].th, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 0: hipLaunchKernelGGL((k_resolve_fast_batch<S[0].tw, S[0].th, CAS, S16, GEN>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 1: hipLaunchKernelGGL((k_resolve_fast_batch<S[1].tw, S[1].th, CAS, S16, GEN>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 2: hipLaunchKernelGGL((k_resolve_fast_batch<S[2].tw, S[2].th, CAS, S16, GEN>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        default: hipLaunchKernelGGL((k_resolve_fast_batch<S[3].tw, S[3].th, CAS, S16, GEN>), dim3(grid), dim3(256), 0, st, e, n, total); break;
     }
     return hipGetLastError();
 }
@@ -560,13 +595,24 @@ bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param
     if (U.mBypassHistoryUpdate || P.mPassThrough || P.mToneMapLumaKaris || P.mUnjitterNeighbourhood ||
         P.mUnjitterCurrentSample)
         return false;
-    if (!P.mVarianceClipping || P.mShrinkChromaAxis) return false;
+    if ((!P.mVarianceClipping && P.mShapedNeighbourhood) || P.mShrinkChromaAxis) return false;
     if (P.mColorClampingOrClipping < 0 || P.mColorClampingOrClipping > 2) return false;
-    if (P.mInterpolationMode != 2) return false;
+    if (P.mInterpolationMode != 2 && P.mInterpolationMode != 0) return false;
     if (P.mUseVelocityVectors == 0) return false;  // matrix reprojection
     if (P.mVelocitySampleMode != 0 && P.mVelocitySampleMode != 2) return false;
     if (P.mDynamicAntiGhosting || P.mVelBasedAlpha || P.mReduceBlendNearClamp || P.mAddNoise) return false;
     return true;
+}
+
+bool fast_path_general(const taa_uniforms_t& U, int param_idx) {
+    const taa_params_t& P = U.param[param_idx];
+    return !P.mVarianceClipping || P.mInterpolationMode != 2;
+}
+
+template <bool CAS, bool S16, bool INSP>
+hipError_t launch_fast_g(const FrameDesc& fd, const taa_uniforms_t& U, int shape, hipStream_t st) {
+    return fast_path_general(U, fd.param_idx) ? launch_fast_t<CAS, S16, INSP, true>(fd, U, shape, st)
+                                              : launch_fast_t<CAS, S16, INSP, false>(fd, U, shape, st);
 }
 
 int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape, void* stream) {
@@ -574,23 +620,30 @@ int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape,
     const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F, cas = fd.cas_on != 0, insp = fd.inspect != nullptr;
     hipError_t e;
     if (insp) {
-        if (cas) e = s16 ? launch_fast_t<true, true, true>(fd, U, shape, st) : launch_fast_t<true, false, true>(fd, U, shape, st);
-        else e = s16 ? launch_fast_t<false, true, true>(fd, U, shape, st) : launch_fast_t<false, false, true>(fd, U, shape, st);
+        if (cas) e = s16 ? launch_fast_g<true, true, true>(fd, U, shape, st) : launch_fast_g<true, false, true>(fd, U, shape, st);
+        else e = s16 ? launch_fast_g<false, true, true>(fd, U, shape, st) : launch_fast_g<false, false, true>(fd, U, shape, st);
     } else {
-        if (cas) e = s16 ? launch_fast_t<true, true, false>(fd, U, shape, st) : launch_fast_t<true, false, false>(fd, U, shape, st);
-        else e = s16 ? launch_fast_t<false, true, false>(fd, U, shape, st) : launch_fast_t<false, false, false>(fd, U, shape, st);
+        if (cas) e = s16 ? launch_fast_g<true, true, false>(fd, U, shape, st) : launch_fast_g<true, false, false>(fd, U, shape, st);
+        else e = s16 ? launch_fast_g<false, true, false>(fd, U, shape, st) : launch_fast_g<false, false, false>(fd, U, shape, st);
     }
     return (int)e;
 }
 
-int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, int grid,
-                      void* stream) {
+int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, bool general,
+                      int grid, void* stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e;
-    if (cas) e = s16 ? launch_fast_batch_t<true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
-                     : launch_fast_batch_t<true, false>(d_entries, n_entries, total_tiles, shape, grid, st);
-    else e = s16 ? launch_fast_batch_t<false, true>(d_entries, n_entries, total_tiles, shape, grid, st)
-                 : launch_fast_batch_t<false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+    if (general) {
+        if (cas) e = s16 ? launch_fast_batch_t<true, true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
+                         : launch_fast_batch_t<true, false, true>(d_entries, n_entries, total_tiles, shape, grid, st);
+        else e = s16 ? launch_fast_batch_t<false, true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
+                     : launch_fast_batch_t<false, false, true>(d_entries, n_entries, total_tiles, shape, grid, st);
+    } else {
+        if (cas) e = s16 ? launch_fast_batch_t<true, true, false>(d_entries, n_entries, total_tiles, shape, grid, st)
+                         : launch_fast_batch_t<true, false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+        else e = s16 ? launch_fast_batch_t<false, true, false>(d_entries, n_entries, total_tiles, shape, grid, st)
+                     : launch_fast_batch_t<false, false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+    }
     return (int)e;
 }
 

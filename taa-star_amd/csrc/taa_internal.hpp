@@ -97,9 +97,11 @@ int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles,
 // with RG velocity, fp16-fed mixed arithmetic. fast_path_supported() says whether a parameter
 // block is one it implements; everything else runs the FP32 kernel.
 bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx);
+// whether a supported block needs the general form (plain min/max box or bilinear history)
+bool fast_path_general(const taa_uniforms_t& U, int param_idx);
 int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape, void* stream);
 int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16,
-                      int grid, void* stream);
+                      bool general, int grid, void* stream);
 
 // Edge-AA fallback on category-1 texels of a segmentation mask (taa_edge_aa.hip).
 int launch_edge_aa(const void* src, const uint32_t* mask, void* dst, int W, int H, int pitch, int seg_pitch, int fmt,

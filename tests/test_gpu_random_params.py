@@ -47,6 +47,9 @@ def random_params(rng, fast_only=False):
     p.mUseVelocityVectors = int(rng.choice([1, 2]))
     if fast_only:
         p.mUseVelocityVectors = 2
+        p.mVarianceClipping = b()  # off: the plain min/max box
+        p.mShapedNeighbourhood = 0
+        p.mInterpolationMode = int(rng.choice([0, 2]))  # bilinear or Catmull-Rom history
         return p
     p.mVarianceClipping = b()
     p.mShapedNeighbourhood = b()
