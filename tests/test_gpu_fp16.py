@@ -27,7 +27,7 @@ pytestmark = pytest.mark.gpu
 
 from gpu_helpers import INDEX_WORDS, psnr, to_f16  # noqa: E402
 from taastar import FLAG_RECTIFIED  # noqa: E402
-from taastar import ARITH_FP16, INSPECT  # noqa: E402
+from taastar import ARITH_FP16, FMT_RGBA16F, FMT_RGBA32F, INSPECT  # noqa: E402
 from taastar.workloads import (cas_for, decode, encode_color, encode_velocity, scene_for, uniforms_for,  # noqa: E402
                                velocity_as_read, workload)
 from scipy.ndimage import binary_dilation  # noqa: E402
@@ -141,3 +141,27 @@ def test_fp16_path_fp32_storage_indices(oracle):
         s = oracle.cas(o["screen"], k.const1[0])
         assert psnr(g_scr[..., :3], s[..., :3]) > 60.0
         h32, prev = o["history"], dep
+
+
+@pytest.mark.parametrize("fmt", [FMT_RGBA16F, FMT_RGBA32F])
+def test_reference_default_block(oracle, fmt):
+    """The reference's default parameter block (taa.hpp:31-76: plain min/max box, clamp, bilinear
+    history) on the FP16 kernel's general form: texel indices, taps and decisions as the FP32
+    transcription's, the sharpened screen above 55 dB against it."""
+    from gpu_helpers import Sequence
+    from taastar import VEL_RG16F, VEL_RG32F, default_params
+    print(f"case fp16 reference defaults fmt={fmt}", flush=True)
+    p = default_params()
+    p.mUseVelocityVectors = 2  # RG velocity: every texel moves (w reads as 1)
+    w = workload("c2", fmt=fmt, vel=VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F, params=lambda: p)
+    seq = Sequence(w, width=480, height=272, cas=True, arith=ARITH_FP16, debug=False)
+    for f in range(4):
+        out = seq.step(oracle)
+        gi, oi = out["gpu_inspect"], out["oracle_inspect"]
+        for wd in INDEX_WORDS:
+            g, o = gi[..., wd], oi[..., wd]
+            if wd == INSPECT["FLAGS"]:
+                g, o = g & ~FLAG_RECTIFIED, o & ~FLAG_RECTIFIED
+            assert np.count_nonzero(g != o) == 0, f"frame {f}: inspect word {wd}"
+        assert psnr(out["gpu_screen"][..., :3], out["oracle_screen"][..., :3]) > 55.0, f"frame {f}"
+        seq.hist_o = out["gpu_history"]
