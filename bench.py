@@ -50,6 +50,9 @@ def parse():
                     help="time only the velocity producer (4K, RG16F, 3 moving objects) and print its line")
     ap.add_argument("--taau-only", action="store_true",
                     help="time only the upsampling resolve (1080p and 1440p inputs to 4K) and print its lines")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the 48-stream batched load (FP16, then FP32 storage and arithmetic) for "
+                         "--sim-seconds each and print its lines")
     return ap.parse_args()
 
 
@@ -391,6 +394,11 @@ MODE_NAMES = {0: "served open loop (Poisson arrivals, churn)", 1: "served closed
               4: "batched: one graph-captured launch per round of all sessions"}
 
 
+MIX48 = (("720p", 16), ("1080p", 16), ("1440p", 8), ("4k", 8))
+MIX48_TEXT = "16x1280x720, 16x1920x1080, 8x2560x1440, 8x3840x2160; every stream its own scene, jitter phase and " \
+             "pan of 0-24 px/frame"
+
+
 def simulate(mode, mix, sessions, fmt, arith, seconds, exact=True, rate=0.0, churn=0.0, budget=0, seed=11,
              max_batch=64, max_inflight=3, warmup=2, own_inputs=True, pan_max=24.0):
     from taastar import ServeReport, ServeSimConfig, check, lib
@@ -481,11 +489,10 @@ def extras(a):
     small["served_over_per_frame_sync"] = round(small["mode1"]["gpix_per_s"] / sb, 3)
     small["batched_over_per_frame_async"] = round(small["mode4"]["gpix_per_s"] / small["mode2"]["gpix_per_s"], 3)
     x["sessions64_360p"] = small
-    # 48 streams in one persistent graph-captured launch, both storages
-    mix48 = [("720p", 16), ("1080p", 16), ("1440p", 8), ("4k", 8)]
+    # 48 streams in one graph-captured launch per round, both storages
+    mix48 = list(MIX48)
     x["batched48"] = {
-        "mix": "16x1280x720, 16x1920x1080, 8x2560x1440, 8x3840x2160; every stream its own scene, jitter phase and "
-               "pan of 0-24 px/frame",
+        "mix": MIX48_TEXT,
         "fp16_storage_fp16_arith": simulate(4, mix48, 48, F16, A16, s),
         "fp32_storage_fp32_arith": simulate(4, mix48, 48, F32, A32, s),
         "fp16_per_frame_async": simulate(2, mix48, 48, F16, A16, s),
@@ -540,6 +547,16 @@ def main():
     if a.taau_only:
         for lo in ((1920, 1080), (2560, 1440)):
             print(json.dumps(dict(kernel="k_resolve_up", **time_taau(lo, (3840, 2160), a.steps, a.warmup))), flush=True)
+        return
+    if a.batched_only:
+        import taastar
+        from taastar.workloads import RESOLUTIONS
+        for fmt, ar, name in ((taastar.FMT_RGBA16F, taastar.ARITH_FP16, "fp16"),
+                              (taastar.FMT_RGBA32F, taastar.ARITH_FP32, "fp32")):
+            for cls, _ in MIX48:
+                tune(*RESOLUTIONS[cls], fmt, ar, True)
+            r = simulate(4, list(MIX48), 48, fmt, ar, a.sim_seconds)
+            print(json.dumps(dict(load="batched48", storage_and_arith=name, mix=MIX48_TEXT, **r)), flush=True)
         return
     r = time_resolve(a.workload, a.arith, a.steps, a.warmup, a.frames, autotune=a.autotune)
     w = r["w"]
