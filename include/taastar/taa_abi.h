@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define TAASTAR_ABI_VERSION 2
+#define TAASTAR_ABI_VERSION 3
 
 typedef int32_t taa_bool32; /* VkBool32: 0 = false, anything else = true */
 

@@ -35,7 +35,7 @@ typedef struct taa_server_config {
 
 typedef struct taa_completion {
     int32_t  session_id;
-    int32_t  pad;
+    int32_t  status;            /* TAA_OK, or the error of the batched launch that failed */
     uint64_t tag;
     double   t_submit_us;       /* host time of taa_server_submit (or the given arrival time) */
     double   t_done_us;         /* GPU completion time on the same clock */
@@ -43,6 +43,7 @@ typedef struct taa_completion {
 
 typedef struct taa_server_stats {
     uint64_t batches, frames, pixels;
+    uint64_t frames_failed;     /* completed with a non-zero status */
     uint64_t sessions_open, admissions_refused;
     uint64_t hbm_used, hbm_budget;
     double   busy_us;           /* sum of batch GPU spans */
@@ -54,10 +55,16 @@ int32_t taa_server_destroy(taa_server_t* s);
 int32_t taa_server_open(taa_server_t* s, int32_t width, int32_t height, int32_t* session_id);
 /* Waits for the session's queued and in-flight frames, then frees it. */
 int32_t taa_server_close(taa_server_t* s, int32_t session_id);
-/* The session's input slots (see taa_session_io) and its output (screen) image. */
+/* The session's input slots (see taa_session_io) and its output (screen) image. Waits (pumping)
+ * until the session has no queued or running frame, so the caller may then write the next
+ * frame's inputs and read the last result; both stay valid until the next taa_server_submit of
+ * this session. */
 int32_t taa_server_io(taa_server_t* s, int32_t session_id, taa_session_io_t* io, void** screen);
 /* Queue one frame; its inputs must already be in the session's slots. arrival_us < 0 stamps the
- * call time. Frames of one session are resolved in order, one at a time. */
+ * call time. Frames of one session are resolved in order, one at a time, each reading the slots
+ * as they are when it runs (several queued frames of a session therefore resolve the same
+ * inputs). Uniforms a batched launch refuses (taa_batch_submit) are refused here with its status;
+ * a frame whose batched launch fails completes with that status. */
 int32_t taa_server_submit(taa_server_t* s, int32_t session_id, const taa_uniforms_t* uniforms, uint64_t tag,
                           double arrival_us);
 /* Harvest finished batches and launch queued frames into free in-flight slots. */

@@ -312,7 +312,28 @@ void session_advance(taa_session* s) {
     s->force_reset = false;
 }
 
+// After work that reads a pinned staging buffer has been enqueued on `st`: mark the buffer as in
+// use until `done` passes. If the event cannot be recorded the stream is drained instead, so that
+// the buffer is never rewritten while a copy from it may still be pending.
+void staging_enqueued(hipEvent_t done, bool& used, hipStream_t st) {
+    if (hipEventRecord(done, st) == hipSuccess) {
+        used = true;
+    } else {
+        hipStreamSynchronize(st);
+        used = false;
+    }
+}
+
 }  // namespace
+
+namespace taastar {
+// What a batched frame's uniforms must satisfy (also checked by the server when a frame is queued).
+int check_batch_uniforms(const taa_uniforms_t* u) {
+    if (int rc = check_uniforms(u)) return rc;
+    if (u->splitScreen) return fail(TAA_ERR_UNSUPPORTED, "split screen in a batch");
+    return TAA_OK;
+}
+}  // namespace taastar
 
 extern "C" {
 
@@ -657,9 +678,7 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
     int best_px = -1;
     for (int i = 0; i < n; ++i) {
         const taa_batch_item_t& it = items[i];
-        int rc = check_uniforms(it.uniforms);
-        if (rc) return rc;
-        if (it.uniforms->splitScreen) return fail(TAA_ERR_UNSUPPORTED, "item %d: split screen in a batch", i);
+        if (int rc = taastar::check_batch_uniforms(it.uniforms)) return rc;
         if (session_upsamples(it.session, it.uniforms)) {
             ++L.n_up;
             continue;
@@ -719,6 +738,7 @@ int32_t taa_batch_submit(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items,
     if (slot.used) {
         hipError_t e = hipEventSynchronize(slot.done);  // the slot's previous batch has been consumed
         if (e != hipSuccess) return hip_fail(e, "staging slot wait");
+        slot.used = false;
     }
     BatchLayout L;
     int rc = build_batch(c, n, items, arith, slot.host, L);
@@ -726,14 +746,15 @@ int32_t taa_batch_submit(taa_ctx_t* c, int32_t n, const taa_batch_item_t* items,
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e = hipMemcpyAsync(slot.dev, slot.host, sizeof(taastar::BatchEntry) * (size_t)n, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "descriptor upload");
-    int le = launch_layout(static_cast<const taastar::BatchEntry*>(slot.dev), L, arith, stream);
-    if (le) return hip_fail((hipError_t)le, "batch launch");
-    if ((e = hipEventRecord(slot.done, st)) != hipSuccess) return hip_fail(e, "event record");
-    slot.used = true;
+    // the copy from the pinned slot is enqueued: the slot stays reserved whatever follows
+    const int le = launch_layout(static_cast<const taastar::BatchEntry*>(slot.dev), L, arith, stream);
+    staging_enqueued(slot.done, slot.used, st);
     c->next_slot = (c->next_slot + 1) % taa_ctx::kSlots;
+    if (le) return hip_fail((hipError_t)le, "batch launch");
     for (int i = 0; i < n; ++i) session_advance(items[i].session);
     return ok();
 }
+
 
 }  // extern "C"
 
@@ -835,6 +856,7 @@ int32_t taa_batch_plan_launch(taa_batch_plan_t* p, void* stream) {
     if (b.used) {
         hipError_t e = hipEventSynchronize(b.done);  // this buffer's previous replay has finished
         if (e != hipSuccess) return hip_fail(e, "plan buffer wait");
+        b.used = false;
     }
     BatchLayout L;
     int rc = build_batch(p->ctx, p->n, p->items.data(), p->arith, b.host, L);
@@ -851,8 +873,7 @@ int32_t taa_batch_plan_launch(taa_batch_plan_t* p, void* stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e = hipGraphLaunch(b.exec[fam], st);
     if (e != hipSuccess) return hip_fail(e, "graph launch");
-    if ((e = hipEventRecord(b.done, st)) != hipSuccess) return hip_fail(e, "event record");
-    b.used = true;
+    staging_enqueued(b.done, b.used, st);  // the graph copies from b.host
     p->next ^= 1;
     for (int i = 0; i < p->n; ++i) session_advance(p->items[i].session);
     return ok();

@@ -107,6 +107,9 @@ int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tile
 int launch_edge_aa(const void* src, const uint32_t* mask, void* dst, int W, int H, int pitch, int seg_pitch, int fmt,
                    float subpix, float thr, float thr_min, void* stream);
 
+// TAA_OK, or the status taa_batch_submit returns for a frame with these uniforms (taa_api.cpp).
+int check_batch_uniforms(const taa_uniforms_t* u);
+
 // Record `msg` as the calling thread's taa_last_error() and return `code` (taa_api.cpp).
 int set_last_error(int code, const char* msg);
 

@@ -18,6 +18,7 @@
 #include <random>
 #include <vector>
 
+#include "taa_internal.hpp"
 #include "taastar/taa_abi.h"
 #include "taastar/taa_scene.h"
 #include "taastar/taa_serve.h"
@@ -152,7 +153,31 @@ int launch_ready(taa_server* s) {
         for (size_t i = 0; i < items.size(); ++i) items[i].uniforms = &sl.reqs[i].U;
         hipEventRecord(sl.ev_start, sl.st);
         int rc = taa_batch_submit(s->ctx, (int)items.size(), items.data(), s->cfg.arith, sl.st);
-        if (rc) return rc;
+        if (rc) {
+            // nothing of this batch runs: its frames complete with the error, their sessions are
+            // free again (and queued behind the failure if more frames wait), the slot is empty
+            const double now = host_us();
+            for (size_t i = 0; i < sl.sids.size(); ++i) {
+                taa_completion_t c;
+                std::memset(&c, 0, sizeof(c));
+                c.session_id = sl.sids[i];
+                c.status = rc;
+                c.tag = sl.reqs[i].tag;
+                c.t_submit_us = sl.reqs[i].t_submit;
+                c.t_done_us = now;
+                s->done.push_back(c);
+                auto it = s->sess.find(sl.sids[i]);
+                if (it != s->sess.end()) {
+                    it->second.busy = false;
+                    if (!it->second.q.empty()) s->ready.push_back(sl.sids[i]);
+                }
+            }
+            s->st.frames_failed += sl.sids.size();
+            sl.sids.clear();
+            sl.reqs.clear();
+            sl.pixels = 0;
+            return rc;
+        }
         hipEventRecord(sl.ev_done, sl.st);
         sl.busy = true;
         s->st.batches++;
@@ -291,6 +316,12 @@ int32_t taa_server_io(taa_server_t* s, int32_t sid, taa_session_io_t* io, void**
     if (!s) return TAA_ERR_INVALID_ARG;
     auto it = s->sess.find(sid);
     if (it == s->sess.end()) return TAA_ERR_INVALID_ARG;
+    // the slots are the ones the session's queued and running frames read: hand them out only
+    // once every such frame has completed, so writes through them cannot race a resolve
+    while (it->second.busy || !it->second.q.empty()) {
+        int rc = taa_server_pump(s);
+        if (rc) return rc;
+    }
     if (screen) *screen = it->second.screen;
     return io ? taa_session_io(it->second.s, io) : TAA_OK;
 }
@@ -299,6 +330,9 @@ int32_t taa_server_submit(taa_server_t* s, int32_t sid, const taa_uniforms_t* U,
     if (!s || !U) return TAA_ERR_INVALID_ARG;
     auto it = s->sess.find(sid);
     if (it == s->sess.end()) return TAA_ERR_INVALID_ARG;
+    // refuse here what the batched launch would refuse, so one bad frame cannot fail a batch of
+    // other sessions' frames
+    if (int rc = taastar::check_batch_uniforms(U)) return rc;
     Sess& se = it->second;
     Req r;
     r.tag = tag;

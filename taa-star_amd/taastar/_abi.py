@@ -115,13 +115,13 @@ class ServerConfig(C.Structure):
 
 
 class Completion(C.Structure):
-    _fields_ = [("session_id", C.c_int32), ("pad", C.c_int32), ("tag", C.c_uint64), ("t_submit_us", C.c_double),
+    _fields_ = [("session_id", C.c_int32), ("status", C.c_int32), ("tag", C.c_uint64), ("t_submit_us", C.c_double),
                 ("t_done_us", C.c_double)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("batches", C.c_uint64), ("frames", C.c_uint64), ("pixels", C.c_uint64),
-                ("sessions_open", C.c_uint64), ("admissions_refused", C.c_uint64), ("hbm_used", C.c_uint64),
+                ("frames_failed", C.c_uint64), ("sessions_open", C.c_uint64), ("admissions_refused", C.c_uint64), ("hbm_used", C.c_uint64),
                 ("hbm_budget", C.c_uint64), ("busy_us", C.c_double)]
 
 

@@ -47,7 +47,7 @@ def test_struct_layouts_match_reference():
 
 
 def test_abi_version(taastar_lib):
-    assert taastar_lib.taa_abi_version() == 2
+    assert taastar_lib.taa_abi_version() == 3
 
 
 def test_param_defaults_match_reference():

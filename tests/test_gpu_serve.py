@@ -261,3 +261,73 @@ def test_simulation_per_frame_and_batched_modes(mode):
         assert 0 < rep.lat_p50_us <= rep.lat_p99_us
     if mode == 4:
         assert rep.mean_batch == 8 and rep.lat_mean_us > 0
+
+
+def test_server_refuses_bad_frames_and_keeps_serving():
+    """A frame whose uniforms a batched launch refuses (split screen, mRayTraceAugment without the
+    mask images, an unknown history filter) is refused by taa_server_submit with that status and
+    never joins a batch: the other sessions' frames complete with status 0 and closing returns."""
+    from taastar._abi import TAA_ERR_UNSUPPORTED
+    w = workload("c3", width=160, height=96)
+    cfg = ServerConfig(device=0, hbm_budget_bytes=0, color_format=FMT_RGBA16F, arith=ARITH_FP16, max_batch=8,
+                       max_inflight=2, cas_sharpness=0.5)
+    srv = C.c_void_p()
+    check(lib().taa_server_create(C.byref(cfg), C.byref(srv)))
+    streams = [Stream(w, 160, 96, 300 + i) for i in range(3)]
+    sids = []
+    for st in streams:
+        sid = C.c_int32()
+        check(lib().taa_server_open(srv, st.W, st.H, C.byref(sid)))
+        sids.append(sid.value)
+    for f in range(3):
+        for k, (st, sid) in enumerate(zip(streams, sids)):
+            io = SessionIO()
+            check(lib().taa_server_io(srv, sid, C.byref(io), None))
+            U = st.upload(io, f)
+            if k == 1:
+                bad = [("splitScreen", 1, TAA_ERR_UNSUPPORTED), ("rta", 1, TAA_ERR_UNSUPPORTED),
+                       ("interp", 5, TAA_ERR_INVALID_ARG)][f]
+                Ub = taastar.Uniforms.from_buffer_copy(U)
+                if bad[0] == "splitScreen":
+                    Ub.splitScreen = 1
+                elif bad[0] == "rta":
+                    Ub.param[0].mRayTraceAugment = 1
+                else:
+                    Ub.param[0].mInterpolationMode = bad[1]
+                assert lib().taa_server_submit(srv, sid, C.byref(Ub), 100 + f, -1.0) == bad[2], bad
+            check(lib().taa_server_submit(srv, sid, C.byref(U), f, -1.0))
+        check(lib().taa_server_drain(srv))
+        comp = (Completion * 16)()
+        n = C.c_int32()
+        check(lib().taa_server_poll(srv, comp, 16, C.byref(n)))
+        assert n.value == 3 and all(comp[i].status == 0 and comp[i].tag == f for i in range(3))
+    st = taastar.ServerStats()
+    check(lib().taa_server_stats(srv, C.byref(st)))
+    assert st.frames == 9 and st.frames_failed == 0
+    for sid in sids:
+        check(lib().taa_server_close(srv, sid))
+    check(lib().taa_server_destroy(srv))
+
+
+def test_server_io_waits_for_the_sessions_frames():
+    """taa_server_io hands out a session's slots only once its queued and running frames are done
+    (so the next frame's inputs cannot overwrite what a running resolve reads)."""
+    w = workload("c3", width=1280, height=720)
+    cfg = ServerConfig(device=0, hbm_budget_bytes=0, color_format=FMT_RGBA16F, arith=ARITH_FP16, max_batch=8,
+                       max_inflight=2, cas_sharpness=0.5)
+    srv = C.c_void_p()
+    check(lib().taa_server_create(C.byref(cfg), C.byref(srv)))
+    st = Stream(w, 1280, 720, 41)
+    sid = C.c_int32()
+    check(lib().taa_server_open(srv, 1280, 720, C.byref(sid)))
+    io = SessionIO()
+    check(lib().taa_server_io(srv, sid.value, C.byref(io), None))
+    U = st.upload(io, 0)
+    for f in range(4):  # four queued frames of one session
+        check(lib().taa_server_submit(srv, sid.value, C.byref(U), f, -1.0))
+    check(lib().taa_server_pump(srv))
+    check(lib().taa_server_io(srv, sid.value, C.byref(io), None))
+    stats = taastar.ServerStats()
+    check(lib().taa_server_stats(srv, C.byref(stats)))
+    assert stats.frames == 4 and io.frames_resolved == 4
+    check(lib().taa_server_destroy(srv))

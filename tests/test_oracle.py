@@ -1018,3 +1018,254 @@ def test_edge_aa_fp32_tracks_fp64(oracle):
     a = oracle.edge_aa(src, mask, dtype=np.float64)
     b = oracle.edge_aa(src, mask, dtype=np.float32)
     np.testing.assert_allclose(b, a, atol=2e-6)
+
+
+# ---------------------------------------------------------------------------------------------
+# branches pinned by closed forms: slow clip, velocity-based alpha, reduce-blend-near-clamp,
+# dynamic anti-ghosting, debug modes 0-7, and CasFilter's green weight and approximate reciprocal
+
+def np_bilinear(img, hu, hv):
+    """Clamp-to-edge bilinear sample of img [H, W, C] at uv positions hu, hv [H, W] (numpy, FP64)."""
+    H, W = img.shape[:2]
+    fx, fy = hu * W - 0.5, hv * H - 0.5
+    x0, y0 = np.floor(fx).astype(int), np.floor(fy).astype(int)
+    tx, ty = (fx - x0)[..., None], (fy - y0)[..., None]
+    xa, xb = np.clip(x0, 0, W - 1), np.clip(x0 + 1, 0, W - 1)
+    ya, yb = np.clip(y0, 0, H - 1), np.clip(y0 + 1, 0, H - 1)
+    top = img[ya, xa] * (1 - tx) + img[ya, xb] * tx
+    bot = img[yb, xa] * (1 - tx) + img[yb, xb] * tx
+    return top * (1 - ty) + bot * ty
+
+
+def slab_clip(p, q, mn, mx):
+    """Where the segment from p (inside the box [mn, mx]) towards q leaves the box: p + t (q - p)
+    with t the smallest slab parameter, capped at 1 (the textbook ray / box slab test)."""
+    d = q - p
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t_hi = np.where(d > 0, (mx - p) / d, np.inf)
+        t_lo = np.where(d < 0, (mn - p) / d, np.inf)
+    t = np.minimum(1.0, np.minimum(t_hi, t_lo).min(-1))
+    return p + t[..., None] * d, t
+
+
+def blend_of(o):
+    return o["inspect"][..., INSPECT["BLEND"]].copy().view(f32).astype(np.float64)
+
+
+@pytest.mark.parametrize("variance", [0, 1])
+def test_slow_clip_is_the_slab_intersection(oracle, variance):
+    """clipAabbSlow (taa.comp:348-369, mode 3): the history moves along the segment from the
+    clip-towards colour (the centre tap for the min/max box, the mean for the variance box, both
+    inside the box) to the history sample and stops where that segment leaves the box; a history
+    inside the box is unchanged."""
+    rng = np.random.default_rng(92)
+    H, W = 8, 16  # powers of two: zero velocity samples the history texel exactly
+    col, dep, vel, _ = frame(H, W, rng)
+    hist = np.concatenate([rng.uniform(-3, 5, (H, W, 3)), np.zeros((H, W, 1))], -1)
+    hist[:3, :4, :3] = col[:3, :4, :3] * 0.999 + 0.0005  # a few inside the box (near the centre tap)
+    p = params(mColorClampingOrClipping=3, mInterpolationMode=0, mVarianceClipping=variance)
+    o = run(oracle, uniforms(p), col, dep, vel, hist)
+    w = windows(col[..., :3].astype(np.float64))
+    if variance:
+        mean, std = w.mean(0), w.std(0)
+        towards, mn, mx = mean, mean - std, mean + std
+    else:
+        towards, mn, mx = col[..., :3].astype(np.float64), w.min(0), w.max(0)
+    expect, t = slab_clip(towards, hist[..., :3], mn, mx)
+    got = o["history"][..., :3]
+    assert np.abs(got - expect).max() < 1e-6
+    inside = t >= 1.0
+    assert 0 < inside.sum() < inside.size and np.abs(got[inside] - hist[..., :3][inside]).max() < 1e-12
+    # outside: on a face of the box, inside every slab
+    out = ~inside
+    on_face = (np.abs(got - mn) < 1e-6) | (np.abs(got - mx) < 1e-6)
+    assert np.all(on_face[out].any(-1))
+    assert np.all(got[out] >= mn[out] - 1e-6) and np.all(got[out] <= mx[out] + 1e-6)
+
+
+def test_velocity_based_alpha(oracle):
+    """taa.comp:855-859: no motion leaves alpha unchanged; once pixelSpeed * mVelBasedAlphaFactor
+    reaches 1 alpha is max(alpha, mVelBasedAlphaMax); in between it moves linearly towards that
+    maximum and never decreases."""
+    rng = np.random.default_rng(93)
+    H, W = 8, 16
+    col, dep, _, hist = frame(H, W, rng)
+    vmax, fac = 0.625, 4.0
+    for a in (0.125, 0.875):
+        for speed, frac in ((0.0, 0.0), (0.0625, 0.25), (0.125, 0.5), (0.25, 1.0), (0.5, 1.0)):
+            vel = np.zeros((H, W, 4), f32)
+            vel[..., 0], vel[..., 1], vel[..., 3] = speed * 0.6, -speed * 0.8, 1.0  # |v| = speed
+            p = params(mAlpha=a, mVelBasedAlpha=1, mVelBasedAlphaMax=vmax, mVelBasedAlphaFactor=fac)
+            alpha = blend_of(run(oracle, uniforms(p), col, dep, vel, hist))
+            expect = max(a, a + (vmax - a) * frac)
+            assert np.abs(alpha - expect).max() < 1e-6, (a, speed)
+    # off: alpha is mAlpha whatever the speed
+    vel[..., 0] = 0.3
+    alpha = blend_of(run(oracle, uniforms(params(mAlpha=0.125, mVelBasedAlphaMax=vmax)), col, dep, vel, hist))
+    assert np.all(alpha == 0.125)
+
+
+def test_reduce_blend_near_clamp(oracle):
+    """taa.comp:873-894: alpha scales by clamp(4 d, 0, 1), d = 2 |min(lh - lmin, lmax - lh)| /
+    (lmax - lmin) on the luminance of the box and of the unrectified history: a history at the
+    box centre keeps alpha, one on a face gives alpha = 0, one an eighth of the box inside (or
+    outside) a face halves it; a box under 0.001 in luminance keeps alpha."""
+    rng = np.random.default_rng(94)
+    H, W = 8, 16
+    g = rng.uniform(0.1, 0.9, (H, W)).astype(f32)
+    g[1:6, 9:15] = f32(0.4)  # flat: the box of the texels inside has zero extent
+    col = np.stack([g, g, g, np.ones_like(g)], -1)  # grey: luminance = the value
+    dep = np.full((H, W), 0.5, f32)
+    vel = np.zeros((H, W, 4), f32)
+    vel[..., 3] = 1
+    w = windows(g.astype(np.float64))
+    lmin, lmax = w.min(0), w.max(0)
+    choices = np.array([0.5, 0.0, 1.0, 1 / 16, 1 / 8, -0.5, 1 + 1 / 16, 0.3])
+    factor = np.array([1.0, 0.0, 0.0, 0.5, 1.0, 1.0, 0.5, 1.0])
+    k = rng.integers(0, len(choices), (H, W))
+    hv = lmin + choices[k] * (lmax - lmin)
+    hist = np.stack([hv, hv, hv, np.zeros_like(hv)], -1)
+    for ycocg in (0, 1):
+        # clamping on: the factor reads the history before rectification (taa.comp:878)
+        p = params(mAlpha=0.5, mReduceBlendNearClamp=1, mInterpolationMode=0, mUseYCoCg=ycocg, mColorClampingOrClipping=1)
+        alpha = blend_of(run(oracle, uniforms(p), col, dep, vel, hist))
+        flat = (lmax - lmin) < 0.001
+        assert flat.sum() == 12
+        expect = np.where(flat, 0.5, 0.5 * factor[k])
+        assert np.abs(alpha - expect).max() < 1e-6, ycocg
+    # off: alpha unchanged
+    alpha = blend_of(run(oracle, uniforms(params(mAlpha=0.5, mInterpolationMode=0)), col, dep, vel, hist))
+    assert np.all(alpha == 0.5)
+
+
+def test_dynamic_anti_ghosting(oracle):
+    """taa.comp:796-811: a texel moves when |v.x| or |v.y| exceeds 1e-5 and v.w >= 0.5. Where none
+    of the five cross taps (clamped to the edge) moves but the filtered history's alpha is > 0, the
+    history is rejected (alpha = mRejectionAlpha); the new history alpha is 1 exactly where the
+    centre moves, 0 elsewhere."""
+    rng = np.random.default_rng(95)
+    H, W = 16, 16
+    col, dep, _, _ = frame(H, W, rng)
+    vel = np.zeros((H, W, 4), f32)
+    kind = rng.integers(0, 6, (H, W))
+    # 0 still; 1 mover shifted by whole texels; 2 mover flag but no motion; 3 motion, w = 0.3;
+    # 4 motion at w = 0.5 exactly; 5 mover with a motion of 5e-6 (below the threshold)
+    sx = rng.integers(-2, 3, (H, W)).astype(f32) / f32(W)
+    vel[..., 0] = np.select([kind == 1, kind == 3, kind == 4, kind == 5], [sx, sx + f32(1 / 16), f32(-1 / 16), f32(5e-6)], 0)
+    vel[..., 1] = np.where(kind == 1, f32(1 / 16), 0)
+    vel[..., 3] = np.select([kind == 1, kind == 2, kind == 3, kind == 4, kind == 5], [1.0, 1.0, 0.3, 0.5, 1.0], 0.0)
+    hist = np.concatenate([rng.uniform(0.1, 1, (H, W, 3)), rng.integers(0, 2, (H, W, 1)).astype(np.float64)], -1)
+    p = params(mDynamicAntiGhosting=1, mAlpha=0.125, mRejectionAlpha=0.75, mInterpolationMode=0)
+    o = run(oracle, uniforms(p), col, dep, vel, hist)
+    v64 = vel.astype(np.float64)
+    mov = ((np.abs(v64[..., 0]) > 1e-5) | (np.abs(v64[..., 1]) > 1e-5)) & (v64[..., 3] >= 0.5)
+    assert np.array_equal(mov, np.isin(kind, (1, 4)) & ((v64[..., 0] != 0) | (v64[..., 1] != 0)))
+    mp = np.pad(mov, 1, mode="edge")
+    movement = mov | mp[1:-1, :-2] | mp[1:-1, 2:] | mp[:-2, 1:-1] | mp[2:, 1:-1]
+    x = (np.arange(W)[None, :] + 0.5) / W
+    y = (np.arange(H)[:, None] + 0.5) / H
+    ha = np_bilinear(hist[..., 3:], x - v64[..., 0], y - v64[..., 1])[..., 0]
+    rejected = ~movement & (ha > 0)
+    assert 0 < rejected.sum() < rejected.size and (~movement & (ha == 0)).any()
+    flags = o["inspect"][..., INSPECT["FLAGS"]]
+    assert np.array_equal((flags & taastar.FLAG_GHOST_REJECT) != 0, rejected)
+    assert np.array_equal(blend_of(o), np.where(rejected, 0.75, 0.125))
+    assert np.array_equal(o["history"][..., 3], mov.astype(np.float64))
+    # off: nothing rejected, history alpha 0
+    o = run(oracle, uniforms(params(mAlpha=0.125, mInterpolationMode=0)), col, dep, vel, hist)
+    assert not np.any(o["inspect"][..., INSPECT["FLAGS"]] & taastar.FLAG_GHOST_REJECT)
+    assert np.all(o["history"][..., 3] == 0)
+
+
+def test_debug_modes_closed_forms(oracle):
+    """taa.comp:912-942: 0 the box extent per channel, 1 its volume, 2 (rejected, length of the
+    rectification), 3 alpha (without beta), 4 the velocity texel, 5 pixelSpeed, 6 the screen, 7 the
+    history output; then * mDebugScale * mDebugMask and, with mDebugCenter, * 0.5 + 0.5."""
+    rng = np.random.default_rng(96)
+    H, W = 8, 16
+    col, dep, _, hist = frame(H, W, rng)
+    hist[..., :3] = rng.uniform(-0.5, 2.5, (H, W, 3))
+    vel = np.zeros((H, W, 4), f32)
+    vel[..., 0], vel[..., 2], vel[..., 3] = f32(2 / 16), f32(0.25), f32(1.0)  # two texels left
+    c = col[..., :3].astype(np.float64)
+    w = windows(c)
+    mn, mx = w.min(0), w.max(0)
+    xs = np.clip(np.arange(W) - 2, 0, W - 1)
+    h = hist[:, xs, :3]  # bilinear at the texel centre two to the left, clamped
+    rej = np.broadcast_to((np.arange(W) + 0.5) / W - 2 / 16 < 0, (H, W))
+    hc = np.clip(h, mn, mx)
+    a = np.where(rej, 0.75, 0.25)[..., None]
+    out = hc * (1 - a) + c * a
+    ones, zeros = np.ones((H, W)), np.zeros((H, W))
+    expect = {
+        0: np.concatenate([mx - mn, zeros[..., None]], -1),
+        1: np.stack([np.prod(mx - mn, -1)] * 3 + [zeros], -1),
+        2: np.stack([rej.astype(np.float64), np.linalg.norm(hc - h, axis=-1), zeros, zeros], -1),
+        3: np.stack([a[..., 0]] * 3 + [zeros], -1),
+        4: vel.astype(np.float64),
+        5: np.stack([ones * 0.125, zeros, zeros, zeros], -1),
+        6: np.concatenate([out, ones[..., None]], -1),
+        7: np.concatenate([out, zeros[..., None]], -1),
+    }
+    for mode, e in expect.items():
+        for scale, mask, center in ((1.0, (1, 1, 1, 1), 0), (2.0, (1, 0.5, 0, 1), 1)):
+            p = params(mColorClampingOrClipping=1, mInterpolationMode=0, mAlpha=0.25, mRejectOutside=1,
+                       mRejectionAlpha=0.75, mDebugMode=mode, mDebugScale=scale, mDebugCenter=center)
+            for i in range(4):
+                p.mDebugMask[i] = mask[i]
+            g = run(oracle, uniforms(p), col, dep, vel, hist)["debug"]
+            ref = e * scale * np.asarray(mask, np.float64)
+            if center:
+                ref = ref * 0.5 + 0.5
+            assert np.abs(g - ref).max() < 1e-9, (mode, scale, center)
+
+
+def test_cas_weight_is_green_for_all_channels(oracle):
+    """ffx_cas.h:508-522: one weight, from green's soft minimum and maximum, for all channels.
+    (a) Green flat above 1 has amplitude 0: a red step is not sharpened, out_r / e_r is the same
+    at every texel. (b) Steps of different contrast in all channels: solving out_c * D - w * S_c =
+    e_c (S_c the four-neighbour sum) for (D, w) from red and green predicts blue, and w matches
+    green's amplitude, not red's."""
+    c0, c1 = oracle.cas_setup(1.0, 32, 8)
+    img = np.full((8, 32, 4), 1.5)
+    img[:, :, 0] = 0.2
+    img[:, 16:, 0] = 0.7
+    out = oracle.cas(img, c1[0], dtype=np.float64)
+    ratio = out[..., 0] / img[..., 0]
+    assert np.ptp(ratio) < 1e-12 and abs(ratio[0, 0] - 1) < 4e-3
+    # (b) red 0.05 -> 0.15 (soft amplitude sqrt(1/3)), green 0.4 -> 0.6 (sqrt(2/3)), blue 0.3 -> 0.5
+    c0, c1 = oracle.cas_setup(0.5, 32, 8)
+    img = np.ones((8, 32, 4))
+    for ch, (lo, hi) in enumerate(((0.05, 0.15), (0.4, 0.6), (0.3, 0.5))):
+        img[:, :16, ch], img[:, 16:, ch] = lo, hi
+    out = oracle.cas(img, c1[0], dtype=np.float64)
+    y = 4
+    peak = float(u2f(c1[0]))
+    for x in (15, 16):
+        e = img[y, x, :3]
+        S = img[y - 1, x, :3] + img[y + 1, x, :3] + img[y, x - 1, :3] + img[y, x + 1, :3]
+        A = np.array([[out[y, x, 0], -S[0]], [out[y, x, 1], -S[1]]])
+        D, wgt = np.linalg.solve(A, e[:2])
+        assert 0 < out[y, x, :3].min() and out[y, x, :3].max() < 1
+        assert abs(out[y, x, 2] * D - wgt * S[2] - e[2]) < 1e-9
+        amp_g = np.sqrt(min(0.4, 1 - 0.6) / 0.6)
+        amp_r = np.sqrt(min(0.05, 1 - 0.15) / 0.15)
+        assert abs(wgt - peak * amp_g) < 0.1 * abs(peak * amp_g)   # the integer approximations: within ~8%
+        assert abs(wgt - peak * amp_r) > 0.15 * abs(peak * amp_g)
+
+
+def test_cas_reciprocal_is_the_approximate_newton_step(oracle):
+    """The final normalisation uses APrxMedRcpF1 (ffx_a.h:1457): one Newton step from an integer
+    estimate, m(x) = b (2 - b x), so x m(x) = 1 - (1 - b x)^2 never exceeds 1. On a flat image of
+    value v the filter returns v x m(x) (x = 1 + 4 w): at or below v, within 4e-3, and strictly
+    below v for almost every v -- an exact reciprocal would return v itself."""
+    c0, c1 = oracle.cas_setup(0.5, 16, 8)
+    below = 0
+    vs = np.linspace(0.01, 0.99, 99)
+    for v in vs:
+        out = oracle.cas(np.full((8, 16, 4), v), c1[0], dtype=np.float64)[..., :3]
+        assert np.ptp(out) == 0
+        r = out[0, 0, 0] / v
+        assert 1 - 4e-3 < r <= 1, v
+        below += r < 1 - 1e-7
+    assert below >= 0.9 * len(vs)
