@@ -10,7 +10,9 @@
  *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
  *   - Every entry point returns an int status (TAA_OK or a negative TAA_ERR_*); the message of
  *     the last failure on the calling thread is available from taa_last_error().
- *   - No entry point throws. Only the context/session/plan constructors allocate device memory.
+ *   - No entry point throws. Only the context/session/plan constructors allocate device memory,
+ *     and taa_resolve_frame for a split screen with CAS (its intermediate screen, allocated on
+ *     first use per device and kept; taa_resolve_frame_noalloc allocates nothing).
  *   - Images are row-major, tightly packed rows of `pitch` pixels (pitch >= width). Every
  *     image must fit in 4 GiB: pitch * height * 16 bytes (64 for the inspect image) < 2^32.
  *
@@ -59,8 +61,14 @@ enum { TAA_VEL_RG32F = 0, TAA_VEL_RG16F = 1, TAA_VEL_RGBA32F = 2, TAA_VEL_RGBA16
  *         the throughput path. It covers the variance or plain min/max box, Catmull-Rom or
  *         bilinear history, simple or closest-depth velocity, clamp or clip, reject-outside, raw
  *         or linear depth test, Lottes weighting and fused CAS, with RG velocity and no debug
- *         image; any other block is resolved by the FP32 path. */
+ *         image; any other block is resolved by the FP32 path (taa_resolve_kernel tells which). */
 enum { TAA_ARITH_FP32 = 0, TAA_ARITH_FP16 = 1 };
+/* the kernel a resolve runs (taa_resolve_kernel) */
+enum {
+    TAA_KERNEL_EXACT = 0,    /* FP32 arithmetic in taa.comp's operation order (k_resolve)      */
+    TAA_KERNEL_FP16 = 1,     /* the FP16-arithmetic throughput kernel (k_resolve_fast)         */
+    TAA_KERNEL_UPSAMPLE = 2  /* the exact FP32 upsampling resolve (k_resolve_up)               */
+};
 
 /* ---- reference structs ------------------------------------------------------------------- */
 
@@ -274,13 +282,27 @@ typedef struct taa_edge_aa_consts {
 /* One taa.comp dispatch over the whole frame (source/taa.hpp:1008-1027) and, when `cas` is not
  * NULL, the CAS pass of source/taa.hpp:1127-1134 fused into the same kernel: result_screen then
  * receives the sharpened screen, exactly what the reference's separate pass would write.
- * Stream-ordered and asynchronous; allocates nothing.
+ * Stream-ordered and asynchronous.
  * An input size different from the output size, or mUpsampling, takes the upsampling path (exact
  * FP32 arithmetic whatever `arith` asks). TAA_ERR_UNSUPPORTED: mRayTraceAugment (it needs the
- * extra images of taa_resolve_frame_segmented); a
- * split screen together with CAS (resolve without CAS, then taa_cas_apply). */
+ * extra images of taa_resolve_frame_segmented).
+ * Split screen (uniforms->splitScreen): columns x > splitX use param[1], the others param[0]
+ * (taa.comp:717). With CAS the frame is sharpened after both parts are resolved, in the
+ * reference's order (taa.hpp:1008-1027, 1111-1139): the parts write an intermediate screen, one
+ * CAS pass reads it into result_screen. That intermediate is the only device memory this call
+ * allocates: one buffer per device, on first use (grown when a larger frame needs it) and kept
+ * until the process ends; calls on different streams are ordered around it. */
 int32_t taa_resolve_frame(const taa_uniforms_t* uniforms, const taa_frame_images_t* images,
                           const taa_cas_consts_t* cas, int32_t arith, void* stream);
+
+/* Which kernel taa_resolve_frame runs for these arguments, without launching anything (the
+ * arguments are checked as the resolve checks them). kernels[0] receives the TAA_KERNEL_* of the
+ * columns of param[0], kernels[1] that of the columns of param[1] (the same as kernels[0] without
+ * split screen). An FP16 request the FP16 kernel does not cover gets TAA_KERNEL_EXACT, and
+ * taa_last_error() then names the first reason (e.g. "a debug image is bound"); otherwise it is
+ * "". No counterpart in the reference (its one shader has no fallback). */
+int32_t taa_resolve_kernel(const taa_uniforms_t* uniforms, const taa_frame_images_t* images,
+                           const taa_cas_consts_t* cas, int32_t arith, int32_t kernels[2]);
 
 /* taa_resolve_frame with the segmentation mask of mRayTraceAugment (taa.comp:589-702, 772-783,
  * 959): where a Parameters block has mRayTraceAugment set, each texel's mask value is computed

@@ -160,6 +160,18 @@ int check_uniforms(const taa_uniforms_t* u, bool segmented = false) {
     return TAA_OK;
 }
 
+// The kernel a resolve of [x_begin, x_end) with parameter block param_idx runs (TAA_KERNEL_*);
+// an FP16 request the FP16 kernel does not cover records the reason in *why.
+int kernel_for(const taastar::FrameDesc& fd, const taa_uniforms_t& U, int param_idx, int arith, const char** why) {
+    *why = nullptr;
+    if (fd.up || U.mUpsampling) {
+        if (arith == TAA_ARITH_FP16) *why = "upsampling (exact FP32 upsampling kernel)";
+        return TAA_KERNEL_UPSAMPLE;
+    }
+    if (arith == TAA_ARITH_FP16 && taastar::fast_path_supported(fd, U, param_idx, why)) return TAA_KERNEL_FP16;
+    return TAA_KERNEL_EXACT;
+}
+
 // Launch the resolve over [x_begin, x_end) with the given parameter block.
 int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, int x_end, int param_idx, int arith, void* stream,
                  const taastar::SegDesc* sg = nullptr) {
@@ -199,16 +211,14 @@ int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, in
     return TAA_OK;
 }
 
-int resolve_impl(const taa_uniforms_t* u, const taa_frame_images_t* img, const taa_cas_consts_t* cas, int arith, void* stream,
-                 const taa_segmentation_images_t* seg = nullptr) {
+// Argument checks of a resolve and its frame descriptor (and segmentation descriptor).
+int prepare_resolve(const taa_uniforms_t* u, const taa_frame_images_t* img, const taa_cas_consts_t* cas, int arith,
+                    const taa_segmentation_images_t* seg, taastar::FrameDesc& fd, taastar::SegDesc& sgd) {
     int rc = check_uniforms(u, seg != nullptr);
     if (rc) return rc;
     if (arith != TAA_ARITH_FP32 && arith != TAA_ARITH_FP16) return fail(TAA_ERR_INVALID_ARG, "bad arith %d", arith);
-    taastar::FrameDesc fd;
     rc = make_desc(img, cas, fd);
     if (rc) return rc;
-    taastar::SegDesc sgd;
-    const taastar::SegDesc* sg = nullptr;
     if (seg) {
         if (fd.up || u->mUpsampling) return fail(TAA_ERR_UNSUPPORTED, "the segmentation mask is built for one size (no upsampling)");
         if (!seg->material || !seg->prev_material || !seg->uv_normal || !seg->prev_segmask || !seg->segmask)
@@ -222,12 +232,99 @@ int resolve_impl(const taa_uniforms_t* u, const taa_frame_images_t* img, const t
             m == fd.depth || m == fd.history || m == fd.hdepth || m == fd.velocity || m == fd.history_out || m == fd.screen)
             return fail(TAA_ERR_INVALID_ARG, "segmask aliases another image");
         sgd = taastar::SegDesc{seg->material, seg->prev_material, seg->uv_normal, seg->prev_segmask, seg->segmask, sp, 0};
-        sg = &sgd;
     }
+    return TAA_OK;
+}
+
+// first column of parameter block 1: paramsIdx = (splitScreen && x > splitX) ? 1 : 0 (taa.comp:717)
+int split_column(const taa_uniforms_t& u, int W) { return std::min(std::max(u.splitX + 1, 0), W); }
+
+// Bytes of the intermediate screen of split screen with CAS.
+size_t split_bytes(const taastar::FrameDesc& fd) { return (size_t)fd.pitch * (size_t)fd.H * color_bpp(fd.color_fmt); }
+
+// The intermediate screen of split screen with CAS taa_resolve_frame uses: one buffer per device,
+// allocated on first use and grown on demand, kept until the process ends. Calls on different
+// streams are ordered through an event recorded after each call's CAS pass.
+struct SplitScratch {
+    std::mutex mu;
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+};
+constexpr int kMaxDevices = 64;
+SplitScratch g_split[kMaxDevices];
+
+// Split screen with CAS, in the reference's order (taa.hpp:1008-1027, 1111-1139): both parts of
+// the resolve write the unsharpened screen to `tmp`, then the CAS pass reads it into the result.
+int resolve_split_cas(taastar::FrameDesc fd, const taa_uniforms_t& u, const taa_cas_consts_t* cas, int arith, void* stream,
+                      const taastar::SegDesc* sg, void* tmp) {
+    void* const screen = fd.screen;
+    fd.screen = tmp;
+    fd.cas_on = 0;
+    fd.cas_peak_bits = 0u;
+    const int split = split_column(u, fd.W);
+    int rc = launch_range(fd, u, 0, split, 0, arith, stream, sg);
+    if (rc) return rc;
+    rc = launch_range(fd, u, split, fd.W, 1, arith, stream, sg);
+    if (rc) return rc;
+    const int e = taastar::launch_cas(tmp, screen, fd.W, fd.H, fd.pitch, fd.color_fmt, cas->const1[0], stream);
+    if (e) return hip_fail((hipError_t)e, "cas launch");
+    return TAA_OK;
+}
+
+int split_cas_library(const taastar::FrameDesc& fd, const taa_uniforms_t& u, const taa_cas_consts_t* cas, int arith,
+                      void* stream, const taastar::SegDesc* sg) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(TAA_ERR_NO_DEVICE, "device %d", dev);
+    SplitScratch& sc = g_split[dev];
+    std::lock_guard<std::mutex> lk(sc.mu);
+    if (!sc.ev && (e = hipEventCreateWithFlags(&sc.ev, hipEventDisableTiming)) != hipSuccess) {
+        sc.ev = nullptr;
+        return hip_fail(e, "hipEventCreate");
+    }
+    const size_t need = split_bytes(fd);
+    if (sc.bytes < need) {
+        if (sc.ptr) {
+            if ((e = hipEventSynchronize(sc.ev)) != hipSuccess) return hip_fail(e, "split scratch wait");
+            hipFree(sc.ptr);
+            sc.ptr = nullptr;
+            sc.bytes = 0;
+        }
+        if ((e = hipMalloc(&sc.ptr, need)) != hipSuccess) {
+            sc.ptr = nullptr;
+            return fail(TAA_ERR_OUT_OF_MEMORY, "split screen intermediate (%zu bytes): %s", need, hipGetErrorString(e));
+        }
+        sc.bytes = need;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if ((e = hipStreamWaitEvent(st, sc.ev, 0)) != hipSuccess) return hip_fail(e, "split scratch order");
+    const int rc = resolve_split_cas(fd, u, cas, arith, stream, sg, sc.ptr);
+    hipEventRecord(sc.ev, st);
+    return rc;
+}
+
+// split_scratch: NULL = the library's per-device intermediate for split screen with CAS; else a
+// device buffer of split_scratch_bytes the caller provides (refused when too small).
+int resolve_impl(const taa_uniforms_t* u, const taa_frame_images_t* img, const taa_cas_consts_t* cas, int arith, void* stream,
+                 const taa_segmentation_images_t* seg = nullptr, void* split_scratch = nullptr,
+                 size_t split_scratch_bytes = 0) {
+    taastar::FrameDesc fd;
+    taastar::SegDesc sgd;
+    int rc = prepare_resolve(u, img, cas, arith, seg, fd, sgd);
+    if (rc) return rc;
+    const taastar::SegDesc* sg = seg ? &sgd : nullptr;
     if (!u->splitScreen) return launch_range(fd, *u, 0, fd.W, 0, arith, stream, sg);
-    if (cas) return fail(TAA_ERR_UNSUPPORTED, "split screen with CAS: resolve without CAS, then taa_cas_apply");
-    // paramsIdx = (splitScreen && x > splitX) ? 1 : 0 (taa.comp:717)
-    const int split = std::min(std::max(u->splitX + 1, 0), fd.W);
+    if (cas && split_scratch) {
+        const size_t need = split_bytes(fd);
+        if (need > split_scratch_bytes)
+            return fail(TAA_ERR_CAPACITY, "split screen with CAS needs a %zu-byte intermediate screen; the scratch holds %zu",
+                        need, split_scratch_bytes);
+        return resolve_split_cas(fd, *u, cas, arith, stream, sg, split_scratch);
+    }
+    if (cas) return split_cas_library(fd, *u, cas, arith, stream, sg);
+    const int split = split_column(*u, fd.W);
     rc = launch_range(fd, *u, 0, split, 0, arith, stream, sg);
     if (rc) return rc;
     return launch_range(fd, *u, split, fd.W, 1, arith, stream, sg);
@@ -377,6 +474,28 @@ int32_t taa_resolve_frame_segmented(const taa_uniforms_t* uniforms, const taa_fr
                                     void* stream) {
     if (!seg) return fail(TAA_ERR_INVALID_ARG, "seg is NULL");
     return resolve_impl(uniforms, images, cas, arith, stream, seg);
+}
+
+int32_t taa_resolve_kernel(const taa_uniforms_t* uniforms, const taa_frame_images_t* images, const taa_cas_consts_t* cas,
+                           int32_t arith, int32_t kernels[2]) {
+    if (!kernels) return fail(TAA_ERR_INVALID_ARG, "kernels is NULL");
+    taastar::FrameDesc fd;
+    taastar::SegDesc sgd;
+    int rc = prepare_resolve(uniforms, images, cas, arith, nullptr, fd, sgd);
+    if (rc) return rc;
+    const char* why[2] = {nullptr, nullptr};
+    const bool split = uniforms->splitScreen != 0;
+    kernels[0] = kernel_for(fd, *uniforms, 0, arith, &why[0]);
+    kernels[1] = split ? kernel_for(fd, *uniforms, 1, arith, &why[1]) : kernels[0];
+    if (!split) why[1] = nullptr;
+    g_err.clear();
+    if (why[0] || why[1]) {
+        char buf[384];
+        if (why[0] && why[1]) std::snprintf(buf, sizeof(buf), "FP16 request resolved by the FP32 path: %s (param[0]); %s (param[1])", why[0], why[1]);
+        else std::snprintf(buf, sizeof(buf), "FP16 request resolved by the FP32 path: %s (param[%d])", why[0] ? why[0] : why[1], why[0] ? 0 : 1);
+        g_err = buf;
+    }
+    return ok();
 }
 
 int32_t taa_cas_apply(const void* src, void* dst, int32_t width, int32_t height, int32_t pitch, int32_t format,
@@ -1029,4 +1148,16 @@ int32_t taa_autotune(taa_ctx_t* c, int32_t w, int32_t h, int32_t fmt, int32_t ar
 
 namespace taastar {
 int set_last_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+int resolve_frame_with_scratch(const taa_uniforms_t* uniforms, const taa_frame_images_t* images, const taa_cas_consts_t* cas,
+                               int arith, void* stream, void* scratch, size_t scratch_bytes) {
+    return resolve_impl(uniforms, images, cas, arith, stream, nullptr, scratch, scratch_bytes);
+}
+
+size_t split_cas_scratch_bytes(const taa_frame_images_t* images) {
+    if (!images) return 0;
+    const int pitch = images->pitch == 0 ? images->width : images->pitch;
+    if (images->height <= 0 || pitch <= 0) return 0;
+    return (size_t)pitch * (size_t)images->height * color_bpp(images->color_format);
+}
 }  // namespace taastar

@@ -588,20 +588,29 @@ hipError_t launch_fast_batch_t(const BatchEntry* e, int n, int total, int shape,
 
 }  // namespace
 
-bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx) {
+bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx, const char** why) {
     const taa_params_t& P = U.param[param_idx];
-    if (fd.vel_fmt != (fd.color_fmt == TAA_FMT_RGBA16F ? TAA_VEL_RG16F : TAA_VEL_RG32F)) return false;
-    if (fd.debug || fd.up || U.mUpsampling) return false;
-    if (U.mBypassHistoryUpdate || P.mPassThrough || P.mToneMapLumaKaris || P.mUnjitterNeighbourhood ||
-        P.mUnjitterCurrentSample)
-        return false;
-    if ((!P.mVarianceClipping && P.mShapedNeighbourhood) || P.mShrinkChromaAxis) return false;
-    if (P.mColorClampingOrClipping < 0 || P.mColorClampingOrClipping > 2) return false;
-    if (P.mInterpolationMode != 2 && P.mInterpolationMode != 0) return false;
-    if (P.mUseVelocityVectors == 0) return false;  // matrix reprojection
-    if (P.mVelocitySampleMode != 0 && P.mVelocitySampleMode != 2) return false;
-    if (P.mDynamicAntiGhosting || P.mVelBasedAlpha || P.mReduceBlendNearClamp || P.mAddNoise) return false;
-    return true;
+    const char* r = nullptr;
+    if (fd.vel_fmt != (fd.color_fmt == TAA_FMT_RGBA16F ? TAA_VEL_RG16F : TAA_VEL_RG32F))
+        r = "velocity format other than RG16F (RGBA16F storage) or RG32F (RGBA32F storage)";
+    else if (fd.debug) r = "a debug image is bound";
+    else if (fd.up || U.mUpsampling) r = "upsampling (input size != output size, or mUpsampling)";
+    else if (U.mBypassHistoryUpdate) r = "mBypassHistoryUpdate";
+    else if (P.mPassThrough) r = "mPassThrough";
+    else if (P.mToneMapLumaKaris) r = "mToneMapLumaKaris";
+    else if (P.mUnjitterNeighbourhood || P.mUnjitterCurrentSample) r = "unjitter";
+    else if (!P.mVarianceClipping && P.mShapedNeighbourhood) r = "mShapedNeighbourhood";
+    else if (P.mShrinkChromaAxis) r = "mShrinkChromaAxis";
+    else if (P.mColorClampingOrClipping < 0 || P.mColorClampingOrClipping > 2) r = "mColorClampingOrClipping 3 (slow clip)";
+    else if (P.mInterpolationMode != 2 && P.mInterpolationMode != 0) r = "B-spline history (mInterpolationMode 1)";
+    else if (P.mUseVelocityVectors == 0) r = "matrix reprojection (mUseVelocityVectors 0)";
+    else if (P.mVelocitySampleMode != 0 && P.mVelocitySampleMode != 2) r = "3x3-longest velocity (mVelocitySampleMode 1)";
+    else if (P.mDynamicAntiGhosting) r = "mDynamicAntiGhosting";
+    else if (P.mVelBasedAlpha) r = "mVelBasedAlpha";
+    else if (P.mReduceBlendNearClamp) r = "mReduceBlendNearClamp";
+    else if (P.mAddNoise) r = "mAddNoise";
+    if (why) *why = r;
+    return r == nullptr;
 }
 
 bool fast_path_general(const taa_uniforms_t& U, int param_idx) {

@@ -95,8 +95,9 @@ int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles,
 
 // Throughput kernel (taa_fast.hip): variance clip + Catmull-Rom + simple/closest-depth velocity
 // with RG velocity, fp16-fed mixed arithmetic. fast_path_supported() says whether a parameter
-// block is one it implements; everything else runs the FP32 kernel.
-bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx);
+// block is one it implements (else *why, when given, names the first reason it is not);
+// everything else runs the FP32 kernel.
+bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx, const char** why = nullptr);
 // whether a supported block needs the general form (plain min/max box or bilinear history)
 bool fast_path_general(const taa_uniforms_t& U, int param_idx);
 int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape, void* stream);
@@ -112,6 +113,14 @@ int check_batch_uniforms(const taa_uniforms_t* u);
 
 // Record `msg` as the calling thread's taa_last_error() and return `code` (taa_api.cpp).
 int set_last_error(int code, const char* msg);
+
+// taa_resolve_frame with the intermediate screen of split screen with CAS in `scratch` (a device
+// buffer of scratch_bytes; TAA_ERR_CAPACITY when the frame needs more) instead of the library's
+// per-device buffer; every other case is taa_resolve_frame itself (taa_api.cpp).
+int resolve_frame_with_scratch(const taa_uniforms_t* uniforms, const taa_frame_images_t* images, const taa_cas_consts_t* cas,
+                               int arith, void* stream, void* scratch, size_t scratch_bytes);
+// Bytes that intermediate screen takes for these images (pitch x height texels of the colour format).
+size_t split_cas_scratch_bytes(const taa_frame_images_t* images);
 
 // Velocity producer (taa_velocity.hip): one launch over width x height; ids may be NULL.
 int launch_velocity(const taa_reproject_t& r, const float* depth, const uint32_t* ids, void* velocity, int fmt, int W,

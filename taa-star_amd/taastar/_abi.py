@@ -17,6 +17,7 @@ TAA_ERR_CAPACITY = -7
 FMT_RGBA32F, FMT_RGBA16F = 0, 1
 VEL_RG32F, VEL_RG16F, VEL_RGBA32F, VEL_RGBA16F = 0, 1, 2, 3
 ARITH_FP32, ARITH_FP16 = 0, 1
+KERNEL_EXACT, KERNEL_FP16, KERNEL_UPSAMPLE = 0, 1, 2
 
 INSPECT_WORDS = 16
 INSPECT = dict(HIST_TC_X=0, HIST_TC_Y=1, CLOSEST_TAP=2, FLAGS=3, COLMIN=4, COLMAX=7, BLEND=10,
@@ -213,6 +214,7 @@ PROTOTYPES = {
     "taa_jitter_offset": (I32, [P(JitterConfig), U64, U32, U32, P(F32), P(U32)]),
     "taa_resolve_frame": (I32, [P(Uniforms), P(FrameImages), P(CasConsts), I32, VP]),
     "taa_resolve_frame_segmented": (I32, [P(Uniforms), P(FrameImages), P(SegmentationImages), P(CasConsts), I32, VP]),
+    "taa_resolve_kernel": (I32, [P(Uniforms), P(FrameImages), P(CasConsts), I32, P(I32)]),
     "taa_cas_apply": (I32, [VP, VP, I32, I32, I32, I32, P(CasConsts), VP]),
     "taa_settings_defaults": (I32, [P(Settings)]),
     "taa_settings_read_ini": (I32, [C.c_char_p, C.c_size_t, P(Settings)]),

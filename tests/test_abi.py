@@ -169,6 +169,49 @@ def test_argument_errors_need_no_gpu():
         assert b"alias" in lib.taa_last_error(), (out_field, in_field, lib.taa_last_error())
 
 
+def test_resolve_kernel_query_needs_no_gpu():
+    """taa_resolve_kernel names the kernel a resolve runs, and why an FP16 request falls back."""
+    from taastar.workloads import params_c2
+    lib = taastar.lib()
+    img = taastar.FrameImages()
+    for i, f in enumerate(("color", "depth", "history", "history_depth", "result_screen", "velocity", "result_history")):
+        setattr(img, f, 0x10000 * (i + 1))
+    img.width, img.height, img.pitch = 64, 32, 64
+    img.color_format, img.velocity_format = taastar.FMT_RGBA16F, taastar.VEL_RG16F
+    U = taastar.Uniforms()
+    U.param[0] = params_c2()
+    U.param[1] = params_c2()
+    k = (C.c_int32 * 2)()
+
+    def query(arith=taastar.ARITH_FP16, images=img):
+        taastar.check(lib.taa_resolve_kernel(C.byref(U), C.byref(images), None, arith, k))
+        return (k[0], k[1]), lib.taa_last_error().decode()
+
+    assert query() == ((taastar.KERNEL_FP16, taastar.KERNEL_FP16), "")
+    assert query(taastar.ARITH_FP32) == ((taastar.KERNEL_EXACT, taastar.KERNEL_EXACT), "")
+    img.velocity_format = taastar.VEL_RGBA16F  # the reference's default velocity format
+    (a, b), why = query()
+    assert (a, b) == (taastar.KERNEL_EXACT, taastar.KERNEL_EXACT) and "velocity format" in why, why
+    img.velocity_format = taastar.VEL_RG16F
+    img.debug = 0x90000
+    (a, b), why = query()
+    assert a == taastar.KERNEL_EXACT and "debug image" in why, why
+    img.debug = None
+    U.splitScreen, U.splitX = 1, 20
+    U.param[1].mVelBasedAlpha = 1
+    (a, b), why = query()
+    assert (a, b) == (taastar.KERNEL_FP16, taastar.KERNEL_EXACT) and "mVelBasedAlpha (param[1])" in why, why
+    U.splitScreen = 0
+    img.input_width, img.input_height = 32, 16
+    (a, b), why = query()
+    assert a == taastar.KERNEL_UPSAMPLE and "upsampling" in why, why
+    img.input_width = img.input_height = 0
+    assert query()[0] == (taastar.KERNEL_FP16, taastar.KERNEL_FP16)
+    # arguments are checked as the resolve checks them
+    assert lib.taa_resolve_kernel(C.byref(U), C.byref(img), None, 7, k) == _abi.TAA_ERR_INVALID_ARG
+    assert lib.taa_resolve_kernel(C.byref(U), C.byref(img), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
+
+
 def test_velocity_argument_errors_need_no_gpu():
     lib = taastar.lib()
     R = taastar.Reproject()

@@ -54,7 +54,7 @@ def test_small_and_boundary_sizes_fp32(oracle, W, H, cas):
 def test_small_and_boundary_sizes_fp16(oracle, W, H, cas):
     print(f"case fp16 {W}x{H} cas={cas}", flush=True)
     seq = Sequence(workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F), width=W, height=H, cas=cas, arith=ARITH_FP16,
-                   debug=False)
+                   debug=False, shipped=True)
     for _ in range(3):
         out = seq.step(oracle)
         assert_fp16_indices(out)
@@ -154,7 +154,7 @@ def test_hostile_velocity_fp16(oracle):
     rejection decisions as the transcription's."""
     print("case hostile fp16", flush=True)
     seq = Sequence(workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F), width=123, height=77, cas=True, arith=ARITH_FP16,
-                   debug=False)
+                   debug=False, shipped=True)
     rng = np.random.default_rng(6)
     render = seq.scene.render
 

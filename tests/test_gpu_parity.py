@@ -120,6 +120,44 @@ def test_split_screen(oracle):
         assert_parity(out)
 
 
+@pytest.mark.parametrize("fmt,arith", [(FMT_RGBA32F, 0), (FMT_RGBA16F, 0), (FMT_RGBA16F, 1)])
+@pytest.mark.parametrize("split_x", [77, -1, 202])
+def test_split_screen_with_cas(oracle, fmt, arith, split_x):
+    """splitScreen with CAS in one call: both parts resolved, then one CAS pass over the whole
+    frame (taa.hpp:1008-1027, 1111-1139), against the oracle's resolve + CasFilter of the same
+    frame; splitX -1 puts every column on param[1], the last column every column on param[0]."""
+    import ctypes as C
+    import taastar
+    from gpu_helpers import FP16_PSNR_MIN_DB, INDEX_WORDS, psnr
+    from taastar.workloads import uniforms_for
+    print(f"case split+cas fmt={fmt} arith={arith} splitX={split_x}", flush=True)
+    w = workload("c2", fmt=fmt, vel=VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F)
+    seq = Sequence(w, width=203, height=81, cas=True, arith=arith, debug=arith == 0)
+    for f in range(3):
+        col, dep, vel4, jit = seq.scene.render(seq.frame)
+        U = uniforms_for(w, seq.frame, jit)
+        U.splitScreen = 1
+        U.splitX = split_x
+        U.param[1].mVarianceClipping = 0
+        U.param[1].mInterpolationMode = 0
+        U.param[1].mColorClampingOrClipping = 1
+        k = (C.c_int32 * 2)()
+        taastar.check(taastar.lib().taa_resolve_kernel(C.byref(U), C.byref(seq.bufs.images()), C.byref(seq.cas), arith, k))
+        assert tuple(k) == ((1, 1) if arith else (0, 0)), tuple(k)
+        out = seq.step(oracle, uniforms=U)
+        if arith == 0:
+            assert_parity(out)
+        else:
+            gi, oi = out["gpu_inspect"], out["oracle_inspect"]
+            for wd in INDEX_WORDS:
+                g, o = gi[..., wd], oi[..., wd]
+                if wd == 3:  # flags: "rectified" is a threshold on arithmetic
+                    g, o = g & ~2, o & ~2
+                assert np.count_nonzero(g != o) == 0, f"frame {f}: inspect word {wd}"
+            assert psnr(out["gpu_screen"][..., :3], out["oracle_screen"][..., :3]) > FP16_PSNR_MIN_DB
+            seq.hist_o = out["gpu_history"]
+
+
 def test_fused_cas_equals_separate_pass(oracle):
     """The fused CAS writes exactly what resolve + sharpen_cas.comp would."""
     from taastar.device import FrameBuffers, cas_apply

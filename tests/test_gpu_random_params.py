@@ -100,7 +100,8 @@ def test_random_parameter_block_fp16(oracle, seed):
     lin = float(rng.choice([0.0, 0.01]))
     print(f"case random fp16 seed={seed} fmt={fmt} cas={cas} linear={lin} {describe(p)}", flush=True)
     w = workload("c2", fmt=fmt, vel=vfmt, linear_depth=lin, params=lambda p=p: p)
-    seq = Sequence(w, width=128, height=80, cas=cas, arith=ARITH_FP16, seed=0x5EED1000 + seed, debug=False)
+    seq = Sequence(w, width=128, height=80, cas=cas, arith=ARITH_FP16, seed=0x5EED1000 + seed, debug=False,
+                   shipped=True)
     for f in range(3):
         out = seq.step(oracle)
         gi, oi = out["gpu_inspect"], out["oracle_inspect"]
@@ -145,14 +146,15 @@ def test_random_parameter_block_upsampling(oracle, seed):
 
 @pytest.mark.parametrize("seed", range(8))
 def test_random_split_screen(oracle, seed):
-    """Split screen with two random blocks (taa.comp:717-718) at a random split column."""
+    """Split screen with two random blocks (taa.comp:717-718) at a random split column; odd seeds
+    with CAS (both parts resolved, then one CAS pass over the frame)."""
     from taastar.workloads import uniforms_for
     rng = np.random.default_rng(0x7AA5F300 + seed)
     p0, p1 = random_params(rng), random_params(rng)
     split = int(rng.integers(1, 159))
     print(f"case random split seed={seed} splitX={split}\n  left  {describe(p0)}\n  right {describe(p1)}", flush=True)
     w = workload("c2", vel=VEL_RGBA32F)
-    seq = Sequence(w, width=160, height=64, cas=False, seed=0x5EED2000 + seed)
+    seq = Sequence(w, width=160, height=64, cas=seed % 2 == 1, seed=0x5EED2000 + seed)
     for f in range(3):
         col, dep, vel4, jit = seq.scene.render(seq.frame)
         U = uniforms_for(w, seq.frame, jit, params=p0)
@@ -335,7 +337,8 @@ def test_random_fp16_sizes_and_hostile_inputs(oracle, seed):
     cas = bool(rng.integers(2))
     print(f"case random fp16 hostile seed={seed} {W}x{H} fmt={fmt} cas={cas} {describe(p)}", flush=True)
     w = workload("c2", fmt=fmt, vel=vfmt, params=lambda p=p: p)
-    seq = Sequence(w, width=W, height=H, cas=cas, arith=ARITH_FP16, seed=0x5EED5000 + seed, debug=False)
+    seq = Sequence(w, width=W, height=H, cas=cas, arith=ARITH_FP16, seed=0x5EED5000 + seed, debug=False,
+                   shipped=True)
     render = seq.scene.render
 
     def hostile(frame, threads=0):

@@ -147,11 +147,13 @@ def test_segmentation_flag_variants(oracle, name):
         assert_seg_parity(seq.step(oracle))
 
 
-def test_segmentation_split_screen(oracle):
-    """Split screen without CAS: the left parameter block has the mask, the right one not."""
-    print("case seg split", flush=True)
+@pytest.mark.parametrize("cas", [False, True])
+def test_segmentation_split_screen(oracle, cas):
+    """Split screen: the left parameter block has the mask, the right one not; with CAS the frame
+    is sharpened after both parts are resolved."""
+    print(f"case seg split cas={cas}", flush=True)
     w = workload("c2")
-    seq = SegSequence(w, 200, 120, cas=False, params=seg_params(), split=97)
+    seq = SegSequence(w, 200, 120, cas=cas, params=seg_params(), split=97)
     for _ in range(3):
         assert_seg_parity(seq.step(oracle), split=97)
 
