@@ -304,6 +304,23 @@ int32_t taa_resolve_frame(const taa_uniforms_t* uniforms, const taa_frame_images
 int32_t taa_resolve_kernel(const taa_uniforms_t* uniforms, const taa_frame_images_t* images,
                            const taa_cas_consts_t* cas, int32_t arith, int32_t kernels[2]);
 
+/* Bytes of the intermediate screen taa_resolve_frame_noalloc holds per device: 64 MiB, enough for
+ * pitch x height texels of 3840x2160 RGBA16F or 2560x1440 RGBA32F. */
+#define TAA_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* taa_resolve_frame for callers that cannot allocate device memory: code under HIP graph capture,
+ * serving loops and hosts that forbid hipMalloc. No counterpart in the reference (its effect owns
+ * its temp images). Identical results; the one difference is where a split screen with CAS keeps
+ * its intermediate screen: in a file-scope device array of TAA_NOALLOC_SCRATCH_BYTES that the
+ * library holds on every device it runs on until the process ends, never in allocated memory. A
+ * frame whose intermediate (pitch x height texels of color_format) exceeds it is refused with
+ * TAA_ERR_CAPACITY before anything is launched. Every call on one device shares that array and
+ * rewrites all of it that it reads, so nothing carries over between calls; calls that overlap on
+ * one device (other streams, other threads) race on it and must be serialised by the caller.
+ * Allocates nothing and calls no allocating routine. */
+int32_t taa_resolve_frame_noalloc(const taa_uniforms_t* uniforms, const taa_frame_images_t* images,
+                                  const taa_cas_consts_t* cas, int32_t arith, void* stream);
+
 /* taa_resolve_frame with the segmentation mask of mRayTraceAugment (taa.comp:589-702, 772-783,
  * 959): where a Parameters block has mRayTraceAugment set, each texel's mask value is computed
  * from the extra images, written to seg->segmask, and a non-zero category rejects the history

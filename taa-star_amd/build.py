@@ -33,6 +33,7 @@ SOURCES = [
     ("taa_fast.hip", "hip"),
     ("taa_velocity.hip", "hip"),
     ("taa_edge_aa.hip", "hip"),
+    ("taa_noalloc.hip", "hip"),
     ("taa_api.cpp", "cxx"),
     ("taa_host.cpp", "cxx"),
     ("taa_scene.cpp", "cxx"),

@@ -13,6 +13,7 @@ TAA_ERR_HIP = -4
 TAA_ERR_BUDGET = -5
 TAA_ERR_NO_DEVICE = -6
 TAA_ERR_CAPACITY = -7
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 FMT_RGBA32F, FMT_RGBA16F = 0, 1
 VEL_RG32F, VEL_RG16F, VEL_RGBA32F, VEL_RGBA16F = 0, 1, 2, 3
@@ -215,6 +216,7 @@ PROTOTYPES = {
     "taa_resolve_frame": (I32, [P(Uniforms), P(FrameImages), P(CasConsts), I32, VP]),
     "taa_resolve_frame_segmented": (I32, [P(Uniforms), P(FrameImages), P(SegmentationImages), P(CasConsts), I32, VP]),
     "taa_resolve_kernel": (I32, [P(Uniforms), P(FrameImages), P(CasConsts), I32, P(I32)]),
+    "taa_resolve_frame_noalloc": (I32, [P(Uniforms), P(FrameImages), P(CasConsts), I32, VP]),
     "taa_cas_apply": (I32, [VP, VP, I32, I32, I32, I32, P(CasConsts), VP]),
     "taa_settings_defaults": (I32, [P(Settings)]),
     "taa_settings_read_ini": (I32, [C.c_char_p, C.c_size_t, P(Settings)]),

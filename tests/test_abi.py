@@ -212,6 +212,23 @@ def test_resolve_kernel_query_needs_no_gpu():
     assert lib.taa_resolve_kernel(C.byref(U), C.byref(img), None, 0, None) == _abi.TAA_ERR_INVALID_ARG
 
 
+def test_noalloc_refuses_an_oversize_intermediate_without_a_gpu():
+    """taa_resolve_frame_noalloc refuses a split screen with CAS whose intermediate screen exceeds
+    its fixed array before it touches a device."""
+    lib = taastar.lib()
+    img = taastar.FrameImages()
+    for i, f in enumerate(("color", "depth", "history", "history_depth", "result_screen", "velocity", "result_history")):
+        setattr(img, f, 0x10000 * (i + 1))
+    img.width, img.height, img.pitch = 3840, 2160, 3840
+    img.color_format, img.velocity_format = taastar.FMT_RGBA32F, taastar.VEL_RG32F  # 16 B/texel: 133 MB
+    U = taastar.Uniforms()
+    U.splitScreen, U.splitX = 1, 100
+    K = taastar.cas_setup(0.5, 3840, 2160)
+    assert lib.taa_resolve_frame_noalloc(C.byref(U), C.byref(img), C.byref(K), 0, None) == _abi.TAA_ERR_CAPACITY
+    msg = lib.taa_last_error().decode()
+    assert "132710400" in msg and str(_abi.NOALLOC_SCRATCH_BYTES) in msg, msg
+
+
 def test_velocity_argument_errors_need_no_gpu():
     lib = taastar.lib()
     R = taastar.Reproject()
