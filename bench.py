@@ -110,15 +110,16 @@ def cpu_baseline(w, frames_host, seconds):
                        f"{rows} rows of one frame on one thread")
 
 
-def time_resolve(name, arith_name, steps, warmup, frames, autotune=False):
-    """K back-to-back single-frame resolves of workload `name` on a private stream, inputs resident
-    in HBM; HIP events between launches on that stream give the per-launch durations."""
+def time_resolve(name, arith_name, steps, warmup, frames, autotune=False, over=None):
+    """K back-to-back single-frame resolves of workload `name` (fields replaced by `over`) on a
+    private stream, inputs resident in HBM; HIP events between launches on that stream give the
+    per-launch durations."""
     import torch
     import taastar
     from taastar import FrameImages, check, lib
     from taastar.workloads import bytes_per_pixel, cas_for, encode_color, encode_velocity, scene_for, uniforms_for, workload
 
-    w = workload(name)
+    w = workload(name, **(over or {}))
     W, H = w["width"], w["height"]
     arith = taastar.ARITH_FP16 if arith_name == "fp16" else taastar.ARITH_FP32
     dev = torch.device("cuda:0")
@@ -187,9 +188,12 @@ def time_resolve(name, arith_name, steps, warmup, frames, autotune=False):
     px = W * H
     bpp = bytes_per_pixel(w)
     avg_ms = total_ms / K
+    kern = (C.c_int32 * 2)()
+    check(L.taa_resolve_kernel(C.byref(unis[0]), C.byref(imgs[0]), casp, arith, kern))
     r = dict(workload=name, width=W, height=H, arith=arith_name, steps=K, warmup=warmup,
              gpix_per_s=K * px / (total_ms * 1e-3) / 1e9, avg_ms=avg_ms, median_ms=per[K // 2],
              bytes_per_pixel=bpp, achieved_gbs=bpp * px / (avg_ms * 1e-3) / 1e9, wall_s=wall,
+             kernel={0: "exact", 1: "fp16", 2: "upsample"}[kern[0]],
              tile_shape=L.taa_tile_shape_selected(W, H, w["fmt"], arith, int(cas is not None)), w=w, host=host)
     del cols, deps, vels, hist, screen
     torch.cuda.empty_cache()
@@ -446,6 +450,12 @@ def extras(a):
         x[f"resolve_{name}_{r['width']}x{r['height']}_{ar}arith"] = {
             k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items() if k not in ("w", "host")}
         print(f"# {name} {ar}: {r['gpix_per_s']:.3f} Gpix/s", file=sys.stderr, flush=True)
+    # the reference's own defaults (taa.hpp:31-76: min/max box, clamp, bilinear history, movers-only
+    # velocity) with its RGBA16F velocity image (shader_cpu_common.h:77), at 4K on the FP16 kernel
+    r = time_resolve("c3", "fp16", 100, 5, 4, over=dict(vel=taastar.VEL_RGBA16F, params=taastar.default_params))
+    x["resolve_reference_defaults_3840x2160_rgba16f_velocity_fp16arith"] = {
+        k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items() if k not in ("w", "host")}
+    print(f"# reference defaults fp16: {r['gpix_per_s']:.3f} Gpix/s ({r['kernel']} kernel)", file=sys.stderr, flush=True)
     for vf, nobj in (("rg16f", 0), ("rgba16f", 3)):
         r = time_velocity(3840, 2160, vf, nobj, 200, 20)
         x[f"velocity_3840x2160_{vf}_{nobj}objects"] = r
@@ -501,13 +511,56 @@ def extras(a):
     # served load: 256 sessions, Poisson arrivals at 6000 frames/s, 10% churn per second
     x["served256"] = {
         "load": "256 sessions from {720p 40%, 1080p 35%, 1440p 15%, 4K 10%}, Poisson 6000 frames/s, churn 10%/s, "
-                "FP16 storage and arithmetic, fused CAS 0.5; sessions of a class share one pre-rendered input frame",
+                "FP16 storage and arithmetic, fused CAS 0.5; every session its own input frame (own scene, jitter "
+                "phase and pan of 0-24 px/frame, rendered before the run; churn admissions take a departed session's "
+                "frame of their class)",
         "budget_200GB": simulate(0, mix, 256, F16, A16, 2 * s, exact=False, rate=6000.0, churn=0.10,
-                                 budget=200 * 10**9, max_batch=256, own_inputs=False),
+                                 budget=200 * 10**9, max_batch=256, own_inputs=True),
         "budget_16GB": simulate(0, mix, 256, F16, A16, 2 * s, exact=False, rate=6000.0, churn=0.10,
-                                budget=16 * 10**9, max_batch=256, own_inputs=False),
+                                budget=16 * 10**9, max_batch=256, own_inputs=True),
     }
     return x
+
+
+def serving_summary(x, resolve_gpix):
+    """The 64-mixed-session comparison and the served load, for the parsed line."""
+    m = x["sessions64"]
+    served = x["served256"]["budget_200GB"]
+    ratio = m["served_over_per_frame_sync"]
+    return {
+        "sessions64": {
+            "mix": m["mix"],
+            "per_frame_sync_gpix_per_s": m["mode3"]["gpix_per_s"],
+            "per_frame_async_gpix_per_s": m["mode2"]["gpix_per_s"],
+            "batched_gpix_per_s": m["mode4"]["gpix_per_s"],
+            "served_gpix_per_s": m["mode1"]["gpix_per_s"],
+            "batched_over_per_frame_sync": m["batched_over_per_frame_sync"],
+            "served_over_per_frame_sync": ratio,
+            "batched_over_per_frame_async": m["batched_over_per_frame_async"],
+            "served_over_per_frame_async": m["served_over_per_frame_async"],
+            "served_lat_p50_ms": m["mode1"]["lat_p50_ms"],
+            "served_lat_p99_ms": m["mode1"]["lat_p99_ms"],
+            "per_frame_sync_lat_p99_ms": m["mode3"]["lat_p99_ms"],
+            "target_3x": {
+                "met": ratio >= 3.0,
+                "ratio": ratio,
+                "bound": "the per-frame launches of this mix (720p-4K) already run near the kernel's own rate "
+                         f"({m['mode2']['gpix_per_s']} async vs {round(resolve_gpix, 2)} Gpix/s for one 4K frame), "
+                         "so batching removes only launch gaps; the ratio is bound by the resolve kernel's "
+                         "throughput, and only a faster kernel raises the served figure",
+            },
+        },
+        "served256": {
+            "load": x["served256"]["load"],
+            "gpix_per_s": served["gpix_per_s"],
+            "frames_per_s": served["frames_per_s"],
+            "lat_p50_ms": served["lat_p50_ms"],
+            "lat_p99_ms": served["lat_p99_ms"],
+            "refused_rate": served["refused_rate"],
+            "refused_rate_16GB_budget": x["served256"]["budget_16GB"]["refused_rate"],
+            "lat_p99_ms_16GB_budget": x["served256"]["budget_16GB"]["lat_p99_ms"],
+        },
+    }
 
 
 def measured_copy_gbs(mib=1024, reps=20):
@@ -580,6 +633,7 @@ def main():
                         f"{'RG16F' if w['vel'] == 1 else 'RG32F'} velocity, D32F depth",
             "frame_pixels": W * H,
             "arith": a.arith,
+            "kernel": r["kernel"],
             "tile_shape": r["tile_shape"],
             "wall_s": round(r["wall_s"], 4),
             "median_step_ms": round(r["median_ms"], 5),
@@ -606,6 +660,7 @@ def main():
         del src, dst
     if not a.no_extras:
         out["extras"] = extras(a)
+        out.update(serving_summary(out["extras"], r["gpix_per_s"]))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, r["host"][:2], a.cpu_seconds)
     print(json.dumps(out), flush=True)

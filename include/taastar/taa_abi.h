@@ -59,9 +59,13 @@ enum { TAA_VEL_RG32F = 0, TAA_VEL_RG16F = 1, TAA_VEL_RGBA32F = 2, TAA_VEL_RGBA16
  *         CAS minimum / maximum in packed fp16), moments, rectification and the blend in fp32,
  *         hardware reciprocal / square root; texel indices and rejection decisions as in FP32 --
  *         the throughput path. It covers the variance or plain min/max box, Catmull-Rom or
- *         bilinear history, simple or closest-depth velocity, clamp or clip, reject-outside, raw
- *         or linear depth test, Lottes weighting and fused CAS, with RG velocity and no debug
- *         image; any other block is resolved by the FP32 path (taa_resolve_kernel tells which). */
+ *         bilinear history, every velocity use (matrix reprojection, movers only, all) and sample
+ *         mode, clamp or clip, reject-outside, dynamic anti-ghosting, raw or linear depth test,
+ *         velocity-based alpha, Lottes weighting, the near-clamp reduction, the debug image and
+ *         fused CAS, with RG or RGBA velocity of the colour storage's precision -- the reference's
+ *         default block with its RGBA16F velocity included. Shaped or chroma-shrunk boxes, slow
+ *         clip, B-spline history, Karis tone mapping, unjitter, noise, pass-through, bypass and
+ *         upsampling are resolved by the FP32 path (taa_resolve_kernel tells which). */
 enum { TAA_ARITH_FP32 = 0, TAA_ARITH_FP16 = 1 };
 /* the kernel a resolve runs (taa_resolve_kernel) */
 enum {

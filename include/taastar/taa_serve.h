@@ -103,10 +103,14 @@ typedef struct taa_serve_sim_config {
     int32_t  warmup_frames;     /* per session, before measuring (modes 1-4) */
     int32_t  exact_mix;         /* != 0: classes assigned in proportion to class_p (largest
                                    remainder) instead of drawn at random */
-    int32_t  per_session_inputs;/* != 0: every session renders its own input frame (own seed,
-                                   jitter phase = session index mod 8, camera pan of random
+    int32_t  per_session_inputs;/* != 0: every session gets an input frame of its own (own seed,
+                                   jitter phase = frame index mod 8, camera pan of random
                                    direction and magnitude uniform in [0, pan_max] px/frame)
-                                   instead of sharing one frame per resolution class */
+                                   instead of sharing one frame per resolution class. The frames
+                                   are rendered before the measured run (the initial sessions'
+                                   and a per-class reserve of 1.5x the expected churn admissions);
+                                   a departing session's frame is handed to the next admission of
+                                   its class, so active sessions never share an input */
     float    pan_max;
 } taa_serve_sim_config_t;
 

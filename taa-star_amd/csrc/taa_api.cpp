@@ -782,7 +782,7 @@ static int check_items(taa_ctx_t* c, int n, const taa_batch_item_t* items) {
 struct BatchLayout {
     int shape = 0;  // tile shape of the main launch
     bool cas = false, s16 = false, fast = false;
-    bool general = false;  // the FP16 kernel's general form (some block has a min/max box or bilinear history)
+    int form = 0;  // the FP16 kernel's form (taastar::fast_path_form): the largest over the frames
     int n_main = 0, total_main = 0;
     int n_up = 0, total_up = 0;
 };
@@ -830,7 +830,15 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
     }
     L.fast = arith == TAA_ARITH_FP16 && L.n_main > 0;
     for (int i = 0; i < L.n_main && L.fast; ++i) L.fast = taastar::fast_path_supported(ents[i].fd, ents[i].U, 0);
-    for (int i = 0; i < L.n_main && L.fast; ++i) L.general = L.general || taastar::fast_path_general(ents[i].U, 0);
+    // one launch, one staged velocity layout: RGBA-velocity frames (form 3) batch on the FP16 kernel
+    // only with each other
+    int n_rgba = 0;
+    for (int i = 0; i < L.n_main && L.fast; ++i) {
+        const int f = taastar::fast_path_form(ents[i].fd, ents[i].U, 0);
+        n_rgba += f == 3;
+        L.form = std::max(L.form, f);
+    }
+    if (L.fast && n_rgba != 0 && n_rgba != L.n_main) L.fast = false;
     return TAA_OK;
 }
 
@@ -838,7 +846,7 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
 static int launch_layout(const taastar::BatchEntry* d_ents, const BatchLayout& L, int arith, void* stream) {
     if (L.n_main) {
         const int grid = taastar::batch_grid(L.total_main);
-        int le = L.fast ? taastar::launch_batch_fast(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, L.general, grid,
+        int le = L.fast ? taastar::launch_batch_fast(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, L.form, grid,
                                                      stream)
                         : taastar::launch_batch(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, arith, grid, stream);
         if (le) return le;
@@ -914,7 +922,7 @@ void plan_free(taa_batch_plan* p) {
 }
 
 bool same_launches(const BatchLayout& a, const BatchLayout& b) {
-    return a.shape == b.shape && a.fast == b.fast && a.general == b.general && a.n_main == b.n_main &&
+    return a.shape == b.shape && a.fast == b.fast && a.form == b.form && a.n_main == b.n_main &&
            a.total_main == b.total_main &&
            a.n_up == b.n_up && a.total_up == b.total_up;
 }

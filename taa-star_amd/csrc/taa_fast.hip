@@ -24,20 +24,24 @@ namespace {
 
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
-template <bool S16> struct VelSlot;
-template <> struct VelSlot<true> { typedef uint32_t T; };  // RG16F bits
-template <> struct VelSlot<false> { typedef float2 T; };   // RG32F
+// staged velocity texel: RG16F / RG32F (VEL4 false) or RGBA16F / RGBA32F (VEL4 true), with the
+// colour storage's precision
+template <bool S16, bool VEL4> struct VelSlot;
+template <> struct VelSlot<true, false> { typedef uint32_t T; };  // RG16F bits
+template <> struct VelSlot<false, false> { typedef float2 T; };   // RG32F
+template <> struct VelSlot<true, true> { typedef uint2 T; };      // RGBA16F bits
+template <> struct VelSlot<false, true> { typedef float4 T; };    // RGBA32F
 
-template <int TW_, int TH_, bool CAS_, bool S16_>
+template <int TW_, int TH_, bool CAS_, bool S16_, bool VEL4_ = false>
 struct FGeo {
     static constexpr int TW = TW_, TH = TH_;
-    static constexpr bool CAS = CAS_, S16 = S16_;
+    static constexpr bool CAS = CAS_, S16 = S16_, VEL4 = VEL4_;
     static constexpr int NT = 256;
     static constexpr int A = CAS ? 2 : 1;
     static constexpr int SW = TW + 2 * A, SH = TH + 2 * A, SP = SW + 1;
     static constexpr int RA = CAS ? 1 : 0;
     static constexpr int RW = TW + 2 * RA, RH = TH + 2 * RA, RP = RW + 1;
-    typedef typename VelSlot<S16>::T VelT;
+    typedef typename VelSlot<S16, VEL4>::T VelT;
     // resolved screen texel kept for the fused CAS: the value the stored image would hold
     typedef typename std::conditional<S16, uint2, float4>::type ResT;
     static constexpr int NIT = (RW * RH + NT - 1) / NT;  // resolve iterations per thread
@@ -59,10 +63,26 @@ struct FGeo {
 __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-template <bool S16>
-__device__ __forceinline__ float2 vel_xy(typename VelSlot<S16>::T v) {
-    if constexpr (S16) return make_float2(h2f(v & 0xffffu), h2f(v >> 16));
-    else return v;
+template <bool S16, bool VEL4>
+__device__ __forceinline__ float2 vel_xy(typename VelSlot<S16, VEL4>::T v) {
+    if constexpr (VEL4) {
+        if constexpr (S16) return make_float2(h2f(v.x & 0xffffu), h2f(v.x >> 16));
+        else return make_float2(v.x, v.y);
+    } else {
+        if constexpr (S16) return make_float2(h2f(v & 0xffffu), h2f(v >> 16));
+        else return v;
+    }
+}
+// the whole texel as the resolve reads it: RG formats as (r, g, 0, 1)
+template <bool S16, bool VEL4>
+__device__ __forceinline__ float4 vel_xyzw(typename VelSlot<S16, VEL4>::T v) {
+    if constexpr (VEL4) {
+        if constexpr (S16) return make_float4(h2f(v.x & 0xffffu), h2f(v.x >> 16), h2f(v.y & 0xffffu), h2f(v.y >> 16));
+        else return v;
+    } else {
+        const float2 q = vel_xy<S16, false>(v);
+        return make_float4(q.x, q.y, 0.0f, 1.0f);
+    }
 }
 
 // Linear-depth rejection |L(hd) - L(d)| > thr with L(x) = n*f / (f + x*(n - f)), decided exactly as
@@ -91,10 +111,10 @@ __device__ __forceinline__ W4 cr_weights(float f) {
     return r;
 }
 
-// one texel of the history as three fp32 channels (fp16 storage feeds mixed FMAs)
+// one texel of the history as three (ALPHA: four) fp32 channels (fp16 storage feeds mixed FMAs)
 template <bool S16>
 struct Texel {
-    float r, g, b;
+    float r, g, b, a;
 };
 template <bool S16>
 __device__ __forceinline__ Texel<S16> load_texel(const void* base, uint32_t row_bytes, int i) {
@@ -102,19 +122,26 @@ __device__ __forceinline__ Texel<S16> load_texel(const void* base, uint32_t row_
         uint2 v = gload<uint2>(base, row_bytes, 8 * i);
         h2 rg = __builtin_bit_cast(h2, v.x);
         h2 ba = __builtin_bit_cast(h2, v.y);
-        return {(float)rg.x, (float)rg.y, (float)ba.x};
+        return {(float)rg.x, (float)rg.y, (float)ba.x, (float)ba.y};
     } else {
         float4 v = gload<float4>(base, row_bytes, 16 * i);
-        return {v.x, v.y, v.z};
+        return {v.x, v.y, v.z, v.w};
     }
 }
 
-// Catmull-Rom 4x4 history filter (separable, fp32 accumulation), rgb only.
-template <bool S16>
-__device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, int pitch, int kx, int ky, float fx, float fy) {
+// history sample: rgb and, when the dynamic anti-ghosting reads it, alpha (taa.comp:807)
+struct Hist {
+    f3 c;
+    float a;
+};
+
+// Catmull-Rom 4x4 history filter (separable, fp32 accumulation), rgb (ALPHA: and alpha).
+template <bool S16, bool ALPHA = false>
+__device__ __forceinline__ Hist history_cr_fast(const void* hist, int W, int H, int pitch, int kx, int ky, float fx, float fy) {
     constexpr uint32_t TB = S16 ? 8u : 16u;  // bytes per texel
     const W4 wx = cr_weights(fx), wy = cr_weights(fy);
     f3 acc = {0.f, 0.f, 0.f};
+    float acca = 0.f;
     // the whole 4x4 footprint inside the image. kx, ky come unclamped from to_int(floor(...)) and
     // saturate at INT_MIN / INT_MAX for huge or infinite history positions, so the test is written
     // without forming kx + 2 (which would wrap and pass for kx = INT_MAX)
@@ -125,6 +152,7 @@ __device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, in
             // byte offset of the row's first texel; the four texels follow as immediate offsets
             const uint32_t rb = ((uint32_t)(ky - 1 + j) * (uint32_t)pitch + (uint32_t)(kx - 1)) * TB;
             f3 row = {0.f, 0.f, 0.f};
+            float rowa = 0.f;
             Texel<S16> t4[4];
             if constexpr (S16) {  // the row's four fp16 texels as two 16-byte loads (8-byte aligned)
                 const u32x4_a8 lo = gload<u32x4_a8>(hist, rb, 0), hi = gload<u32x4_a8>(hist, rb, 16);
@@ -132,7 +160,7 @@ __device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, in
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const h2 rg = __builtin_bit_cast(h2, w8[2 * i]), ba = __builtin_bit_cast(h2, w8[2 * i + 1]);
-                    t4[i] = {(float)rg.x, (float)rg.y, (float)ba.x};
+                    t4[i] = {(float)rg.x, (float)rg.y, (float)ba.x, (float)ba.y};
                 }
             } else {
 #pragma unroll
@@ -144,10 +172,12 @@ __device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, in
                 row.x = __builtin_fmaf(t.r, wx.w[i], row.x);
                 row.y = __builtin_fmaf(t.g, wx.w[i], row.y);
                 row.z = __builtin_fmaf(t.b, wx.w[i], row.z);
+                if constexpr (ALPHA) rowa = __builtin_fmaf(t.a, wx.w[i], rowa);
             }
             acc.x = __builtin_fmaf(row.x, wy.w[j], acc.x);
             acc.y = __builtin_fmaf(row.y, wy.w[j], acc.y);
             acc.z = __builtin_fmaf(row.z, wy.w[j], acc.z);
+            if constexpr (ALPHA) acca = __builtin_fmaf(rowa, wy.w[j], acca);
         }
     } else {
         const int kxc = clampi(kx, -2, W + 1), kyc = clampi(ky, -2, H + 1);
@@ -158,25 +188,28 @@ __device__ __forceinline__ f3 history_cr_fast(const void* hist, int W, int H, in
         for (int j = 0; j < 4; ++j) {
             const uint32_t rb = (uint32_t)clampi(kyc - 1 + j, 0, H - 1) * (uint32_t)pitch;
             f3 row = {0.f, 0.f, 0.f};
+            float rowa = 0.f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 Texel<S16> t = load_texel<S16>(hist, (rb + (uint32_t)xs[i]) * TB, 0);
                 row.x = __builtin_fmaf(t.r, wx.w[i], row.x);
                 row.y = __builtin_fmaf(t.g, wx.w[i], row.y);
                 row.z = __builtin_fmaf(t.b, wx.w[i], row.z);
+                if constexpr (ALPHA) rowa = __builtin_fmaf(t.a, wx.w[i], rowa);
             }
             acc.x = __builtin_fmaf(row.x, wy.w[j], acc.x);
             acc.y = __builtin_fmaf(row.y, wy.w[j], acc.y);
             acc.z = __builtin_fmaf(row.z, wy.w[j], acc.z);
+            if constexpr (ALPHA) acca = __builtin_fmaf(rowa, wy.w[j], acca);
         }
     }
-    return acc;
+    return {acc, acca};
 }
 
 // Bilinear history (sample_history_rgba with mInterpolationMode 0, taa.comp:545-549): the 2x2 texels
 // at (kx, ky) .. (kx + 1, ky + 1), clamped to the edge, weights (1 - f, f); fp32 accumulation.
 template <bool S16>
-__device__ __forceinline__ f3 history_bilinear_fast(const void* hist, int W, int H, int pitch, int kx, int ky, float fx, float fy) {
+__device__ __forceinline__ Hist history_bilinear_fast(const void* hist, int W, int H, int pitch, int kx, int ky, float fx, float fy) {
     constexpr uint32_t TB = S16 ? 8u : 16u;
     const uint32_t xa = (uint32_t)clampi(kx, 0, W - 1), xb = (uint32_t)clampi(kx, -1, W - 2) + 1u;
     const uint32_t ya = (uint32_t)clampi(ky, 0, H - 1) * (uint32_t)pitch, yb = ((uint32_t)clampi(ky, -1, H - 2) + 1u) * (uint32_t)pitch;
@@ -185,7 +218,9 @@ __device__ __forceinline__ f3 history_bilinear_fast(const void* hist, int W, int
     const float ux = 1.0f - fx, uy = 1.0f - fy;
     const f3 top = {__builtin_fmaf(b.r, fx, a.r * ux), __builtin_fmaf(b.g, fx, a.g * ux), __builtin_fmaf(b.b, fx, a.b * ux)};
     const f3 bot = {__builtin_fmaf(d.r, fx, c.r * ux), __builtin_fmaf(d.g, fx, c.g * ux), __builtin_fmaf(d.b, fx, c.b * ux)};
-    return {__builtin_fmaf(bot.x, fy, top.x * uy), __builtin_fmaf(bot.y, fy, top.y * uy), __builtin_fmaf(bot.z, fy, top.z * uy)};
+    const float ta = __builtin_fmaf(b.a, fx, a.a * ux), ba = __builtin_fmaf(d.a, fx, c.a * ux);
+    return {{__builtin_fmaf(bot.x, fy, top.x * uy), __builtin_fmaf(bot.y, fy, top.y * uy), __builtin_fmaf(bot.z, fy, top.z * uy)},
+            __builtin_fmaf(ba, fy, ta * uy)};
 }
 
 // CAS (ffx_cas.h:424-537) with contraction: the same filter as cas_filter(), the four-neighbour
@@ -243,7 +278,12 @@ __device__ __forceinline__ uint2 cas_filter_h2(uint2 b, uint2 d, uint2 e, uint2 
 // chosen per launch from the parameter block); without it the kernel is the variance box with
 // Catmull-Rom history and no branch on either (the production configuration keeps the schedule
 // measured for it: the general form runs it 15% slower)
-template <class G, bool INSP, bool GEN>
+// EXT (implies GEN): the reference's other per-texel options -- RGBA velocity (.z into the
+// expected history depth, .w for movers only with matrix reprojection for the rest), the 3x3-longest
+// velocity quirk, dynamic anti-ghosting, velocity-based alpha, the near-clamp blend reduction and
+// the debug image -- in a form of their own, so that blocks without them keep the general form's
+// register budget
+template <class G, bool INSP, bool GEN, bool EXT>
 __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm) {
     constexpr bool S16 = G::S16;
     const taa_params_t& P = U.param[fd.param_idx];
@@ -353,12 +393,51 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         const float4 ctr = scd[l];
         const f3 cur = {ctr.x, ctr.y, ctr.z};
         const float depth = ctr.w;
-        if (!closest_mode) {  // velocity of the centre tap
-            vo = 0;
-            closest = -1;
+        const float uvx = sux[lx], uvy = suy[ly];
+        float hu, hv, expected = depth;  // history position and expected history depth (taa.comp:391-438)
+        bool used_velocity = true;
+        if (!EXT) {
+            if (!closest_mode) {  // velocity of the centre tap
+                vo = 0;
+                closest = -1;
+            }
+            const float2 v = vel_xy<S16, G::VEL4>(sv[l + vo]);
+            hu = uvx - v.x;  // taa.comp:421
+            hv = uvy - v.y;
+        } else {
+            // movers only (mode 1) reads .w of the centre tap; RG formats read w = 1
+            const float4 vc = vel_xyzw<S16, G::VEL4>(sv[l]);
+            used_velocity = !(P.mUseVelocityVectors == 0 || (P.mUseVelocityVectors == 1 && vc.w < 0.5f));
+            if (used_velocity) {
+                float4 vs = vc;
+                if (closest_mode) {
+                    vs = vel_xyzw<S16, G::VEL4>(sv[l + vo]);
+                } else {
+                    closest = -1;
+                    if (P.mVelocitySampleMode == 1) {  // taa.comp:398-412: the last tap's xy, not the longest
+                        const float2 q = vel_xy<S16, G::VEL4>(sv[l + G::SP + 1]);
+                        vs.x = q.x;
+                        vs.y = q.y;
+                    }
+                }
+                hu = uvx - vs.x;
+                hv = uvy - vs.y;
+                expected = depth - vs.z;
+            } else {  // reprojection through the matrices (taa.comp:424-431), IEEE divisions as the FP32 path
+                closest = -1;
+                const float* Mi = U.mInverseViewProjMatrix;
+                const float* Mh = U.mHistoryViewProjMatrix;
+                const float cs[4] = {uvx * 2.0f - 1.0f, uvy * 2.0f - 1.0f, depth, 1.0f};
+                float wv[4], hc[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) wv[r] = Mi[r] * cs[0] + Mi[4 + r] * cs[1] + Mi[8 + r] * cs[2] + Mi[12 + r] * cs[3];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) hc[r] = Mh[r] * wv[0] + Mh[4 + r] * wv[1] + Mh[8 + r] * wv[2] + Mh[12 + r] * wv[3];
+                hu = (hc[0] / hc[3]) * 0.5f + 0.5f;
+                hv = (hc[1] / hc[3]) * 0.5f + 0.5f;
+                expected = hc[2] / hc[3];
+            }
         }
-        const float2 v = vel_xy<S16>(sv[l + vo]);
-        const float hu = sux[lx] - v.x, hv = suy[ly] - v.y;  // taa.comp:421
 
         // previous depth under the history position (taa.comp:818), fetched ahead of the history
         // gather so that the two latencies overlap
@@ -374,9 +453,12 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         const float tX = hu * fW - 0.5f, tY = hv * fH - 0.5f;
         const float fbx = floorf(tX), fby = floorf(tY);
         const int kx = to_int(fbx), ky = to_int(fby);
-        const f3 hrgb = cr_hist ? history_cr_fast<S16>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby)
-                                : history_bilinear_fast<S16>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby);
+        const bool ghost = EXT && P.mDynamicAntiGhosting != 0;
+        const Hist hs = !cr_hist ? history_bilinear_fast<S16>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby)
+                        : ghost  ? history_cr_fast<S16, true>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby)
+                                 : history_cr_fast<S16, false>(fd.history, W, H, (int)pitch, kx, ky, tX - fbx, tY - fby);
         __builtin_amdgcn_s_setprio(0);
+        const f3 hrgb = hs.c;
         f3 h = hrgb;
         if (ycocg) {
             const float t = 0.25f * (hrgb.x + hrgb.z);
@@ -385,18 +467,32 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
 
         // ---- rejection (taa.comp:787-823), decisions as in the FP32 path
         bool rejected = false;
-        int flags = TAA_FLAG_USED_VELOCITY;
+        int flags = used_velocity ? TAA_FLAG_USED_VELOCITY : 0;
         if (reject_out && (hu < 0.0f || hv < 0.0f || hu >= 1.0f || hv >= 1.0f)) {
             rejected = true;
             flags |= TAA_FLAG_OUTSIDE;
+        }
+        float write_mask = 0.0f;  // history alpha: the dynamic mask of the anti-ghosting (taa.comp:796-811)
+        if (ghost) {
+            constexpr float eps = 1e-5f;
+            auto mov = [&](int o) {
+                const float4 q = vel_xyzw<S16, G::VEL4>(sv[l + o]);
+                return (fabsf(q.x) > eps || fabsf(q.y) > eps) && (fabsf(q.w) >= 0.5f);
+            };
+            const bool movC = mov(0);
+            if (!(mov(-1) || mov(1) || mov(-G::SP) || mov(G::SP) || movC) && hs.a > 0.0f) {
+                rejected = true;
+                flags |= TAA_FLAG_GHOST_REJECT;
+            }
+            write_mask = movC ? 1.0f : 0.0f;
         }
         if (depth_cull) {
             bool rej;
             if (lin_depth) {
                 const float n = U.mCamNearPlane, f = U.mCamFarPlane;
-                rej = linear_depth_reject(hd, depth, n, f, U.pad1);
+                rej = linear_depth_reject(hd, expected, n, f, U.pad1);
             } else {
-                rej = fabsf(hd - depth) > 0.1f * (1.0f - hd);
+                rej = fabsf(hd - expected) > 0.1f * (1.0f - hd);
             }
             if (rej) {
                 rejected = true;
@@ -418,6 +514,7 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
                 mx = vmax(mx, f3{t.x, t.y, t.z});
             }
         }
+        const f3 h_orig = h;
         if (rect == 2) {
             // clipAabb (taa.comp:323-345): ma = max_i |v_i| / e_i, clip when ma > 1, with one
             // reciprocal: ma = M / D for M = max_i |v_i| * prod_{j != i} e_j and D = e_x e_y e_z
@@ -437,15 +534,32 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         }
 
         // ---- blend (taa.comp:850-900)
+        auto luma = [&](f3 c) { return ycocg ? c.x : 0.25f * (c.x + c.z) + 0.5f * c.y; };
+        float pixel_speed = 0.0f;
+        if (EXT) {
+            const float vux = uvx - hu, vuy = uvy - hv;
+            pixel_speed = sqrt_fast(vux * vux + vuy * vuy);  // taa.comp:434-435
+        }
         float alpha = P.mAlpha;
         if (rejected) {
             alpha = P.mRejectionAlpha;
-        } else if (lottes) {
-            const float lc = ycocg ? cur.x : 0.25f * (cur.x + cur.z) + 0.5f * cur.y;
-            const float lh = ycocg ? h.x : 0.25f * (h.x + h.z) + 0.5f * h.y;
-            const float diff = fabsf(lc - lh) * rcp_fast(fmaxf(fmaxf(lc, lh), 0.2f));
-            const float w = 1.0f - diff;
-            alpha = __builtin_fmaf(w * w, P.mMinAlpha - P.mMaxAlpha, P.mMaxAlpha);
+        } else {
+            if (EXT && P.mVelBasedAlpha) {  // taa.comp:855-859
+                const float t = fminf(fmaxf(pixel_speed * P.mVelBasedAlphaFactor, 0.0f), 1.0f);
+                alpha = fmaxf(alpha, __builtin_fmaf(t, P.mVelBasedAlphaMax - alpha, alpha));
+            }
+            if (lottes) {
+                const float lc = luma(cur), lh = luma(h);
+                const float diff = fabsf(lc - lh) * rcp_fast(fmaxf(fmaxf(lc, lh), 0.2f));
+                const float w = 1.0f - diff;
+                alpha = __builtin_fmaf(w * w, P.mMinAlpha - P.mMaxAlpha, P.mMaxAlpha);
+            }
+            if (EXT && P.mReduceBlendNearClamp) {  // taa.comp:873-894 (the Unreal-based form)
+                const float mnl = luma(mn), mxl = luma(mx), hl = luma(h_orig);
+                float dist = 2.0f * fabsf(fminf(hl - mnl, mxl - hl)) * rcp_fast(mxl - mnl);
+                if (mxl - mnl < 0.001f) dist = 1.0f;
+                alpha *= fminf(fmaxf(4.0f * dist, 0.0f), 1.0f);
+            }
         }
         if (reset) alpha = 1.0f;
         f3 aa = {__builtin_fmaf(alpha, cur.x - h.x, h.x), __builtin_fmaf(alpha, cur.y - h.y, h.y), __builtin_fmaf(alpha, cur.z - h.z, h.z)};
@@ -461,11 +575,36 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
         if constexpr (S16) hb = make_uint2(pack_f2h(aa.x, aa.y), f2h(aa.z));
         if (owned) {
             if constexpr (S16) {
-                gstore<uint2>(fd.history_out, gi * 8u, hb);
+                // history alpha = the dynamic mask (0 or 1: 0x3c00 in fp16); the screen's alpha is 1
+                gstore<uint2>(fd.history_out, gi * 8u, make_uint2(hb.x, hb.y | (write_mask != 0.0f ? 0x3c000000u : 0u)));
                 if constexpr (!G::CAS) gstore<uint2>(fd.screen, gi * 8u, make_uint2(hb.x, hb.y | 0x3c000000u));
             } else {
-                st_rgba<S16>(fd.history_out, gi, make_float4(aa.x, aa.y, aa.z, 0.0f));
+                st_rgba<S16>(fd.history_out, gi, make_float4(aa.x, aa.y, aa.z, write_mask));
                 if constexpr (!G::CAS) st_rgba<S16>(fd.screen, gi, make_float4(aa.x, aa.y, aa.z, 1.0f));
+            }
+            if (EXT && fd.debug) {  // taa.comp:912-942 (mode 8: no mask here, category 0)
+                float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+                switch (P.mDebugMode) {
+                    case 0: g = make_float4(mx.x - mn.x, mx.y - mn.y, mx.z - mn.z, 0.f); break;
+                    case 1: { const float q = (mx.x - mn.x) * (mx.y - mn.y) * (mx.z - mn.z); g = make_float4(q, q, q, 0.f); break; }
+                    case 2: {
+                        const f3 d = h - h_orig;
+                        g = make_float4(rejected ? 1.f : 0.f, sqrt_fast(d.x * d.x + d.y * d.y + d.z * d.z), 0.f, 0.f);
+                        break;
+                    }
+                    case 3: g = make_float4(alpha, alpha, alpha, 0.f); break;
+                    case 4: g = vel_xyzw<S16, G::VEL4>(sv[l]); break;
+                    case 5: g = make_float4(pixel_speed, 0.f, 0.f, 0.f); break;
+                    case 6: g = make_float4(aa.x, aa.y, aa.z, 1.0f); break;
+                    case 7: g = make_float4(aa.x, aa.y, aa.z, write_mask); break;
+                    case 8: g = make_float4(0.f, 0.f, 1.f, 0.f); break;
+                    default: break;
+                }
+                const float ds = P.mDebugScale;
+                g = make_float4(g.x * ds * P.mDebugMask[0], g.y * ds * P.mDebugMask[1], g.z * ds * P.mDebugMask[2],
+                                g.w * ds * P.mDebugMask[3]);
+                if (P.mDebugCenter) g = make_float4(g.x * 0.5f + 0.5f, g.y * 0.5f + 0.5f, g.z * 0.5f + 0.5f, g.w * 0.5f + 0.5f);
+                st_rgba16(fd.debug, gi, g);
             }
             if constexpr (INSP) {
                 if (rejected) flags |= TAA_FLAG_REJECTED;
@@ -535,17 +674,17 @@ struct FastArgs {
     taa_uniforms_t U;
 };
 
-template <int TW, int TH, bool CAS, bool S16, bool INSP, bool GEN>
-__global__ __launch_bounds__(256) void k_resolve_fast(const FastArgs a) {
-    using G = FGeo<TW, TH, CAS, S16>;
+template <int TW, int TH, bool CAS, bool S16, bool INSP, int FORM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FORM == 2 ? 5 : 1))) void k_resolve_fast(const FastArgs a) {
+    using G = FGeo<TW, TH, CAS, S16, FORM == 3>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
-    fast_tile<G, INSP, GEN>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
+    fast_tile<G, INSP, FORM != 0, FORM >= 2>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
 }
 
-template <int TW, int TH, bool CAS, bool S16, bool GEN>
+template <int TW, int TH, bool CAS, bool S16, int FORM>
 __global__ __launch_bounds__(256) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
-    using G = FGeo<TW, TH, CAS, S16>;
+    using G = FGeo<TW, TH, CAS, S16, FORM == 3>;
     __shared__ typename G::Smem sm;
     const int per = (total_tiles + 7) >> 3;
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
@@ -553,12 +692,15 @@ __global__ __launch_bounds__(256) void k_resolve_fast_batch(const BatchEntry* __
         const int t = xcd * per + j;
         if (t >= total_tiles) break;
         const BatchEntry& e = batch_entry(entries, n_entries, t);
-        fast_tile<G, false, GEN>(e.fd, e.U, t - e.tile_begin, sm);
+        fast_tile<G, false, FORM != 0, FORM >= 2>(e.fd, e.U, t - e.tile_begin, sm);
         __syncthreads();
     }
 }
 
-template <bool CAS, bool S16, bool INSP, bool GEN>
+// FORM: 0 the production form (variance box, Catmull-Rom, RG velocity, no branch on either),
+// 1 the general form (min/max box or bilinear history), 2 the extended form with RG velocity,
+// 3 the extended form with RGBA velocity
+template <bool CAS, bool S16, bool INSP, int FORM>
 hipError_t launch_fast_t(const FrameDesc& fd, const taa_uniforms_t& U, int shape, hipStream_t st) {
     FastArgs a;
     a.fd = fd;
@@ -566,34 +708,37 @@ hipError_t launch_fast_t(const FrameDesc& fd, const taa_uniforms_t& U, int shape
     const int n = fd.tiles_x * fd.tiles_y;
     constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
     switch (shape) {
-        case 0: hipLaunchKernelGGL((k_resolve_fast<S[0].tw, S[0].th, CAS, S16, INSP, GEN>), dim3(n), dim3(256), 0, st, a); break;
-        case 1: hipLaunchKernelGGL((k_resolve_fast<S[1].tw, S[1].th, CAS, S16, INSP, GEN>), dim3(n), dim3(256), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((k_resolve_fast<S[2].tw, S[2].th, CAS, S16, INSP, GEN>), dim3(n), dim3(256), 0, st, a); break;
-        default: hipLaunchKernelGGL((k_resolve_fast<S[3].tw, S[3].th, CAS, S16, INSP, GEN>), dim3(n), dim3(256), 0, st, a); break;
+        case 0: hipLaunchKernelGGL((k_resolve_fast<S[0].tw, S[0].th, CAS, S16, INSP, FORM>), dim3(n), dim3(256), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_resolve_fast<S[1].tw, S[1].th, CAS, S16, INSP, FORM>), dim3(n), dim3(256), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_resolve_fast<S[2].tw, S[2].th, CAS, S16, INSP, FORM>), dim3(n), dim3(256), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_resolve_fast<S[3].tw, S[3].th, CAS, S16, INSP, FORM>), dim3(n), dim3(256), 0, st, a); break;
     }
     return hipGetLastError();
 }
 
-template <bool CAS, bool S16, bool GEN>
+template <bool CAS, bool S16, int FORM>
 hipError_t launch_fast_batch_t(const BatchEntry* e, int n, int total, int shape, int grid, hipStream_t st) {
     constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
     switch (shape) {
-        case 0: hipLaunchKernelGGL((k_resolve_fast_batch<S[0].tw, S[0].th, CAS, S16, GEN>), dim3(grid), dim3(256), 0, st, e, n, total); break;
-        case 1: hipLaunchKernelGGL((k_resolve_fast_batch<S[1].tw, S[1].th, CAS, S16, GEN>), dim3(grid), dim3(256), 0, st, e, n, total); break;
-        case 2: hipLaunchKernelGGL((k_resolve_fast_batch<S[2].tw, S[2].th, CAS, S16, GEN>), dim3(grid), dim3(256), 0, st, e, n, total); break;
-        default: hipLaunchKernelGGL((k_resolve_fast_batch<S[3].tw, S[3].th, CAS, S16, GEN>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 0: hipLaunchKernelGGL((k_resolve_fast_batch<S[0].tw, S[0].th, CAS, S16, FORM>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 1: hipLaunchKernelGGL((k_resolve_fast_batch<S[1].tw, S[1].th, CAS, S16, FORM>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 2: hipLaunchKernelGGL((k_resolve_fast_batch<S[2].tw, S[2].th, CAS, S16, FORM>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        default: hipLaunchKernelGGL((k_resolve_fast_batch<S[3].tw, S[3].th, CAS, S16, FORM>), dim3(grid), dim3(256), 0, st, e, n, total); break;
     }
     return hipGetLastError();
 }
+
+bool rgba_velocity(const FrameDesc& fd) { return fd.vel_fmt == TAA_VEL_RGBA16F || fd.vel_fmt == TAA_VEL_RGBA32F; }
 
 }  // namespace
 
 bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx, const char** why) {
     const taa_params_t& P = U.param[param_idx];
+    const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
     const char* r = nullptr;
-    if (fd.vel_fmt != (fd.color_fmt == TAA_FMT_RGBA16F ? TAA_VEL_RG16F : TAA_VEL_RG32F))
-        r = "velocity format other than RG16F (RGBA16F storage) or RG32F (RGBA32F storage)";
-    else if (fd.debug) r = "a debug image is bound";
+    if (s16 ? (fd.vel_fmt != TAA_VEL_RG16F && fd.vel_fmt != TAA_VEL_RGBA16F) : (fd.vel_fmt != TAA_VEL_RG32F && fd.vel_fmt != TAA_VEL_RGBA32F))
+        r = "velocity format of another precision than the colour storage (RG16F / RGBA16F with RGBA16F, RG32F / RGBA32F with "
+            "RGBA32F)";
     else if (fd.up || U.mUpsampling) r = "upsampling (input size != output size, or mUpsampling)";
     else if (U.mBypassHistoryUpdate) r = "mBypassHistoryUpdate";
     else if (P.mPassThrough) r = "mPassThrough";
@@ -603,25 +748,30 @@ bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param
     else if (P.mShrinkChromaAxis) r = "mShrinkChromaAxis";
     else if (P.mColorClampingOrClipping < 0 || P.mColorClampingOrClipping > 2) r = "mColorClampingOrClipping 3 (slow clip)";
     else if (P.mInterpolationMode != 2 && P.mInterpolationMode != 0) r = "B-spline history (mInterpolationMode 1)";
-    else if (P.mUseVelocityVectors == 0) r = "matrix reprojection (mUseVelocityVectors 0)";
-    else if (P.mVelocitySampleMode != 0 && P.mVelocitySampleMode != 2) r = "3x3-longest velocity (mVelocitySampleMode 1)";
-    else if (P.mDynamicAntiGhosting) r = "mDynamicAntiGhosting";
-    else if (P.mVelBasedAlpha) r = "mVelBasedAlpha";
-    else if (P.mReduceBlendNearClamp) r = "mReduceBlendNearClamp";
+    else if (P.mVelocitySampleMode < 0 || P.mVelocitySampleMode > 2) r = "unknown mVelocitySampleMode";
     else if (P.mAddNoise) r = "mAddNoise";
     if (why) *why = r;
     return r == nullptr;
 }
 
-bool fast_path_general(const taa_uniforms_t& U, int param_idx) {
+int fast_path_form(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx) {
     const taa_params_t& P = U.param[param_idx];
-    return !P.mVarianceClipping || P.mInterpolationMode != 2;
+    if (rgba_velocity(fd)) return 3;
+    // with RG velocity .w reads 1, so "movers only" (mode 1) uses every velocity, as mode 2 does
+    if (P.mUseVelocityVectors == 0 || P.mVelocitySampleMode == 1 || P.mDynamicAntiGhosting || P.mVelBasedAlpha ||
+        P.mReduceBlendNearClamp || fd.debug != nullptr)
+        return 2;
+    return (!P.mVarianceClipping || P.mInterpolationMode != 2) ? 1 : 0;
 }
 
 template <bool CAS, bool S16, bool INSP>
 hipError_t launch_fast_g(const FrameDesc& fd, const taa_uniforms_t& U, int shape, hipStream_t st) {
-    return fast_path_general(U, fd.param_idx) ? launch_fast_t<CAS, S16, INSP, true>(fd, U, shape, st)
-                                              : launch_fast_t<CAS, S16, INSP, false>(fd, U, shape, st);
+    switch (fast_path_form(fd, U, fd.param_idx)) {
+        case 0: return launch_fast_t<CAS, S16, INSP, 0>(fd, U, shape, st);
+        case 1: return launch_fast_t<CAS, S16, INSP, 1>(fd, U, shape, st);
+        case 2: return launch_fast_t<CAS, S16, INSP, 2>(fd, U, shape, st);
+        default: return launch_fast_t<CAS, S16, INSP, 3>(fd, U, shape, st);
+    }
 }
 
 int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape, void* stream) {
@@ -638,21 +788,24 @@ int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape,
     return (int)e;
 }
 
-int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, bool general,
+template <bool CAS, bool S16>
+hipError_t launch_fast_batch_f(const BatchEntry* e, int n, int total, int shape, int form, int grid, hipStream_t st) {
+    switch (form) {
+        case 0: return launch_fast_batch_t<CAS, S16, 0>(e, n, total, shape, grid, st);
+        case 1: return launch_fast_batch_t<CAS, S16, 1>(e, n, total, shape, grid, st);
+        case 2: return launch_fast_batch_t<CAS, S16, 2>(e, n, total, shape, grid, st);
+        default: return launch_fast_batch_t<CAS, S16, 3>(e, n, total, shape, grid, st);
+    }
+}
+
+int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, int form,
                       int grid, void* stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e;
-    if (general) {
-        if (cas) e = s16 ? launch_fast_batch_t<true, true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
-                         : launch_fast_batch_t<true, false, true>(d_entries, n_entries, total_tiles, shape, grid, st);
-        else e = s16 ? launch_fast_batch_t<false, true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
-                     : launch_fast_batch_t<false, false, true>(d_entries, n_entries, total_tiles, shape, grid, st);
-    } else {
-        if (cas) e = s16 ? launch_fast_batch_t<true, true, false>(d_entries, n_entries, total_tiles, shape, grid, st)
-                         : launch_fast_batch_t<true, false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
-        else e = s16 ? launch_fast_batch_t<false, true, false>(d_entries, n_entries, total_tiles, shape, grid, st)
-                     : launch_fast_batch_t<false, false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
-    }
+    if (cas) e = s16 ? launch_fast_batch_f<true, true>(d_entries, n_entries, total_tiles, shape, form, grid, st)
+                     : launch_fast_batch_f<true, false>(d_entries, n_entries, total_tiles, shape, form, grid, st);
+    else e = s16 ? launch_fast_batch_f<false, true>(d_entries, n_entries, total_tiles, shape, form, grid, st)
+                 : launch_fast_batch_f<false, false>(d_entries, n_entries, total_tiles, shape, form, grid, st);
     return (int)e;
 }
 

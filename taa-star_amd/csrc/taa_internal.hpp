@@ -98,11 +98,15 @@ int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles,
 // block is one it implements (else *why, when given, names the first reason it is not);
 // everything else runs the FP32 kernel.
 bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx, const char** why = nullptr);
-// whether a supported block needs the general form (plain min/max box or bilinear history)
-bool fast_path_general(const taa_uniforms_t& U, int param_idx);
+// the form of the FP16 kernel a supported block runs: 0 the production form (variance box,
+// Catmull-Rom history, RG velocity), 1 the general form (min/max box or bilinear history), 2 the
+// extended form (matrix reprojection, 3x3-longest velocity, anti-ghosting, velocity-based alpha,
+// near-clamp, debug image) with RG velocity, 3 the extended form with RGBA velocity (a batched
+// launch's frames must all have RGBA velocity or none)
+int fast_path_form(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx);
 int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape, void* stream);
 int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16,
-                      bool general, int grid, void* stream);
+                      int form, int grid, void* stream);
 
 // Edge-AA fallback on category-1 texels of a segmentation mask (taa_edge_aa.hip).
 int launch_edge_aa(const void* src, const uint32_t* mask, void* dst, int W, int H, int pitch, int seg_pitch, int fmt,

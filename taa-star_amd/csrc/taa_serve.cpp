@@ -529,6 +529,7 @@ double union_length(std::vector<std::pair<double, double>> iv, double lo, double
 struct SimSession {
     int sid;
     int cls;
+    int own = -1;  // index of the session's own input frame (per-session inputs), else -1
     uint64_t frame = 0;
     bool departing = false;
     float jitter[8][2];
@@ -557,6 +558,24 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         int rc = make_pool_frame(pool[i], cfg->class_w[i], cfg->class_h[i], cfg->color_format, cfg->seed ^ (0x7AA50005ull + i));
         if (rc) return rc;
     }
+    // per-session inputs: every admitted session gets an input frame of its own (own scene, pan and
+    // jitter phase), rendered before the measured run -- one per initial session plus a reserve per
+    // class for the sessions churn admits; a departing session's frame goes back to its class's
+    // free list, so the active sessions' inputs stay distinct and nothing is rendered while timing
+    std::vector<PoolFrame> own;
+    std::vector<std::vector<int>> own_free(cfg->n_classes);
+    uint64_t own_rendered = 0;
+    auto render_own = [&](int cls) -> int {
+        PoolFrame pf;
+        const float pan = (float)(U01(rng) * (double)std::max(0.0f, cfg->pan_max));
+        const uint64_t k = own_rendered++;
+        int r = make_pool_frame(pf, cfg->class_w[cls], cfg->class_h[cls], cfg->color_format,
+                                cfg->seed ^ (0x7AA50004ull + k), 1, pan, (int)(k % 8));
+        if (r) return r;
+        own.push_back(pf);
+        own_free[cls].push_back((int)own.size() - 1);
+        return 0;
+    };
     hipStream_t copy_st;
     hipStreamCreateWithFlags(&copy_st, hipStreamNonBlocking);
 
@@ -582,15 +601,15 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         if (r) return r;
         taa_session_io_t io;
         taa_server_io(srv, sid, &io, nullptr);
-        PoolFrame own;
-        if (cfg->per_session_inputs) {  // this session's own scene, pan and jitter phase
-            const int k = (int)out->admissions - 1;
-            const float pan = (float)(U01(rng) * (double)std::max(0.0f, cfg->pan_max));
-            int r = make_pool_frame(own, cfg->class_w[cls], cfg->class_h[cls], cfg->color_format,
-                                    cfg->seed ^ (0x7AA50004ull + (uint64_t)k), 1, pan, k % 8);
-            if (r) return r;
+        int fi = -1;
+        if (cfg->per_session_inputs) {  // a frame of its own (rendered now only if the reserve ran out)
+            if (own_free[cls].empty()) {
+                if (int r = render_own(cls)) return r;
+            }
+            fi = own_free[cls].back();
+            own_free[cls].pop_back();
         }
-        const PoolFrame& pf = cfg->per_session_inputs ? own : pool[cls];
+        const PoolFrame& pf = fi >= 0 ? own[fi] : pool[cls];
         const size_t px = (size_t)pf.w * pf.h;
         hipMemcpyAsync(io.color, pf.color, px * color_bpp(cfg->color_format), hipMemcpyDeviceToDevice, copy_st);
         hipMemcpyAsync(io.depth, pf.depth, px * 4, hipMemcpyDeviceToDevice, copy_st);
@@ -600,12 +619,8 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         SimSession x;
         x.sid = sid;
         x.cls = cls;
+        x.own = fi;
         std::memcpy(x.jitter, pf.jitter, sizeof(x.jitter));
-        if (cfg->per_session_inputs) {
-            hipFree(own.color);
-            hipFree(own.depth);
-            hipFree(own.vel);
-        }
         ss.push_back(x);
         out->sessions_peak = std::max<int32_t>(out->sessions_peak, (int32_t)ss.size());
         return 0;
@@ -629,6 +644,14 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
             rc = open_one(draw_class());
             if (rc) break;
         }
+    }
+    if (rc == 0 && cfg->per_session_inputs) {
+        // the initial sessions rendered their frames as they were admitted; now the reserve for the
+        // sessions churn admits: the expected number of churn events over the run, times 1.5,
+        // split by the class mix, so that no frame is rendered while the run is measured
+        const double events = cfg->churn_per_s * (double)cfg->sessions * cfg->duration_s * 1.5 + 4.0;
+        for (int i = 0; i < cfg->n_classes && rc == 0; ++i)
+            for (int j = 0; j < (int)std::ceil(events * cfg->class_p[i] / acc) && rc == 0; ++j) rc = render_own(i);
     }
     std::vector<double> lat;
     std::vector<taa_completion_t> comp(4096);
@@ -705,6 +728,7 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
                     if (!ss.empty()) {  // one session departs, a new one (class from the mix) asks for admission
                         const size_t k = (size_t)(U01(rng) * ss.size()) % ss.size();
                         const int sid = ss[k].sid;
+                        if (ss[k].own >= 0) own_free[ss[k].cls].push_back(ss[k].own);
                         ss.erase(ss.begin() + k);
                         taa_server_close(srv, sid);
                     }
@@ -835,10 +859,11 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
     out->sessions_end = (int32_t)ss.size();
     taa_server_destroy(srv);
     hipStreamDestroy(copy_st);
-    for (auto& pf : pool) {
-        hipFree(pf.color);
-        hipFree(pf.depth);
-        hipFree(pf.vel);
-    }
+    for (auto* v : {&pool, &own})
+        for (auto& pf : *v) {
+            hipFree(pf.color);
+            hipFree(pf.depth);
+            hipFree(pf.vel);
+        }
     return rc;
 }

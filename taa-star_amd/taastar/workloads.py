@@ -131,8 +131,11 @@ def bytes_per_pixel(w, cas=True):
     vb = {VEL_RG32F: 8, VEL_RG16F: 4, VEL_RGBA32F: 16, VEL_RGBA16F: 8}[w["vel"]]
     p = w["params"]()
     reads = cb + vb + cb  # colour, velocity, history
-    if p.mVelocitySampleMode == 2 or p.mDepthCulling:
-        reads += 4        # current depth
+    # the current depth: closest-depth velocity, the depth test, or matrix reprojection (mode 0, or
+    # mode 1 on an RGBA velocity image whose .w can send a texel there)
+    rgba = w["vel"] in (VEL_RGBA32F, VEL_RGBA16F)
+    if p.mVelocitySampleMode == 2 or p.mDepthCulling or p.mUseVelocityVectors == 0 or (p.mUseVelocityVectors == 1 and rgba):
+        reads += 4
     if p.mDepthCulling:
         reads += 4        # previous depth
     writes = cb + cb      # history, screen (sharpened when CAS is fused)

@@ -68,6 +68,16 @@ class Sequence:
         self.frame = 0
         self.torch = torch
 
+    def kernel(self, uniforms=None):
+        """(kernel of param[0]'s columns, of param[1]'s) the next step's resolve runs (taa_resolve_kernel)."""
+        import ctypes as C
+        from taastar import check, lib
+        U = uniforms if uniforms is not None else uniforms_for(self.w, self.frame, (0.0, 0.0), params=self.params)
+        k = (C.c_int32 * 2)()
+        check(lib().taa_resolve_kernel(C.byref(U), C.byref(self.bufs.images()), C.byref(self.cas) if self.cas else None,
+                                       self.arith, k))
+        return k[0], k[1]
+
     def step(self, oracle, run_oracle=True, uniforms=None):
         col, dep, vel4, jit = self.scene.render(self.frame)
         col_s = encode_color(col, self.fmt)

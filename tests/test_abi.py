@@ -189,18 +189,25 @@ def test_resolve_kernel_query_needs_no_gpu():
 
     assert query() == ((taastar.KERNEL_FP16, taastar.KERNEL_FP16), "")
     assert query(taastar.ARITH_FP32) == ((taastar.KERNEL_EXACT, taastar.KERNEL_EXACT), "")
-    img.velocity_format = taastar.VEL_RGBA16F  # the reference's default velocity format
+    # the reference's own defaults: RGBA16F velocity, movers only, a debug image bound
+    img.velocity_format = taastar.VEL_RGBA16F
+    img.debug = 0x90000
+    U.param[0] = taastar.default_params()
+    U.param[1] = taastar.default_params()
+    assert query() == ((taastar.KERNEL_FP16, taastar.KERNEL_FP16), "")
+    img.velocity_format = taastar.VEL_RGBA32F  # another precision than the RGBA16F colour
     (a, b), why = query()
     assert (a, b) == (taastar.KERNEL_EXACT, taastar.KERNEL_EXACT) and "velocity format" in why, why
     img.velocity_format = taastar.VEL_RG16F
-    img.debug = 0x90000
+    U.param[0].mToneMapLumaKaris = 1
     (a, b), why = query()
-    assert a == taastar.KERNEL_EXACT and "debug image" in why, why
+    assert a == taastar.KERNEL_EXACT and "mToneMapLumaKaris" in why, why
+    U.param[0].mToneMapLumaKaris = 0
     img.debug = None
     U.splitScreen, U.splitX = 1, 20
-    U.param[1].mVelBasedAlpha = 1
+    U.param[1].mShrinkChromaAxis = 1
     (a, b), why = query()
-    assert (a, b) == (taastar.KERNEL_FP16, taastar.KERNEL_EXACT) and "mVelBasedAlpha (param[1])" in why, why
+    assert (a, b) == (taastar.KERNEL_FP16, taastar.KERNEL_EXACT) and "mShrinkChromaAxis (param[1])" in why, why
     U.splitScreen = 0
     img.input_width, img.input_height = 32, 16
     (a, b), why = query()

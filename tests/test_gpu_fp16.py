@@ -125,3 +125,45 @@ def test_reference_default_block(oracle, fmt):
             assert np.count_nonzero(g != o) == 0, f"frame {f}: inspect word {wd}"
         assert psnr(out["gpu_screen"][..., :3], out["oracle_screen"][..., :3]) > 55.0, f"frame {f}"
         seq.hist_o = out["gpu_history"]
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("fmt", [FMT_RGBA16F, FMT_RGBA32F])
+def test_reference_defaults_rgba_velocity_and_debug_image(oracle, fmt, mode):
+    """The reference's defaults as they are shipped (taa.hpp:31-76 with IMAGE_FORMAT_VELOCITY =
+    RGBA16F, shader_cpu_common.h:77): movers-only velocity (.w), the background reprojected through
+    the matrices, .z in the expected history depth, a bound debug image -- on the FP16 kernel, not
+    the exact one. Indices, taps and decisions bit-exact against the FP32 transcription, the
+    sharpened screen > 55 dB against it, the debug image within 2e-3 relative (modes of the
+    neighbourhood, the decisions, the velocity and the speed) or 1e-2 on 99.9% of the texels (modes
+    6 and 7, the outputs themselves)."""
+    import ctypes as C
+    from gpu_helpers import Sequence, rel_err
+    from taastar import KERNEL_FP16, VEL_RGBA16F, VEL_RGBA32F, default_params
+    print(f"case fp16 reference defaults rgba velocity fmt={fmt} debug mode {mode}", flush=True)
+    p = default_params()
+    p.mDebugMode = mode
+    p.mDebugScale = 2.0
+    p.mDebugCenter = mode % 2
+    w = workload("c2", fmt=fmt, vel=VEL_RGBA16F if fmt == FMT_RGBA16F else VEL_RGBA32F, params=lambda: p)
+    seq = Sequence(w, width=240, height=136, cas=True, arith=ARITH_FP16, debug=True, shipped=True)
+    for f in range(4):
+        assert seq.kernel() == (KERNEL_FP16, KERNEL_FP16)
+        out = seq.step(oracle)
+        gi, oi = out["gpu_inspect"], out["oracle_inspect"]
+        for wd in INDEX_WORDS:
+            g, o = gi[..., wd], oi[..., wd]
+            if wd == INSPECT["FLAGS"]:
+                g, o = g & ~FLAG_RECTIFIED, o & ~FLAG_RECTIFIED
+            assert np.count_nonzero(g != o) == 0, f"frame {f}: inspect word {wd}"
+        assert psnr(out["gpu_screen"][..., :3], out["oracle_screen"][..., :3]) > 55.0, f"frame {f}"
+        g, o = out["gpu_debug"], out["oracle_debug"]
+        assert np.array_equal(np.isfinite(g), np.isfinite(o)), f"frame {f}: debug finiteness"
+        fin = np.isfinite(o)
+        e = rel_err(g[fin], o[fin], floor=1e-2)
+        if mode in (6, 7):
+            assert np.quantile(e, 0.999) <= 1e-2, f"frame {f}: debug mode {mode}"
+        else:
+            assert e.max() <= 2e-3, f"frame {f}: debug mode {mode}: {e.max()}"
+        seq.hist_o = out["gpu_history"]
+

@@ -19,8 +19,8 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 from gpu_helpers import INDEX_WORDS, Sequence, assert_parity, psnr  # noqa: E402
-from taastar import (ARITH_FP16, FMT_RGBA16F, FMT_RGBA32F, INSPECT, VEL_RG16F, VEL_RG32F, VEL_RGBA16F,  # noqa: E402
-                     VEL_RGBA32F)
+from taastar import (ARITH_FP16, FMT_RGBA16F, FMT_RGBA32F, INSPECT, KERNEL_FP16, VEL_RG16F, VEL_RG32F,  # noqa: E402
+                     VEL_RGBA16F, VEL_RGBA32F)
 from taastar import FLAG_RECTIFIED  # noqa: E402
 from taastar.workloads import params_c2, workload  # noqa: E402
 
@@ -45,11 +45,15 @@ def random_params(rng, fast_only=False):
     p.mColorClampingOrClipping = int(rng.integers(3))
     p.mVelocitySampleMode = int(rng.choice([0, 2]))
     p.mUseVelocityVectors = int(rng.choice([1, 2]))
-    if fast_only:
-        p.mUseVelocityVectors = 2
+    if fast_only:  # everything but shaped / chroma-shrunk boxes, slow clip, B-spline, Karis, unjitter, noise
+        p.mUseVelocityVectors = int(rng.integers(3))  # 0: matrix reprojection, 1: movers only
+        p.mVelocitySampleMode = int(rng.integers(3))  # 1: the 3x3-longest quirk
         p.mVarianceClipping = b()  # off: the plain min/max box
         p.mShapedNeighbourhood = 0
         p.mInterpolationMode = int(rng.choice([0, 2]))  # bilinear or Catmull-Rom history
+        p.mReduceBlendNearClamp = b()
+        p.mVelBasedAlpha = b()
+        p.mDynamicAntiGhosting = b()
         return p
     p.mVarianceClipping = b()
     p.mShapedNeighbourhood = b()
@@ -90,19 +94,21 @@ def test_random_parameter_block_fp32(oracle, seed):
         assert_parity(seq.step(oracle))
 
 
-@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("seed", range(16))
 def test_random_parameter_block_fp16(oracle, seed):
     rng = np.random.default_rng(0x7AA5F100 + seed)
     p = random_params(rng, fast_only=True)
     fmt = int(rng.choice([FMT_RGBA32F, FMT_RGBA16F]))
-    vfmt = VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F
+    rgba = bool(rng.integers(2))  # RG or RGBA velocity of the colour storage's precision
+    vfmt = (VEL_RGBA16F if rgba else VEL_RG16F) if fmt == FMT_RGBA16F else (VEL_RGBA32F if rgba else VEL_RG32F)
     cas = bool(rng.integers(2))
     lin = float(rng.choice([0.0, 0.01]))
-    print(f"case random fp16 seed={seed} fmt={fmt} cas={cas} linear={lin} {describe(p)}", flush=True)
+    print(f"case random fp16 seed={seed} fmt={fmt} vfmt={vfmt} cas={cas} linear={lin} {describe(p)}", flush=True)
     w = workload("c2", fmt=fmt, vel=vfmt, linear_depth=lin, params=lambda p=p: p)
     seq = Sequence(w, width=128, height=80, cas=cas, arith=ARITH_FP16, seed=0x5EED1000 + seed, debug=False,
                    shipped=True)
     for f in range(3):
+        assert seq.kernel() == (KERNEL_FP16, KERNEL_FP16), "every drawn block runs on the FP16 kernel"
         out = seq.step(oracle)
         gi, oi = out["gpu_inspect"], out["oracle_inspect"]
         for wd in INDEX_WORDS:
