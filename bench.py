@@ -339,9 +339,9 @@ def time_segmented_resolve(W, H, steps, warmup):
                 without_mask_avg_ms=round(a_plain, 5))
 
 
-def time_taau(lo, hi, steps, warmup):
+def time_taau(lo, hi, steps, warmup, arith="fp32"):
     """The upsampling resolve (mUpsampling, input size lo, output size hi), RGBA16F storage, the c3
-    feature set and fused CAS: exact FP32 arithmetic (the upsampling kernel)."""
+    feature set and fused CAS, in the given arithmetic (the upsampling kernel)."""
     import numpy as np
     import torch
     import taastar
@@ -371,7 +371,8 @@ def time_taau(lo, hi, steps, warmup):
     img.input_width, img.input_height, img.input_pitch = Wl, Hl, Wl
     img.color_format, img.velocity_format = w["fmt"], w["vel"]
     L = taastar.lib()
-    avg, med = time_stream(lambda st: L.taa_resolve_frame(C.byref(U), C.byref(img), C.byref(K), taastar.ARITH_FP32,
+    ar = taastar.ARITH_FP16 if arith == "fp16" else taastar.ARITH_FP32
+    avg, med = time_stream(lambda st: L.taa_resolve_frame(C.byref(U), C.byref(img), C.byref(K), ar,
                                                           C.c_void_p(st.cuda_stream)), steps, warmup)
     # inputs once at the input size (colour 8, depth 4, velocity 4, previous depth 4), history in
     # and out and the screen at the output size (8 each)
@@ -468,11 +469,12 @@ def extras(a):
     x["resolve_segmented_1920x1080_fp32"] = r
     print(f"# segmented resolve 1080p: {r['avg_ms']:.4f} ms (without mask {r['without_mask_avg_ms']:.4f} ms)",
           file=sys.stderr, flush=True)
-    for lo in ((1920, 1080), (2560, 1440)):
-        r = time_taau(lo, (3840, 2160), 100, 10)
-        x[f"taau_{r['input']}_to_{r['output']}_rgba16f_fp32arith"] = r
-        print(f"# TAAU {r['input']} -> {r['output']}: {r['gpix_per_s']:.2f} Gpix/s ({r['avg_ms']:.4f} ms)",
-              file=sys.stderr, flush=True)
+    for ar in ("fp16", "fp32"):
+        for lo in ((1920, 1080), (2560, 1440)):
+            r = time_taau(lo, (3840, 2160), 100, 10, arith=ar)
+            x[f"taau_{r['input']}_to_{r['output']}_rgba16f_{ar}arith"] = r
+            print(f"# TAAU {ar} {r['input']} -> {r['output']}: {r['gpix_per_s']:.2f} Gpix/s ({r['avg_ms']:.4f} ms)",
+                  file=sys.stderr, flush=True)
     for cls in RESOLUTIONS:  # autotune every class the mixed loads use (FP16 and FP32 storage)
         W, H = RESOLUTIONS[cls]
         tune(W, H, F16, A16, True)
@@ -563,6 +565,22 @@ def serving_summary(x, resolve_gpix):
     }
 
 
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB (the known-size dispatch that scales rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes); returns its size in bytes, 0 for none."""
+    if mib <= 0:
+        return 0
+    import torch
+    n = mib * (1 << 20)
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    return n
+
+
 def measured_copy_gbs(mib=1024, reps=20):
     """Device-to-device copy rate (bytes read + written per second) of a 1 GiB buffer: the practical
     HBM ceiling on this device, reported beside the 8 TB/s peak."""
@@ -599,7 +617,9 @@ def main():
         return
     if a.taau_only:
         for lo in ((1920, 1080), (2560, 1440)):
-            print(json.dumps(dict(kernel="k_resolve_up", **time_taau(lo, (3840, 2160), a.steps, a.warmup))), flush=True)
+            print(json.dumps(dict(kernel="k_resolve_up", arith=a.arith, **time_taau(lo, (3840, 2160), a.steps, a.warmup,
+                                                                                   arith=a.arith))), flush=True)
+        copy_calibration(a.copy_calib)
         return
     if a.batched_only:
         import taastar
@@ -650,14 +670,7 @@ def main():
     }
     out["roofline"]["measured_copy_gbs"] = measured_copy_gbs()
     if a.copy_calib > 0:
-        n = a.copy_calib * (1 << 20)
-        src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        out["config"]["copy_calib_bytes"] = n
-        del src, dst
+        out["config"]["copy_calib_bytes"] = copy_calibration(a.copy_calib)
     if not a.no_extras:
         out["extras"] = extras(a)
         out.update(serving_summary(out["extras"], r["gpix_per_s"]))

@@ -71,7 +71,8 @@ enum { TAA_ARITH_FP32 = 0, TAA_ARITH_FP16 = 1 };
 enum {
     TAA_KERNEL_EXACT = 0,    /* FP32 arithmetic in taa.comp's operation order (k_resolve)      */
     TAA_KERNEL_FP16 = 1,     /* the FP16-arithmetic throughput kernel (k_resolve_fast)         */
-    TAA_KERNEL_UPSAMPLE = 2  /* the exact FP32 upsampling resolve (k_resolve_up)               */
+    TAA_KERNEL_UPSAMPLE = 2, /* the exact FP32 upsampling resolve (k_resolve_up)               */
+    TAA_KERNEL_UPSAMPLE_FP16 = 3 /* the upsampling resolve with the FP16 path's arithmetic      */
 };
 
 /* ---- reference structs ------------------------------------------------------------------- */

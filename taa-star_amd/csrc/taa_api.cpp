@@ -164,10 +164,7 @@ int check_uniforms(const taa_uniforms_t* u, bool segmented = false) {
 // an FP16 request the FP16 kernel does not cover records the reason in *why.
 int kernel_for(const taastar::FrameDesc& fd, const taa_uniforms_t& U, int param_idx, int arith, const char** why) {
     *why = nullptr;
-    if (fd.up || U.mUpsampling) {
-        if (arith == TAA_ARITH_FP16) *why = "upsampling (exact FP32 upsampling kernel)";
-        return TAA_KERNEL_UPSAMPLE;
-    }
+    if (fd.up || U.mUpsampling) return arith == TAA_ARITH_FP16 ? TAA_KERNEL_UPSAMPLE_FP16 : TAA_KERNEL_UPSAMPLE;
     if (arith == TAA_ARITH_FP16 && taastar::fast_path_supported(fd, U, param_idx, why)) return TAA_KERNEL_FP16;
     return TAA_KERNEL_EXACT;
 }
@@ -200,7 +197,7 @@ int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, in
         const taastar::TileShapeDef d = shape_def(cas ? taastar::kUpShapeCas : taastar::kUpShapePlain, cas);
         fd.tiles_x = taastar::tiles_for(x_end - x_begin, d.tw);
         fd.tiles_y = taastar::tiles_for(fd.H, d.th);
-        const int e = taastar::launch_resolve_up(fd, U, stream);
+        const int e = taastar::launch_resolve_up(fd, U, arith, stream);
         if (e != 0) return hip_fail((hipError_t)e, "resolve launch");
         return TAA_OK;
     }
@@ -852,8 +849,8 @@ static int launch_layout(const taastar::BatchEntry* d_ents, const BatchLayout& L
         if (le) return le;
     }
     if (L.n_up)
-        return taastar::launch_batch_up(d_ents + L.n_main, L.n_up, L.total_up, L.cas, L.s16, taastar::batch_grid(L.total_up),
-                                        stream);
+        return taastar::launch_batch_up(d_ents + L.n_main, L.n_up, L.total_up, L.cas, L.s16, arith,
+                                        taastar::batch_grid(L.total_up), stream);
     return 0;
 }
 

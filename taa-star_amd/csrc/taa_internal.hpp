@@ -76,8 +76,9 @@ inline int tiles_for(int extent, int t) { return (extent + t - 1) / t; }
 
 // Launchers (taa_kernels.hip). All return hipError_t as int.
 int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int arith, void* stream);
-// the upsampling (TAAU) path: exact FP32 arithmetic, default tile shape
-int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, void* stream);
+// the upsampling (TAAU) path, default tile shape: exact FP32 arithmetic, or with TAA_ARITH_FP16 the
+// FP16 path's where no selection depends on it
+int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, int arith, void* stream);
 // the resolve with the segmentation mask (exact FP32 arithmetic, default tile shape)
 int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, void* stream);
 int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uint32_t peak_bits, void* stream);
@@ -89,8 +90,9 @@ inline int batch_grid(int total_tiles) { return (total_tiles + 7) & ~7; }
 
 int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas,
                  bool s16, int arith, int grid, void* stream);
-// the upsampling frames of a batch (exact FP32 upsampling resolve, kUpShapeCas / kUpShapePlain tiles)
-int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int grid,
+// the upsampling frames of a batch (the upsampling resolve in the batch's arithmetic, kUpShapeCas /
+// kUpShapePlain tiles)
+int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int arith, int grid,
                     void* stream);
 
 // Throughput kernel (taa_fast.hip): variance clip + Catmull-Rom + simple/closest-depth velocity

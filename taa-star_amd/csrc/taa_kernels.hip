@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "taa_device.hpp"
+#include "taa_fast_math.hpp"
 #include "taa_internal.hpp"
 
 namespace taastar {
@@ -170,7 +171,12 @@ __device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, c
 
 // One output texel of taa.comp main (taa.comp:707-960), FP32 arithmetic. `l` indexes the texel
 // in the staged LDS planes (row pitch SP). SEG: the segmentation mask's images are in `sg`.
-template <bool S16, int SP, bool UP, bool SEG = false, bool STG = false, bool VEL16 = false>
+// FA: the FP16 path's arithmetic (TAA_ARITH_FP16) where the bits of a selection do not depend on
+// it -- fp32 moments with fused multiply-adds and the hardware square root, the Catmull-Rom and
+// bilinear history filters with fp16 texels in mixed FMAs, one reciprocal for the clip, Lottes and
+// the near-clamp ratio, the blend as one FMA per channel; every texel index, tap choice and
+// rejection decision stays that of the FP32 path (used by the upsampling resolve's FP16 form)
+template <bool S16, int SP, bool UP, bool SEG = false, bool STG = false, bool VEL16 = false, bool FA = false>
 __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
                                          int x, int y, int l, const float* __restrict__ s0,
                                          const float* __restrict__ s1, const float* __restrict__ s2,
@@ -266,7 +272,23 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         }
     }
     f3 colMin, colMax, clipTowards;
-    if (P.mVarianceClipping) {
+    if (FA && P.mVarianceClipping) {
+        f3 m1 = c[0], m2 = c[0] * c[0];
+#pragma unroll
+        for (int k = 1; k < 9; ++k) {
+            m1 = m1 + c[k];
+            m2 = {__builtin_fmaf(c[k].x, c[k].x, m2.x), __builtin_fmaf(c[k].y, c[k].y, m2.y), __builtin_fmaf(c[k].z, c[k].z, m2.z)};
+        }
+        constexpr float inv9 = 1.0f / 9.0f;
+        const f3 mean = m1 * inv9;
+        const f3 sigma = {sqrt_fast(fmaxf(__builtin_fmaf(-mean.x, mean.x, m2.x * inv9), 0.0f)),
+                          sqrt_fast(fmaxf(__builtin_fmaf(-mean.y, mean.y, m2.y * inv9), 0.0f)),
+                          sqrt_fast(fmaxf(__builtin_fmaf(-mean.z, mean.z, m2.z * inv9), 0.0f))};
+        const float g = P.mVarClipGamma;
+        colMin = {__builtin_fmaf(-g, sigma.x, mean.x), __builtin_fmaf(-g, sigma.y, mean.y), __builtin_fmaf(-g, sigma.z, mean.z)};
+        colMax = {__builtin_fmaf(g, sigma.x, mean.x), __builtin_fmaf(g, sigma.y, mean.y), __builtin_fmaf(g, sigma.z, mean.z)};
+        clipTowards = mean;
+    } else if (P.mVarianceClipping) {
 
         f3 m1 = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7] + c[8];
         f3 m2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + c[3] * c[3] + c[4] * c[4] + c[5] * c[5] + c[6] * c[6] +
@@ -385,7 +407,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         }
     }
     const float vux = uvx - hu, vuy = uvy - hv;
-    const float pixelSpeed = sqrtf(vux * vux + vuy * vuy);
+    const float pixelSpeed = FA ? sqrt_fast(vux * vux + vuy * vuy) : sqrtf(vux * vux + vuy * vuy);
 
     // the previous depth under the history position (taa.comp:818), fetched here, ahead of the
     // history gather, so that the two loads overlap; the test itself keeps its place below
@@ -399,7 +421,16 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
 
     // ---- sample_history_rgba (taa.comp:545-549)
     float4 historyRaw;
-    if (P.mInterpolationMode == 0) historyRaw = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, hu, hv);
+    if (FA && P.mInterpolationMode != 1) {
+        // the base texel floor(uv * size - 0.5) of the FP32 path (taa.comp:457-458); the weights from its fraction
+        const float tX = hu * (float)W - 0.5f, tY = hv * (float)H - 0.5f;
+        const float fbx = floorf(tX), fby = floorf(tY);
+        const int kx = to_int(fbx), ky = to_int(fby);
+        const Hist h = P.mInterpolationMode == 0 ? history_bilinear_fast<S16>(fd.history, W, H, fd.pitch, kx, ky, tX - fbx, tY - fby)
+                       : P.mDynamicAntiGhosting ? history_cr_fast<S16, true>(fd.history, W, H, fd.pitch, kx, ky, tX - fbx, tY - fby)
+                                                : history_cr_fast<S16, false>(fd.history, W, H, fd.pitch, kx, ky, tX - fbx, tY - fby);
+        historyRaw = make_float4(h.c.x, h.c.y, h.c.z, h.a);
+    } else if (P.mInterpolationMode == 0) historyRaw = bilinear_rgba<S16>(fd.history, W, H, fd.pitch, hu, hv);
     else if (P.mInterpolationMode == 1) historyRaw = history_bspline<S16>(fd, hu, hv);
     else if (P.mDynamicAntiGhosting) historyRaw = history_catmullrom<S16, true>(fd, hu, hv);
     else historyRaw = history_catmullrom<S16, false>(fd, hu, hv);
@@ -450,7 +481,8 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         bool rej;
         if (U.pad1 > 0.0f) {
             float n = U.mCamNearPlane, f = U.mCamFarPlane;
-            rej = fabsf(linearize_depth(historyDepth, n, f) - linearize_depth(expectedHistoryDepth, n, f)) > U.pad1;
+            rej = FA ? linear_depth_reject(historyDepth, expectedHistoryDepth, n, f, U.pad1)
+                     : fabsf(linearize_depth(historyDepth, n, f) - linearize_depth(expectedHistoryDepth, n, f)) > U.pad1;
         } else {
             float depthEpsilon = 0.1f * (1.0f - historyDepth);
             rej = fabsf(historyDepth - expectedHistoryDepth) > depthEpsilon;
@@ -464,6 +496,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     // ---- rectification (taa.comp:826-845)
     const f3 origHistoryColor = historyColor;
     if (P.mColorClampingOrClipping == 1) historyColor = vmin(vmax(historyColor, colMin), colMax);
+    else if (FA && P.mColorClampingOrClipping == 2) historyColor = clip_aabb_fast(colMin, colMax, historyColor);
     else if (P.mColorClampingOrClipping == 2) historyColor = clipAabb(colMin, colMax, historyColor);
     else if (P.mColorClampingOrClipping == 3) historyColor = clipAabbSlow(colMin, colMax, clipTowards, historyColor);
     const f3 rdiff = historyColor - origHistoryColor;
@@ -478,14 +511,15 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
             alpha = fmaxf(alpha, gmix(alpha, P.mVelBasedAlphaMax, gclamp(pixelSpeed * P.mVelBasedAlphaFactor, 0.0f, 1.0f)));
         if (P.mLumaWeightingLottes) {
             float lc = luminance(P, currentColor), lh = luminance(P, historyColor);
-            float diff = fabsf(lc - lh) / fmaxf(fmaxf(lc, lh), 0.2f);
+            float diff = FA ? fabsf(lc - lh) * rcp_fast(fmaxf(fmaxf(lc, lh), 0.2f)) : fabsf(lc - lh) / fmaxf(fmaxf(lc, lh), 0.2f);
             float w = 1.0f - diff;
             float ww = w * w;
             alpha = gmix(P.mMaxAlpha, P.mMinAlpha, ww);
         }
         if (P.mReduceBlendNearClamp) {
             float mnl = luminance(P, colMin), mxl = luminance(P, colMax), hl = luminance(P, origHistoryColor);
-            float distToClamp = 2.0f * fabsf(fminf(hl - mnl, mxl - hl)) / (mxl - mnl);
+            float distToClamp = FA ? 2.0f * fabsf(fminf(hl - mnl, mxl - hl)) * rcp_fast(mxl - mnl)
+                                   : 2.0f * fabsf(fminf(hl - mnl, mxl - hl)) / (mxl - mnl);
             if (mxl - mnl < 0.001f) distToClamp = 1.0f;
             alpha *= gclamp(4.0f * distToClamp, 0.0f, 1.0f);
         }
@@ -493,6 +527,10 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     if (U.mResetHistory) { alpha = 1.0f; beta = 1.0f; }
     const float ab = alpha * beta;
     f3 aa = {gmix(historyColor.x, currentColor.x, ab), gmix(historyColor.y, currentColor.y, ab), gmix(historyColor.z, currentColor.z, ab)};
+    if (FA)
+        aa = {__builtin_fmaf(ab, currentColor.x - historyColor.x, historyColor.x),
+              __builtin_fmaf(ab, currentColor.y - historyColor.y, historyColor.y),
+              __builtin_fmaf(ab, currentColor.z - historyColor.z, historyColor.z)};
     if (P.mUseYCoCg) aa = ycocg_to_rgb(aa);
     if (P.mAddNoise) {  // taa.comp:551-556 (GLSL sin(): outside the parity contract)
         float sx = uvx + U.mSinTime[0] + 0.6959174f, sy = uvy + U.mSinTime[0] + 0.6959174f;
@@ -556,7 +594,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
 // ------------------------------------------------------------------------------------------
 // one tile
 
-template <class G, bool S16, bool SEG = G::SEG>
+template <class G, bool S16, bool SEG = G::SEG, bool FA = false>
 __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm,
                                              const SegDesc* sg = nullptr) {
     const taa_params_t& P = U.param[fd.param_idx];
@@ -682,10 +720,10 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
             const int xl = to_int(uvx * (float)fd.Wl), yl = to_int(uvy * (float)fd.Hl);
             const bool in = ui.w > 0 && xl - 2 >= ui.x0 && xl + 2 < ui.x0 + ui.w && yl - 2 >= ui.y0 && yl + 2 < ui.y0 + ui.h;
             if (in)
-                r = resolve_pixel_f32<S16, G::SP, true, false, true>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
+                r = resolve_pixel_f32<S16, G::SP, true, false, true, false, FA>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
                                                                      owned, sg, &spl, &ui, rx, ry);
             else
-                r = resolve_pixel_f32<S16, G::SP, true, false, false>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
+                r = resolve_pixel_f32<S16, G::SP, true, false, false, false, FA>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
                                                                       owned, sg, &spl, &ui, rx, ry);
         } else {
             r = resolve_pixel_f32<S16, G::SP, false, SEG, false, G::VEL16>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
@@ -718,7 +756,8 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_unif
             if (px >= fd.x_end || py >= H) continue;
             const int li = (oy + 1) * G::RP + (ox + 1);
             auto at = [&](int k) { return f3{sm.r0[k], sm.r1[k], sm.r2[k]}; };
-            f3 o = cas_filter(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak);
+            f3 o = FA ? cas_filter_fma(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak)
+                      : cas_filter(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak);
             st_rgba<S16>(fd.screen, (uint32_t)(py * pitch + px), make_float4(o.x, o.y, o.z, 1.0f));
         }
     }
@@ -746,15 +785,15 @@ __global__ __launch_bounds__(256) void k_resolve_seg(const ResolveArgs a, const 
     resolve_tile<G, S16, true>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm, &sg);
 }
 
-// Upsampling path (input size != output size, or mUpsampling): exact FP32 arithmetic; the input
-// texels under the tile are staged in LDS (converted colour, depth, velocity), the rest of the
-// resolve as above, fused CAS from LDS.
-template <int TW, int TH, int PAD, bool CAS, bool S16>
-__global__ __launch_bounds__(256) void k_resolve_up(const ResolveArgs a) {
+// Upsampling path (input size != output size, or mUpsampling): exact FP32 arithmetic, or with FA
+// the FP16 path's (TAA_ARITH_FP16); the input texels under the tile are staged in LDS (converted
+// colour, depth, velocity), the rest of the resolve as above, fused CAS from LDS.
+template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FA && S16 ? 5 : 1))) void k_resolve_up(const ResolveArgs a) {
     using G = Geo<TW, TH, PAD, CAS, true>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
-    resolve_tile<G, S16>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
+    resolve_tile<G, S16, false, FA>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
 }
 
 // Batched resolve: the grid walks the flattened tile space of every frame in the batch with a
@@ -777,7 +816,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3
 
 // Batched upsampling resolve: the frames of a batch whose input size differs from the output size
 // (or that set mUpsampling), walked like k_resolve_batch, each tile as in k_resolve_up.
-template <int TW, int TH, int PAD, bool CAS, bool S16>
+template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA>
 __global__ __launch_bounds__(256) void k_resolve_up_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = Geo<TW, TH, PAD, CAS, true>;
     __shared__ typename G::Smem sm;
@@ -787,7 +826,7 @@ __global__ __launch_bounds__(256) void k_resolve_up_batch(const BatchEntry* __re
         const int t = xcd * per + j;
         if (t >= total_tiles) break;
         const BatchEntry& e = batch_entry(entries, n_entries, t);
-        resolve_tile<G, S16>(e.fd, e.U, t - e.tile_begin, sm);
+        resolve_tile<G, S16, false, FA>(e.fd, e.U, t - e.tile_begin, sm);
         __syncthreads();  // LDS is reused by the next tile
     }
 }
@@ -847,22 +886,27 @@ int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int 
     return (int)e;
 }
 
-int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, void* stream) {
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    ResolveArgs a;
-    a.fd = fd;
-    a.U = U;
+template <bool FA>
+hipError_t launch_resolve_up_t(const FrameDesc& fd, const ResolveArgs& a, hipStream_t st) {
     const int n = fd.tiles_x * fd.tiles_y;
     const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
     constexpr TileShapeDef C = kShapesCas[kUpShapeCas], Pl = kShapesPlain[kUpShapePlain];
     if (fd.cas_on) {
-        if (s16) hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, true>), dim3(n), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, false>), dim3(n), dim3(256), 0, st, a);
+        if (s16) hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, true, FA>), dim3(n), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, false, FA>), dim3(n), dim3(256), 0, st, a);
     } else {
-        if (s16) hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, true>), dim3(n), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, false>), dim3(n), dim3(256), 0, st, a);
+        if (s16) hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, true, FA>), dim3(n), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, false, FA>), dim3(n), dim3(256), 0, st, a);
     }
-    return (int)hipGetLastError();
+    return hipGetLastError();
+}
+
+int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, int arith, void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    ResolveArgs a;
+    a.fd = fd;
+    a.U = U;
+    return (int)(arith == TAA_ARITH_FP16 ? launch_resolve_up_t<true>(fd, a, st) : launch_resolve_up_t<false>(fd, a, st));
 }
 
 int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, void* stream) {
@@ -891,18 +935,25 @@ int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uin
     return (int)hipGetLastError();
 }
 
-int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int grid,
-                    void* stream) {
-    hipStream_t st = static_cast<hipStream_t>(stream);
+template <bool FA>
+hipError_t launch_batch_up_t(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int grid,
+                             hipStream_t st) {
     constexpr TileShapeDef C = kShapesCas[kUpShapeCas], Pl = kShapesPlain[kUpShapePlain];
     if (cas) {
-        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, true>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
-        else hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, false>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, true, FA>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        else hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, false, FA>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
     } else {
-        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, true>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
-        else hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, false>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, true, FA>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        else hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, false, FA>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
     }
-    return (int)hipGetLastError();
+    return hipGetLastError();
+}
+
+int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int arith, int grid,
+                    void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return (int)(arith == TAA_ARITH_FP16 ? launch_batch_up_t<true>(d_entries, n_entries, total_tiles, cas, s16, grid, st)
+                                         : launch_batch_up_t<false>(d_entries, n_entries, total_tiles, cas, s16, grid, st));
 }
 
 int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, int arith,

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import (ARITH_FP16, ARITH_FP32, KERNEL_EXACT, KERNEL_FP16, KERNEL_UPSAMPLE, FLAG_DEPTH_REJECT, FLAG_GHOST_REJECT, FLAG_OUTSIDE, FLAG_RECTIFIED,
+from ._abi import (ARITH_FP16, ARITH_FP32, KERNEL_EXACT, KERNEL_FP16, KERNEL_UPSAMPLE, KERNEL_UPSAMPLE_FP16, FLAG_DEPTH_REJECT, FLAG_GHOST_REJECT, FLAG_OUTSIDE, FLAG_RECTIFIED,
                    FLAG_REJECTED, FLAG_USED_VELOCITY, FMT_RGBA16F, FMT_RGBA32F, INSPECT, INSPECT_WORDS, LIB_PATH,
                    PROTOTYPES, VEL_RG16F, VEL_RG32F, VEL_RGBA16F, VEL_RGBA32F, BatchItem, CasConsts, FrameImages,
                    JitterConfig, Params, SceneConfig, ServeReport, ServerConfig, ServerStats, ServeSimConfig,
@@ -20,7 +20,7 @@ __all__ = [
     "BatchItem", "Settings", "read_settings", "write_settings", "EdgeAaConsts", "Reproject", "SegmentationImages", "RTFLAG", "SEG_TAA", "SEG_EDGE_AA", "SEG_RAY_TRACE", "TileShape", "SceneConfig", "ServerConfig", "ServerStats", "ServeSimConfig", "ServeReport",
     "Completion", "default_params", "cas_setup", "halton", "jitter_offset", "Scene",
     "FMT_RGBA32F", "FMT_RGBA16F", "VEL_RG32F", "VEL_RG16F", "VEL_RGBA32F", "VEL_RGBA16F", "ARITH_FP32",
-    "ARITH_FP16", "KERNEL_EXACT", "KERNEL_FP16", "KERNEL_UPSAMPLE", "INSPECT", "INSPECT_WORDS", "LIB_PATH", "PROTOTYPES",
+    "ARITH_FP16", "KERNEL_EXACT", "KERNEL_FP16", "KERNEL_UPSAMPLE", "KERNEL_UPSAMPLE_FP16", "INSPECT", "INSPECT_WORDS", "LIB_PATH", "PROTOTYPES",
 ]
 
 

@@ -18,7 +18,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20
 FMT_RGBA32F, FMT_RGBA16F = 0, 1
 VEL_RG32F, VEL_RG16F, VEL_RGBA32F, VEL_RGBA16F = 0, 1, 2, 3
 ARITH_FP32, ARITH_FP16 = 0, 1
-KERNEL_EXACT, KERNEL_FP16, KERNEL_UPSAMPLE = 0, 1, 2
+KERNEL_EXACT, KERNEL_FP16, KERNEL_UPSAMPLE, KERNEL_UPSAMPLE_FP16 = 0, 1, 2, 3
 
 INSPECT_WORDS = 16
 INSPECT = dict(HIST_TC_X=0, HIST_TC_Y=1, CLOSEST_TAP=2, FLAGS=3, COLMIN=4, COLMAX=7, BLEND=10,

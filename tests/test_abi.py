@@ -209,9 +209,9 @@ def test_resolve_kernel_query_needs_no_gpu():
     (a, b), why = query()
     assert (a, b) == (taastar.KERNEL_FP16, taastar.KERNEL_EXACT) and "mShrinkChromaAxis (param[1])" in why, why
     U.splitScreen = 0
-    img.input_width, img.input_height = 32, 16
-    (a, b), why = query()
-    assert a == taastar.KERNEL_UPSAMPLE and "upsampling" in why, why
+    img.input_width, img.input_height = 32, 16  # upsampling: its own kernel, in the FP16 path's arithmetic
+    assert query() == ((taastar.KERNEL_UPSAMPLE_FP16, taastar.KERNEL_UPSAMPLE_FP16), "")
+    assert query(taastar.ARITH_FP32)[0] == (taastar.KERNEL_UPSAMPLE, taastar.KERNEL_UPSAMPLE)
     img.input_width = img.input_height = 0
     assert query()[0] == (taastar.KERNEL_FP16, taastar.KERNEL_FP16)
     # arguments are checked as the resolve checks them

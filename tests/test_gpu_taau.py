@@ -1,8 +1,9 @@
 """The upsampling (TAAU) path on the GPU against the FP32 transcription: input (render) size
 different from the output size, and mUpsampling (taa.comp:222-257, 724, 750-755, 818).
 
-Bar: bit-exact like the FP32 path at one size — texel indices, depth-test texels, closest taps,
-flags and the neighbourhood box identical; history and screen within 1e-5 relative (observed 0).
+Bar: FP32 arithmetic bit-exact like the FP32 path at one size -- texel indices, depth-test texels,
+closest taps, flags and the neighbourhood box identical; history and screen within 1e-5 relative
+(observed 0). FP16 arithmetic: indices, taps and decisions identical, the screen > 55 dB.
 """
 import ctypes as C
 
@@ -66,6 +67,9 @@ class Upsampler:
         img.width, img.height, img.pitch = self.W, self.H, self.W
         img.input_width, img.input_height, img.input_pitch = self.Wl, self.Hl, self.Wl
         img.color_format, img.velocity_format = self.fmt, self.vfmt
+        k = (C.c_int32 * 2)()
+        check(lib().taa_resolve_kernel(C.byref(U), C.byref(img), C.byref(self.cas) if self.cas else None, self.arith, k))
+        self.last_kernel = (k[0], k[1])
         check(lib().taa_resolve_frame(C.byref(U), C.byref(img), C.byref(self.cas) if self.cas else None, self.arith,
                                       None))
         torch.cuda.synchronize()
@@ -133,12 +137,33 @@ def test_taau_feature_variants(oracle, name):
             assert np.array_equal(out["gpu_debug"], out["oracle_debug"])
 
 
-def test_taau_fp16_request_takes_the_exact_path(oracle):
-    """ARITH_FP16 on the upsampling path resolves with the FP32 arithmetic (bit-exact)."""
-    w = workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F)
-    up = Upsampler(w, (96, 54), (192, 108), arith=ARITH_FP16)
-    for _ in range(3):
-        assert_parity(up.step(oracle))
+@pytest.mark.parametrize("fmt", [FMT_RGBA32F, FMT_RGBA16F])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_taau_fp16_arithmetic(oracle, name, fmt):
+    """ARITH_FP16 on the upsampling path: the upsampling resolve with the FP16 path's arithmetic
+    (TAA_KERNEL_UPSAMPLE_FP16). Texel indices, depth-test texels, closest taps and decisions
+    bit-exact against the FP32 transcription; the screen > 55 dB against it (sharpened: peak 1;
+    unsharpened: its display-referred, clamped value); the oracle follows the GPU's history."""
+    from gpu_helpers import INDEX_WORDS, psnr
+    from taastar import FLAG_RECTIFIED, INSPECT, KERNEL_UPSAMPLE_FP16
+    c = dict(CASES[name])
+    print(f"case taau fp16 {name} fmt={fmt}", flush=True)
+    w = workload("c2", fmt=fmt, vel=VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F)
+    up = Upsampler(w, c.pop("lo"), c.pop("hi"), arith=ARITH_FP16, **c)
+    for f in range(5):
+        out = up.step(oracle)
+        assert up.last_kernel == (KERNEL_UPSAMPLE_FP16, KERNEL_UPSAMPLE_FP16)
+        gi, oi = out["gpu_inspect"], out["oracle_inspect"]
+        for wd in INDEX_WORDS:
+            g, o = gi[..., wd], oi[..., wd]
+            if wd == INSPECT["FLAGS"]:
+                g, o = g & ~FLAG_RECTIFIED, o & ~FLAG_RECTIFIED
+            assert np.count_nonzero(g != o) == 0, f"frame {f}: inspect word {wd}"
+        g, o = out["gpu_screen"][..., :3], out["oracle_screen"][..., :3]
+        if up.cas is None:
+            g, o = np.clip(g, 0, 1), np.clip(o, 0, 1)
+        assert psnr(g, o) > 55.0, f"frame {f}: screen PSNR {psnr(g, o):.2f} dB"
+        up.hist_o = out["gpu_history"]
 
 
 def test_scaled_session_equals_resolve_frame(oracle):
@@ -148,7 +173,7 @@ def test_scaled_session_equals_resolve_frame(oracle):
     from taastar import SessionIO
     print("case taau scaled session", flush=True)
     w = workload("c2", fmt=FMT_RGBA16F, vel=VEL_RG16F)
-    up = Upsampler(w, (96, 54), (192, 108))
+    up = Upsampler(w, (96, 54), (192, 108), arith=ARITH_FP16)  # the session below resolves in FP16 too
     ctx, s = C.c_void_p(), C.c_void_p()
     check(lib().taa_ctx_create(0, 0, C.byref(ctx)))
     check(lib().taa_session_open_scaled(ctx, 192, 108, 96, 54, w["fmt"], w["vel"], C.byref(s)))
