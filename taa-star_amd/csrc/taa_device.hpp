@@ -295,6 +295,17 @@ __device__ f3 current_upsampled(const float (&sx)[2], const bool (&hx)[2], const
     return f3{0.0f, 0.0f, 0.0f};
 }
 
+// RN(h * a) for the fp16 value h in the low (HI false) or high half of `bits`: one mixed FMA whose
+// f16 operand converts exactly and whose addend is -0 (x + -0 = x for every x, the sign of a zero
+// product included), so the same bits as converting h and then multiplying, in one instruction
+template <bool HI>
+__device__ __forceinline__ float mul_f16(uint32_t bits, float a) {
+    float r;
+    if constexpr (HI) asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(bits), "v"(a), "s"(-0.0f));
+    else asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(bits), "v"(a), "s"(-0.0f));
+    return r;
+}
+
 // Catmull-Rom history fetch (taa.comp:441-514), 16-tap texelFetch form with the edge clamp
 // at size-1; identical to the 9-tap bilinear form up to rounding. Every tap is
 // texel * w_x * w_y summed in the order of taa.comp:480-495. The alpha lane (read only by the
@@ -323,11 +334,22 @@ __device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
         ys[i] = (interior ? ky - 1 + i : clampi(ky - 1 + i, 0, H - 1)) * fd.pitch;
     }
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    auto add = [&](int i, int j, float4 t) {
-        const float a = wx[i], b = wy[j];
-        const float4 s = make_float4(t.x * a * b, t.y * a * b, t.z * a * b, ALPHA ? t.w * a * b : 0.0f);
+    // tap (i, j) given texel * wx[i] (each channel rounded): texel * wx[i] * wy[j], summed in order
+    auto accum = [&](int i, int j, float4 ta) {
+        const float b = wy[j];
+        const float4 s = make_float4(ta.x * b, ta.y * b, ta.z * b, ALPHA ? ta.w * b : 0.0f);
         if (i == 0 && j == 0) acc = s;
         else acc = make_float4(acc.x + s.x, acc.y + s.y, acc.z + s.z, ALPHA ? acc.w + s.w : 0.0f);
+    };
+    auto add = [&](int i, int j, float4 t) {
+        const float a = wx[i];
+        accum(i, j, make_float4(t.x * a, t.y * a, t.z * a, ALPHA ? t.w * a : 0.0f));
+    };
+    // the same from the texel's two fp16 dwords, the conversion folded into the first product
+    auto add16 = [&](int i, int j, uint32_t lo, uint32_t hi) {
+        const float a = wx[i];
+        accum(i, j, make_float4(mul_f16<false>(lo, a), mul_f16<true>(lo, a), mul_f16<false>(hi, a),
+                                ALPHA ? mul_f16<true>(hi, a) : 0.0f));
     };
     if (S16 && interior) {  // each row's four fp16 texels as two 16-byte loads, all eight issued first
         u32x4_a8 rlo[4], rhi[4];
@@ -343,10 +365,10 @@ __device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const u32x4_a8 lo = rlo[j], hi = rhi[j];
-            add(0, j, rgba16_bits(lo.x, lo.y));
-            add(1, j, rgba16_bits(lo.z, lo.w));
-            add(2, j, rgba16_bits(hi.x, hi.y));
-            add(3, j, rgba16_bits(hi.z, hi.w));
+            add16(0, j, lo.x, lo.y);
+            add16(1, j, lo.z, lo.w);
+            add16(2, j, hi.x, hi.y);
+            add16(3, j, hi.z, hi.w);
         }
         return acc;
     }

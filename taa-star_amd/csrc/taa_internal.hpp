@@ -76,6 +76,11 @@ inline int tiles_for(int extent, int t) { return (extent + t - 1) / t; }
 
 // Launchers (taa_kernels.hip). All return hipError_t as int.
 int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int arith, void* stream);
+// whether launch_resolve runs the exact kernel's production form for this frame (variance clip box,
+// clip, YCoCg, Catmull-Rom history, closest-depth RG velocity of the colour's precision, depth
+// culling, Lottes, outside rejection; no debug or inspect image): the general kernel's arithmetic
+// with the other branches compiled out
+bool exact_prod_form(const FrameDesc& fd, const taa_uniforms_t& U);
 // the upsampling (TAAU) path, default tile shape: exact FP32 arithmetic, or with TAA_ARITH_FP16 the
 // FP16 path's where no selection depends on it
 int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, int arith, void* stream);

@@ -594,10 +594,56 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
 // ------------------------------------------------------------------------------------------
 // one tile
 
-template <class G, bool S16, bool SEG = G::SEG, bool FA = false>
-__device__ __forceinline__ void resolve_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile, typename G::Smem& sm,
+// The production feature set of the exact kernel (variance clip box, clip toward its centre, YCoCg,
+// Catmull-Rom history, closest-depth velocity from an RG image of the colour's precision, depth
+// culling, Lottes luma weighting, outside rejection; no debug or inspect image) with its switches
+// as constants in local copies of the parameters, so that the compiler drops every other branch.
+// The arithmetic is the general kernel's, operation for operation. exact_prod_form() decides on the
+// host which launches take it.
+struct ProdFix {
+    FrameDesc fd;
+    taa_uniforms_t U;
+    taa_params_t P;
+};
+__device__ __forceinline__ void prod_fix(ProdFix& f, const FrameDesc& fd, const taa_uniforms_t& U, bool s16) {
+    f.fd = fd;
+    f.fd.debug = nullptr;
+    f.fd.inspect = nullptr;
+    f.fd.vel_fmt = s16 ? TAA_VEL_RG16F : TAA_VEL_RG32F;
+    f.U = U;
+    f.U.mBypassHistoryUpdate = 0;
+    f.P = U.param[fd.param_idx];
+    taa_params_t& p = f.P;
+    p.mPassThrough = 0;
+    p.mUnjitterNeighbourhood = 0;
+    p.mUnjitterCurrentSample = 0;
+    p.mVarianceClipping = 1;
+    p.mShapedNeighbourhood = 0;
+    p.mUseYCoCg = 1;
+    p.mShrinkChromaAxis = 0;
+    p.mUseVelocityVectors = 2;
+    p.mVelocitySampleMode = 2;
+    p.mDepthCulling = 1;
+    p.mInterpolationMode = 2;
+    p.mDynamicAntiGhosting = 0;
+    p.mRayTraceAugment = 0;
+    p.mColorClampingOrClipping = 2;
+    p.mVelBasedAlpha = 0;
+    p.mLumaWeightingLottes = 1;
+    p.mReduceBlendNearClamp = 0;
+    p.mRejectOutside = 1;
+    p.mAddNoise = 0;
+    p.mToneMapLumaKaris = 0;
+}
+
+template <class G, bool S16, bool SEG = G::SEG, bool FA = false, bool PROD = false>
+__device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_uniforms_t& U_in, int tile, typename G::Smem& sm,
                                              const SegDesc* sg = nullptr) {
-    const taa_params_t& P = U.param[fd.param_idx];
+    ProdFix pf;
+    if constexpr (PROD) prod_fix(pf, fd_in, U_in, S16);
+    const FrameDesc& fd = PROD ? pf.fd : fd_in;
+    const taa_uniforms_t& U = PROD ? pf.U : U_in;
+    const taa_params_t& P = PROD ? pf.P : U.param[fd.param_idx];
     const int W = fd.W, H = fd.H, pitch = fd.pitch;
     const int ty = tile / fd.tiles_x, tx = tile - ty * fd.tiles_x;
     const int x0 = fd.x_begin + tx * G::TW, y0 = ty * G::TH;
@@ -768,12 +814,13 @@ struct ResolveArgs {
     taa_uniforms_t U;
 };
 
-template <int TW, int TH, int PAD, bool CAS, bool S16>
+// PROD: the production feature set with its switches fixed (prod_fix)
+template <int TW, int TH, int PAD, bool CAS, bool S16, bool PROD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3))) void k_resolve(const ResolveArgs a) {
     using G = Geo<TW, TH, PAD, CAS, false, false, S16>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
-    resolve_tile<G, S16>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
+    resolve_tile<G, S16, false, false, PROD>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
 }
 
 // the resolve with the segmentation mask of mRayTraceAugment
@@ -846,7 +893,7 @@ __global__ __launch_bounds__(256) void k_cas(const void* __restrict__ src, void*
 // ------------------------------------------------------------------------------------------
 // dispatch tables
 
-template <bool CAS, bool S16>
+template <bool CAS, bool S16, bool PROD>
 hipError_t launch_resolve_t(const FrameDesc& fd, const taa_uniforms_t& U, int shape, hipStream_t st) {
     ResolveArgs a;
     a.fd = fd;
@@ -854,10 +901,10 @@ hipError_t launch_resolve_t(const FrameDesc& fd, const taa_uniforms_t& U, int sh
     const int n = fd.tiles_x * fd.tiles_y;
     constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
     switch (shape) {
-        case 0: hipLaunchKernelGGL((k_resolve<S[0].tw, S[0].th, S[0].pad, CAS, S16>), dim3(n), dim3(256), 0, st, a); break;
-        case 1: hipLaunchKernelGGL((k_resolve<S[1].tw, S[1].th, S[1].pad, CAS, S16>), dim3(n), dim3(256), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((k_resolve<S[2].tw, S[2].th, S[2].pad, CAS, S16>), dim3(n), dim3(256), 0, st, a); break;
-        default: hipLaunchKernelGGL((k_resolve<S[3].tw, S[3].th, S[3].pad, CAS, S16>), dim3(n), dim3(256), 0, st, a); break;
+        case 0: hipLaunchKernelGGL((k_resolve<S[0].tw, S[0].th, S[0].pad, CAS, S16, PROD>), dim3(n), dim3(256), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_resolve<S[1].tw, S[1].th, S[1].pad, CAS, S16, PROD>), dim3(n), dim3(256), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_resolve<S[2].tw, S[2].th, S[2].pad, CAS, S16, PROD>), dim3(n), dim3(256), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_resolve<S[3].tw, S[3].th, S[3].pad, CAS, S16, PROD>), dim3(n), dim3(256), 0, st, a); break;
     }
     return hipGetLastError();
 }
@@ -876,13 +923,30 @@ hipError_t launch_batch_t(const BatchEntry* e, int n, int total, int shape, int 
 
 }  // namespace
 
+bool exact_prod_form(const FrameDesc& fd, const taa_uniforms_t& U) {
+    const taa_params_t& p = U.param[fd.param_idx];
+    const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
+    return !fd.debug && !fd.inspect && !fd.up && !U.mUpsampling && !U.mBypassHistoryUpdate &&
+           fd.vel_fmt == (s16 ? TAA_VEL_RG16F : TAA_VEL_RG32F) && !p.mPassThrough && !p.mUnjitterNeighbourhood &&
+           !p.mUnjitterCurrentSample && p.mVarianceClipping && !p.mShapedNeighbourhood && p.mUseYCoCg &&
+           !p.mShrinkChromaAxis && p.mUseVelocityVectors == 2 && p.mVelocitySampleMode == 2 && p.mDepthCulling &&
+           p.mInterpolationMode == 2 && !p.mDynamicAntiGhosting && !p.mRayTraceAugment && p.mColorClampingOrClipping == 2 &&
+           !p.mVelBasedAlpha && p.mLumaWeightingLottes && !p.mReduceBlendNearClamp && p.mRejectOutside && !p.mAddNoise &&
+           !p.mToneMapLumaKaris;
+}
+
 int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int arith, void* stream) {
     (void)arith;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
     hipError_t e;
-    if (fd.cas_on) e = s16 ? launch_resolve_t<true, true>(fd, U, shape, st) : launch_resolve_t<true, false>(fd, U, shape, st);
-    else e = s16 ? launch_resolve_t<false, true>(fd, U, shape, st) : launch_resolve_t<false, false>(fd, U, shape, st);
+    if (exact_prod_form(fd, U)) {
+        if (fd.cas_on) e = s16 ? launch_resolve_t<true, true, true>(fd, U, shape, st) : launch_resolve_t<true, false, true>(fd, U, shape, st);
+        else e = s16 ? launch_resolve_t<false, true, true>(fd, U, shape, st) : launch_resolve_t<false, false, true>(fd, U, shape, st);
+    } else {
+        if (fd.cas_on) e = s16 ? launch_resolve_t<true, true, false>(fd, U, shape, st) : launch_resolve_t<true, false, false>(fd, U, shape, st);
+        else e = s16 ? launch_resolve_t<false, true, false>(fd, U, shape, st) : launch_resolve_t<false, false, false>(fd, U, shape, st);
+    }
     return (int)e;
 }
 

@@ -44,7 +44,7 @@ SIZES = [(1, 1), (2, 3), (1, 37), (37, 1), (5, 5), (29, 31), (30, 30), (31, 29),
 @pytest.mark.parametrize("W,H", SIZES)
 def test_small_and_boundary_sizes_fp32(oracle, W, H, cas):
     print(f"case fp32 {W}x{H} cas={cas}", flush=True)
-    seq = Sequence(workload("c2"), width=W, height=H, cas=cas)
+    seq = Sequence(workload("c2"), width=W, height=H, cas=cas, debug=False, shipped=True)
     for _ in range(3):
         assert_parity(seq.step(oracle))
 
@@ -129,12 +129,13 @@ def hostile_velocity(v4, rng):
 @pytest.mark.parametrize("mode", [0, 2])
 def test_hostile_velocity_and_depth_fp32(oracle, mode):
     """NaN / Inf / huge velocities (and NaN depths) follow the transcription bit for bit:
-    int(x) saturates, NaN converts to 0, every fetch is clamped to the edge."""
+    int(x) saturates, NaN converts to 0, every fetch is clamped to the edge (mode 2 also through
+    the exact kernel's production form, bit for bit as the general form)."""
     print(f"case hostile fp32 vel-mode {mode}", flush=True)
     from taastar.workloads import params_c2
     p = params_c2()
     p.mVelocitySampleMode = mode
-    seq = Sequence(workload("c2"), width=123, height=77, cas=True, params=p)
+    seq = Sequence(workload("c2"), width=123, height=77, cas=True, params=p, debug=False, shipped=True)
     rng = np.random.default_rng(5)
     render = seq.scene.render
 
@@ -172,6 +173,6 @@ def test_hostile_velocity_fp16(oracle):
 def test_largest_frame_fp32(oracle):
     """7680x4320 (33 Mpx) with RGBA16F storage on the exact path: one reset frame and one blend."""
     print("case 7680x4320 fp32", flush=True)
-    seq = Sequence(workload("c3"), width=7680, height=4320, cas=True)
+    seq = Sequence(workload("c3"), width=7680, height=4320, cas=True, debug=False, shipped=True)
     for _ in range(2):
         assert_parity(seq.step(oracle))

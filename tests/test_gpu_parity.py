@@ -36,9 +36,10 @@ def test_c1_full_sequence_bit_exact(oracle):
 @pytest.mark.parametrize("fmt,vfmt", [(FMT_RGBA32F, VEL_RG32F), (FMT_RGBA16F, VEL_RG16F)])
 def test_c2_features_fused_cas(oracle, fmt, vfmt):
     """c2 feature set (closest-depth velocity, linear depth rejection, Lottes, reject outside,
-    fused CAS 0.5) at 480x272, 8 frames, FP32 and FP16 storage."""
+    fused CAS 0.5) at 480x272, 8 frames, FP32 and FP16 storage; without the inspect image the same
+    frames take the exact kernel's production form, which must write the same bits."""
     print(f"case c2 480x272 x8 fmt={fmt}", flush=True)
-    seq = Sequence(workload("c2", fmt=fmt, vel=vfmt), width=480, height=272, cas=True)
+    seq = Sequence(workload("c2", fmt=fmt, vel=vfmt), width=480, height=272, cas=True, debug=False, shipped=True)
     for f in range(8):
         out = seq.step(oracle)
         assert_parity(out)
@@ -193,9 +194,10 @@ def test_deterministic_rerun(oracle):
 
 @pytest.mark.parametrize("name", ["c2", "c3"])
 def test_full_size_against_oracle(oracle, name):
-    """The real 1080p / 4K frames: 3 frames through the oracle, index/box bit-exact."""
+    """The real 1080p / 4K frames: 3 frames through the oracle, index/box bit-exact; the
+    production form (no inspect image) writes the same bits."""
     print(f"case full-size {name}", flush=True)
-    seq = Sequence(workload(name), cas=True)
+    seq = Sequence(workload(name), cas=True, debug=False, shipped=True)
     for f in range(3):
         out = seq.step(oracle)
         assert_parity(out)
