@@ -192,11 +192,8 @@ int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, in
     fd.param_idx = param_idx;
     fd.tiles_x = taastar::tiles_for(x_end - x_begin, s.tw);
     fd.tiles_y = taastar::tiles_for(fd.H, s.th);
-    if (fd.up || U.mUpsampling) {  // upsampling path: its own tile shape
+    if (fd.up || U.mUpsampling) {  // upsampling path: the launcher tiles by its form's own shape
         fd.up = 1;
-        const taastar::TileShapeDef d = shape_def(cas ? taastar::kUpShapeCas : taastar::kUpShapePlain, cas);
-        fd.tiles_x = taastar::tiles_for(x_end - x_begin, d.tw);
-        fd.tiles_y = taastar::tiles_for(fd.H, d.th);
         const int e = taastar::launch_resolve_up(fd, U, arith, stream);
         if (e != 0) return hip_fail((hipError_t)e, "resolve launch");
         return TAA_OK;

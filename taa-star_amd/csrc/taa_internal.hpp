@@ -68,8 +68,11 @@ constexpr int kDefaultShapeCas = 2;
 // default flags: 102 us against 105 us for 30x14 and 151 us for 30x30)
 constexpr int kSegShapeCas = 0;
 constexpr int kSegShapePlain = kDefaultShapePlain;
-// tile shapes of the upsampling resolve (its LDS holds the staged input rectangle)
+// tile shapes of the upsampling resolve (its LDS holds the staged input rectangle); its production
+// form (fewer registers) takes 30x14 CAS tiles: 1080p -> 4K in the FP16 arithmetic 0.205 ms against
+// 0.232 ms with 14x14 (30x30 and 62x14: 0.261 and 0.267 ms); the general form stays at 14x14
 constexpr int kUpShapeCas = 0;
+constexpr int kUpShapeCasProd = 1;
 constexpr int kUpShapePlain = kDefaultShapePlain;
 
 inline int tiles_for(int extent, int t) { return (extent + t - 1) / t; }
@@ -81,8 +84,12 @@ int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int 
 // culling, Lottes, outside rejection; no debug or inspect image): the general kernel's arithmetic
 // with the other branches compiled out
 bool exact_prod_form(const FrameDesc& fd, const taa_uniforms_t& U);
-// the upsampling (TAAU) path, default tile shape: exact FP32 arithmetic, or with TAA_ARITH_FP16 the
-// FP16 path's where no selection depends on it
+// the same switches, whatever the sizes: launch_resolve_up runs the upsampling kernel's production
+// form when they hold
+bool prod_params(const FrameDesc& fd, const taa_uniforms_t& U);
+// the upsampling (TAAU) path: exact FP32 arithmetic, or with TAA_ARITH_FP16 the FP16 path's where no
+// selection depends on it; tiled by its form's shape (kUpShapeCas / kUpShapeCasProd / kUpShapePlain),
+// whatever fd.tiles_x / tiles_y say
 int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, int arith, void* stream);
 // the resolve with the segmentation mask (exact FP32 arithmetic, default tile shape)
 int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, void* stream);

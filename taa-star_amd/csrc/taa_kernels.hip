@@ -53,7 +53,9 @@ struct Geo {
         float nx[SEG ? SH * SP : 1], ny[SEG ? SH * SP : 1], nz[SEG ? SH * SP : 1], lum[SEG ? SH * SP : 1];
         uint32_t mat[SEG ? SH * SP : 1];
         uint32_t vel[VEL16 ? SH * SP : 1];  // RG16F velocity of the staged region
-        // upsampling path: converted colour, depth and velocity (as ld_vel reads it) of the staged rectangle
+        // upsampling path: converted colour, depth and velocity (as ld_vel reads it) of the staged
+        // rectangle. Staging only x and y of an RG velocity in the production form frees enough LDS
+        // for a fifth 30x14 workgroup per CU, which ran slower (1080p -> 4K FP16: 0.221 against 0.205 ms)
         float4 uvel[UCW * UCH];
         float uc0[UCW * UCH], uc1[UCW * UCH], uc2[UCW * UCH], ud[UCW * UCH];
         // upsampling path, per resolved column (row): the bilinear axis of the depth and velocity
@@ -176,7 +178,7 @@ __device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, c
 // bilinear history filters with fp16 texels in mixed FMAs, one reciprocal for the clip, Lottes and
 // the near-clamp ratio, the blend as one FMA per channel; every texel index, tap choice and
 // rejection decision stays that of the FP32 path (used by the upsampling resolve's FP16 form)
-template <bool S16, int SP, bool UP, bool SEG = false, bool STG = false, bool VEL16 = false, bool FA = false>
+template <bool S16, int SP, bool UP, bool SEG = false, bool STG = false, bool VEL16 = false, bool FA = false, int UCW = 1>
 __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, const taa_uniforms_t& U, const taa_params_t& P,
                                          int x, int y, int l, const float* __restrict__ s0,
                                          const float* __restrict__ s1, const float* __restrict__ s2,
@@ -197,9 +199,10 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     }
     auto lo = [&](int xx, int yy) { return (uint32_t)(clampi(yy, 0, Hl - 1) * pl + clampi(xx, 0, Wl - 1)); };
     // upsampling path: input texel (xx, yy) (clamped coordinates) from the staged rectangle (STG: the
-    // caller has checked that every fetch of this texel lies in it), else from global memory --
-    // the same values either way
-    auto staged = [&](int xx, int yy) { return (yy - up->y0) * up->n + (xx - up->x0); };
+    // tile staged every input texel its resolve reads; row pitch UCW), else from global memory --
+    // the same values either way. The staged rectangle holds at position (sx, sy) the texel
+    // (clamp(x0 + sx), clamp(y0 + sy)), so a position relative to it needs no clamp.
+    auto staged = [&](int xx, int yy) { return (yy - up->y0) * UCW + (xx - up->x0); };
     auto color_at = [&](int xx, int yy) -> f3 {
         if constexpr (STG) {
             const int k = staged(xx, yy);
@@ -212,10 +215,18 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         if constexpr (STG) return up->d[staged(xx, yy)];
         else return ld_f32(fd.depth, (uint32_t)(yy * pl + xx));
     };
-    auto vel_at = [&](int xx, int yy) -> float4 {
-        if constexpr (STG) return up->v[staged(xx, yy)];
+    // the taps of the bilinear depth and velocity samples: (x, y) as the tile's tables hold them --
+    // STG: the position in the staged rectangle, x - x0 and (y - y0) * UCW; else the clamped texel
+    auto depth_tap = [&](int xx, int yy) -> float {
+        if constexpr (STG) return up->d[yy + xx];
+        else return ld_f32(fd.depth, (uint32_t)(yy * pl + xx));
+    };
+    auto vel_tap_up = [&](int xx, int yy) -> float4 {
+        if constexpr (STG) return up->v[yy + xx];
         else return ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(yy * pl + xx));
     };
+    // STG: the output texel's own input texel (xl, yl) in the staged rectangle
+    const int sbase = STG ? staged(xl, yl) : 0;
     // velocity texel (x + dx, y + dy) clamped to the image (global index gidx), dx, dy in -1..1:
     // from the staged RG16F region when the tile staged it (svel), else from global memory
     auto vel_tap = [&](int dx, int dy, uint32_t gidx) -> float4 {
@@ -257,7 +268,10 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     if (!P.mUnjitterNeighbourhood) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            if constexpr (UP) {
+            if constexpr (STG) {
+                const int li = sbase + oy[k] * UCW + ox[k];
+                c[k] = f3{up->c0[li], up->c1[li], up->c2[li]};
+            } else if constexpr (UP) {
                 c[k] = color_at(clampi(xl + ox[k], 0, Wl - 1), clampi(yl + oy[k], 0, Hl - 1));
             } else {
                 int li = l + oy[k] * SP + ox[k];
@@ -341,12 +355,14 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
             currentColor = stage_convert(P, xyz(bilinear_rgba<S16>(fd.color, Wl, Hl, pl, u, v)));
         } else if (P.mUnjitterNeighbourhood) {
             // c[0] is the unjittered neighbourhood centre; getCurrentColor samples the texel itself
-            if constexpr (UP) currentColor = color_at(clampi(xl, 0, Wl - 1), clampi(yl, 0, Hl - 1));
+            if constexpr (STG) currentColor = f3{up->c0[sbase], up->c1[sbase], up->c2[sbase]};
+            else if constexpr (UP) currentColor = color_at(clampi(xl, 0, Wl - 1), clampi(yl, 0, Hl - 1));
             else currentColor = {s0[l], s1[l], s2[l]};
         }
     }
     float depth;  // taa.comp:756
-    if constexpr (UP) depth = depth_at(clampi(xl, 0, Wl - 1), clampi(yl, 0, Hl - 1));
+    if constexpr (STG) depth = up->d[sbase];
+    else if constexpr (UP) depth = depth_at(clampi(xl, 0, Wl - 1), clampi(yl, 0, Hl - 1));
     else depth = sd[l];
 
     // ---- getHistoryPosition (taa.comp:391-438)
@@ -355,7 +371,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     {
         // SAMPLE_TEX(uCurrentVelocity, uv): a texel fetch at one size, bilinear when upsampling
         float4 vs;
-        if constexpr (UP) vs = bilinear4_at(up_taps(0, 0), vel_at);
+        if constexpr (UP) vs = bilinear4_at(up_taps(0, 0), vel_tap_up);
         else vs = vel_tap(0, 0, gi);
         bool canUseVelocity = !(P.mUseVelocityVectors == 0 || (P.mUseVelocityVectors == 1 && vs.w < 0.5f));
         if (canUseVelocity) {
@@ -363,7 +379,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
             if (P.mVelocitySampleMode == 1) {
                 // taa.comp:398-412: the longest vector is tracked but the last tap is assigned
                 float4 last;
-                if constexpr (UP) last = bilinear4_at(up_taps(1, 1), vel_at);
+                if constexpr (UP) last = bilinear4_at(up_taps(1, 1), vel_tap_up);
                 else last = vel_tap(1, 1, (uint32_t)(min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1)));
                 vs.x = last.x;
                 vs.y = last.y;
@@ -371,12 +387,12 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
                 // findClosestUvAndZ_3x3 (taa.comp:371-389): scan order row-major, strict <
                 int best = 0;
                 if constexpr (UP) {  // depth taps at the output uv, interpolated
-                    float dC = bilinear1_at(up_taps(-1, -1), depth_at);
+                    float dC = bilinear1_at(up_taps(-1, -1), depth_tap);
                     for (int k = 1; k < 9; ++k) {
-                        float d = bilinear1_at(up_taps(k % 3 - 1, k / 3 - 1), depth_at);
+                        float d = bilinear1_at(up_taps(k % 3 - 1, k / 3 - 1), depth_tap);
                         if (d < dC) { dC = d; best = k; }
                     }
-                    vs = bilinear4_at(up_taps(best % 3 - 1, best / 3 - 1), vel_at);
+                    vs = bilinear4_at(up_taps(best % 3 - 1, best / 3 - 1), vel_tap_up);
                 } else {
                     float dC = sd[l - SP - 1];
 #pragma unroll
@@ -466,7 +482,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         const float eps = 1e-5f;
         auto mov = [&](int dx, int dy) {
             float4 v;
-            if constexpr (UP) v = bilinear4_at(up_taps(dx, dy), vel_at);
+            if constexpr (UP) v = bilinear4_at(up_taps(dx, dy), vel_tap_up);
             else v = vel_tap(dx, dy, (uint32_t)(clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1)));
             return (fabsf(v.x) > eps || fabsf(v.y) > eps) && (fabsf(v.w) >= 0.5f);
         };
@@ -691,9 +707,27 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
             sm.uax, sm.uay, sm.usx, sm.usy, sm.uhx, sm.uhy, G::RW, G::RH};
     if constexpr (G::UP) {
         const int Wl = fd.Wl, Hl = fd.Hl, pl = fd.pitch_l;
+        // the staged rectangle: the input texels under the resolve region, +3 on every side
+        const int cx_lo = clampi(x0 - G::RA, 0, W - 1), cx_hi = clampi(x0 + G::TW + G::RA - 1, 0, W - 1);
+        const int cy_lo = clampi(y0 - G::RA, 0, H - 1), cy_hi = clampi(y0 + G::TH + G::RA - 1, 0, H - 1);
+        const int xl_lo = to_int(((float)cx_lo + 0.5f) / (float)W * (float)Wl);
+        const int xl_hi = to_int(((float)cx_hi + 0.5f) / (float)W * (float)Wl);
+        const int yl_lo = to_int(((float)cy_lo + 0.5f) / (float)H * (float)Hl);
+        const int yl_hi = to_int(((float)cy_hi + 0.5f) / (float)H * (float)Hl);
+        const int w = xl_hi - xl_lo + 7, h = yl_hi - yl_lo + 7;
+        // staged only for inputs no larger than the output (scale >= 1), where the bounds above hold
+        const bool stage = Wl <= W && Hl <= H && w <= G::UCW && h <= G::UCH;
+        if (stage) {
+            ui.x0 = xl_lo - 3;
+            ui.y0 = yl_lo - 3;
+            ui.w = w;
+            ui.h = h;
+        }
         // per-column and per-row tables of the resolve region (clamped texel, uv as tabulated
         // below): the bilinear axes of SAMPLE_TEX at uv + (d - 1) / input size, and one axis of
-        // getCurrentUpsampledColor's search at both corner offsets
+        // getCurrentUpsampledColor's search at both corner offsets. A staged tile keeps the bilinear
+        // taps as positions in the staged rectangle (x - x0, (y - y0) * UCW), which hold the same
+        // texels.
         {
             const float toUx = 1.0f / (float)Wl, toUy = 1.0f / (float)Hl;
             const float jux = U.mJitterNdc[0] * 0.5f * P.mUnjitterFactor, juy = U.mJitterNdc[1] * 0.5f * P.mUnjitterFactor;
@@ -706,8 +740,17 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
                 const int c = col ? clampi(x0 - G::RA + j, 0, W - 1) : clampi(y0 - G::RA + j, 0, H - 1);
                 const float uv = col ? ((float)c + 0.5f) / (float)W : ((float)c + 0.5f) / (float)H;
                 if (q < 3) {
-                    const BilinearAxis ba = col ? bilinear_axis(Wl, uv + toUx * (float)(q - 1))
-                                                : bilinear_axis(Hl, uv + toUy * (float)(q - 1));
+                    BilinearAxis ba = col ? bilinear_axis(Wl, uv + toUx * (float)(q - 1))
+                                          : bilinear_axis(Hl, uv + toUy * (float)(q - 1));
+                    if (stage) {
+                        if (col) {
+                            ba.a -= ui.x0;
+                            ba.b -= ui.x0;
+                        } else {
+                            ba.a = (ba.a - ui.y0) * G::UCW;
+                            ba.b = (ba.b - ui.y0) * G::UCW;
+                        }
+                    }
                     (col ? sm.uax : sm.uay)[q * n + j] = ba;
                 } else {
                     bool hit;
@@ -718,19 +761,7 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
                 }
             }
         }
-        const int cx_lo = clampi(x0 - G::RA, 0, W - 1), cx_hi = clampi(x0 + G::TW + G::RA - 1, 0, W - 1);
-        const int cy_lo = clampi(y0 - G::RA, 0, H - 1), cy_hi = clampi(y0 + G::TH + G::RA - 1, 0, H - 1);
-        const int xl_lo = to_int(((float)cx_lo + 0.5f) / (float)W * (float)Wl);
-        const int xl_hi = to_int(((float)cx_hi + 0.5f) / (float)W * (float)Wl);
-        const int yl_lo = to_int(((float)cy_lo + 0.5f) / (float)H * (float)Hl);
-        const int yl_hi = to_int(((float)cy_hi + 0.5f) / (float)H * (float)Hl);
-        const int w = xl_hi - xl_lo + 7, h = yl_hi - yl_lo + 7;
-        // staged only for inputs no larger than the output (scale >= 1), where the bounds below hold
-        if (Wl <= W && Hl <= H && w <= G::UCW && h <= G::UCH) {
-            ui.x0 = xl_lo - 3;
-            ui.y0 = yl_lo - 3;
-            ui.w = w;
-            ui.h = h;
+        if (stage) {
             for (int i = tid; i < w * h; i += G::NT) {
                 const int sy = i / w, sx = i - sy * w;
                 const int gx = clampi(ui.x0 + sx, 0, Wl - 1), gy = clampi(ui.y0 + sy, 0, Hl - 1);
@@ -760,14 +791,13 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
         const float uvx = sm.ux[cx - gx0], uvy = sm.uy[cy - gy0];
         PixelResult r;
         if constexpr (G::UP) {
-            // every input fetch of this texel lies within 2 input texels of (xl, yl) = int(uv * input
-            // size) (bilinear taps at uv +- one texel, the neighbourhood, the upsampled sample):
-            // inside the staged rectangle the staged variant reads only LDS
-            const int xl = to_int(uvx * (float)fd.Wl), yl = to_int(uvy * (float)fd.Hl);
-            const bool in = ui.w > 0 && xl - 2 >= ui.x0 && xl + 2 < ui.x0 + ui.w && yl - 2 >= ui.y0 && yl + 2 < ui.y0 + ui.h;
-            if (in)
-                r = resolve_pixel_f32<S16, G::SP, true, false, true, false, FA>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
-                                                                     owned, sg, &spl, &ui, rx, ry);
+            // every input fetch of a texel lies within 2 input texels of its (xl, yl) = int(uv * input
+            // size) (bilinear taps at uv +- one texel, the neighbourhood, the upsampled sample), and
+            // xl, yl grow with the output column and row, so a staged tile's rectangle holds them
+            // all: the staged variant reads only LDS
+            if (ui.w > 0)
+                r = resolve_pixel_f32<S16, G::SP, true, false, true, false, FA, G::UCW>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
+                                                                               uvx, uvy, owned, sg, &spl, &ui, rx, ry);
             else
                 r = resolve_pixel_f32<S16, G::SP, true, false, false, false, FA>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
                                                                       owned, sg, &spl, &ui, rx, ry);
@@ -835,12 +865,13 @@ __global__ __launch_bounds__(256) void k_resolve_seg(const ResolveArgs a, const 
 // Upsampling path (input size != output size, or mUpsampling): exact FP32 arithmetic, or with FA
 // the FP16 path's (TAA_ARITH_FP16); the input texels under the tile are staged in LDS (converted
 // colour, depth, velocity), the rest of the resolve as above, fused CAS from LDS.
-template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA>
+// PROD: the production feature set with its switches fixed (prod_fix)
+template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA, bool PROD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FA && S16 ? 5 : 1))) void k_resolve_up(const ResolveArgs a) {
     using G = Geo<TW, TH, PAD, CAS, true>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
-    resolve_tile<G, S16, false, FA>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
+    resolve_tile<G, S16, false, FA, PROD>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
 }
 
 // Batched resolve: the grid walks the flattened tile space of every frame in the batch with a
@@ -923,10 +954,10 @@ hipError_t launch_batch_t(const BatchEntry* e, int n, int total, int shape, int 
 
 }  // namespace
 
-bool exact_prod_form(const FrameDesc& fd, const taa_uniforms_t& U) {
+bool prod_params(const FrameDesc& fd, const taa_uniforms_t& U) {
     const taa_params_t& p = U.param[fd.param_idx];
     const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
-    return !fd.debug && !fd.inspect && !fd.up && !U.mUpsampling && !U.mBypassHistoryUpdate &&
+    return !fd.debug && !fd.inspect && !U.mBypassHistoryUpdate &&
            fd.vel_fmt == (s16 ? TAA_VEL_RG16F : TAA_VEL_RG32F) && !p.mPassThrough && !p.mUnjitterNeighbourhood &&
            !p.mUnjitterCurrentSample && p.mVarianceClipping && !p.mShapedNeighbourhood && p.mUseYCoCg &&
            !p.mShrinkChromaAxis && p.mUseVelocityVectors == 2 && p.mVelocitySampleMode == 2 && p.mDepthCulling &&
@@ -934,6 +965,8 @@ bool exact_prod_form(const FrameDesc& fd, const taa_uniforms_t& U) {
            !p.mVelBasedAlpha && p.mLumaWeightingLottes && !p.mReduceBlendNearClamp && p.mRejectOutside && !p.mAddNoise &&
            !p.mToneMapLumaKaris;
 }
+
+bool exact_prod_form(const FrameDesc& fd, const taa_uniforms_t& U) { return !fd.up && !U.mUpsampling && prod_params(fd, U); }
 
 int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int arith, void* stream) {
     (void)arith;
@@ -950,17 +983,22 @@ int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int 
     return (int)e;
 }
 
-template <bool FA>
-hipError_t launch_resolve_up_t(const FrameDesc& fd, const ResolveArgs& a, hipStream_t st) {
-    const int n = fd.tiles_x * fd.tiles_y;
+// the launch's tiling follows the form's tile shape: tiles_x / tiles_y of `a` are set here
+template <bool FA, bool PROD>
+hipError_t launch_resolve_up_t(ResolveArgs a, hipStream_t st) {
+    FrameDesc& fd = a.fd;
     const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
-    constexpr TileShapeDef C = kShapesCas[kUpShapeCas], Pl = kShapesPlain[kUpShapePlain];
+    constexpr TileShapeDef C = kShapesCas[PROD ? kUpShapeCasProd : kUpShapeCas], Pl = kShapesPlain[kUpShapePlain];
+    const TileShapeDef d = fd.cas_on ? C : Pl;
+    fd.tiles_x = tiles_for(fd.x_end - fd.x_begin, d.tw);
+    fd.tiles_y = tiles_for(fd.H, d.th);
+    const int n = fd.tiles_x * fd.tiles_y;
     if (fd.cas_on) {
-        if (s16) hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, true, FA>), dim3(n), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, false, FA>), dim3(n), dim3(256), 0, st, a);
+        if (s16) hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, true, FA, PROD>), dim3(n), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_resolve_up<C.tw, C.th, C.pad, true, false, FA, PROD>), dim3(n), dim3(256), 0, st, a);
     } else {
-        if (s16) hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, true, FA>), dim3(n), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, false, FA>), dim3(n), dim3(256), 0, st, a);
+        if (s16) hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, true, FA, PROD>), dim3(n), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_resolve_up<Pl.tw, Pl.th, Pl.pad, false, false, FA, PROD>), dim3(n), dim3(256), 0, st, a);
     }
     return hipGetLastError();
 }
@@ -970,7 +1008,11 @@ int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, int arith, v
     ResolveArgs a;
     a.fd = fd;
     a.U = U;
-    return (int)(arith == TAA_ARITH_FP16 ? launch_resolve_up_t<true>(fd, a, st) : launch_resolve_up_t<false>(fd, a, st));
+    const bool prod = prod_params(fd, U);
+    hipError_t e;
+    if (arith == TAA_ARITH_FP16) e = prod ? launch_resolve_up_t<true, true>(a, st) : launch_resolve_up_t<true, false>(a, st);
+    else e = prod ? launch_resolve_up_t<false, true>(a, st) : launch_resolve_up_t<false, false>(a, st);
+    return (int)e;
 }
 
 int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, void* stream) {

@@ -14,7 +14,7 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from gpu_helpers import assert_parity, round_storage, to_f16  # noqa: E402
+from gpu_helpers import assert_parity, assert_same_bits, round_storage, to_f16  # noqa: E402
 from taastar import (ARITH_FP16, ARITH_FP32, FMT_RGBA16F, FMT_RGBA32F, INSPECT_WORDS, VEL_RG16F, VEL_RG32F,  # noqa: E402
                      VEL_RGBA32F, FrameImages, check, lib)
 from taastar.device import color_dtype, vel_shape_dtype  # noqa: E402
@@ -29,7 +29,10 @@ def _need_gpu():
 
 
 class Upsampler:
-    """A render-resolution scene resolved into an output-resolution history, GPU and oracle."""
+    """A render-resolution scene resolved into an output-resolution history, GPU and oracle. Every
+    frame is also resolved into a second history chain without the debug and inspect images (the
+    launch that takes the upsampling kernel's production form when the parameters are the
+    production set); both chains must hold the same bits."""
 
     def __init__(self, w, lo, hi, cas=True, upsampling=True, params=None, arith=ARITH_FP32):
         self.w, self.fmt, self.vfmt = w, w["fmt"], w["vel"]
@@ -44,6 +47,8 @@ class Upsampler:
         self.vel, self.prev_depth = z(self.Hl, self.Wl, vc, dtype=vd), z(self.Hl, self.Wl, dtype=torch.float32)
         self.hist = [z(self.H, self.W, 4, dtype=cd), z(self.H, self.W, 4, dtype=cd)]
         self.screen = z(self.H, self.W, 4, dtype=cd)
+        self.hist_s = [z(self.H, self.W, 4, dtype=cd), z(self.H, self.W, 4, dtype=cd)]
+        self.screen_s = z(self.H, self.W, 4, dtype=cd)
         self.debug = z(self.H, self.W, 4, dtype=torch.float16)
         self.inspect = z(self.H, self.W, INSPECT_WORDS, dtype=torch.int32)
         self.hist_o = np.zeros((self.H, self.W, 4), np.float32)
@@ -72,7 +77,14 @@ class Upsampler:
         self.last_kernel = (k[0], k[1])
         check(lib().taa_resolve_frame(C.byref(U), C.byref(img), C.byref(self.cas) if self.cas else None, self.arith,
                                       None))
+        img.history, img.result_history = self.hist_s[(self.k + 1) & 1].data_ptr(), self.hist_s[self.k & 1].data_ptr()
+        img.result_screen, img.debug, img.inspect = self.screen_s.data_ptr(), None, None
+        check(lib().taa_resolve_frame(C.byref(U), C.byref(img), C.byref(self.cas) if self.cas else None, self.arith,
+                                      None))
         torch.cuda.synchronize()
+        assert_same_bits(self.screen, self.screen_s, f"frame {self.k}: screen without debug and inspect images")
+        assert_same_bits(self.hist[self.k & 1], self.hist_s[self.k & 1],
+                         f"frame {self.k}: history without debug and inspect images")
         o = oracle.resolve(U, decode(col_s), dep, velocity_as_read(vel_s), self.hist_o, self.prev_o)
         o_hist = round_storage(o["history"], self.fmt)
         o_scr = round_storage(o["screen"], self.fmt)
