@@ -260,8 +260,12 @@ enum { TAA_SEG_TAA = 0, TAA_SEG_EDGE_AA = 1, TAA_SEG_RAY_TRACE = 2 };
 /* The extra images of the segmentation mask (taa.comp:34-37, 589-702): R32UI material ids of this
  * and the previous frame (bit 31 marks a moving object, fwd_geometry.frag:286), the RGBA32F
  * uv + spherical-normal G-buffer (IMAGE_FORMAT_NORMAL; z azimuth, w elevation), the previous
- * frame's mask and the mask written by this frame (TAA_IMAGE_FORMAT_SEGMASK, R32UI). All at the
- * frame's size, rows of `pitch` texels (0 = the frame's pitch); segmask must not alias an input. */
+ * frame's mask and the mask written by this frame (TAA_IMAGE_FORMAT_SEGMASK, R32UI).
+ * The renderer's images (material, prev_material, uv_normal) have the input (render) size, the
+ * masks the output size (taa.hpp:336-338). `pitch`: row pitch in texels of the masks, 0 = the
+ * frame's pitch; at one size also that of the renderer's images. `input_pitch`: row pitch of the
+ * renderer's images when the input size differs from the output size, 0 = the frame's input pitch.
+ * segmask must not alias an input. */
 typedef struct taa_segmentation_images {
     const uint32_t* material;      /* set0 b10 uCurrentMaterial  */
     const uint32_t* prev_material; /* set0 b11 uPreviousMaterial */
@@ -269,7 +273,8 @@ typedef struct taa_segmentation_images {
     const uint32_t* prev_segmask;  /* set0 b12 uPreviousSegMask  */
     uint32_t*       segmask;       /* set0 b9  uSegMask (written) */
     int32_t         pitch;
-    int32_t         reserved[3];   /* must be 0 */
+    int32_t         input_pitch;
+    int32_t         reserved[2];   /* must be 0 */
 } taa_segmentation_images_t;
 
 /* Push constants of the edge-AA fallback pass: push_constants_for_fxaa (source/taa.hpp:93-98),
@@ -331,9 +336,16 @@ int32_t taa_resolve_frame_noalloc(const taa_uniforms_t* uniforms, const taa_fram
  * from the extra images, written to seg->segmask, and a non-zero category rejects the history
  * (alpha = mRejectionAlpha) as in the reference; debug mode 8 shows the category. Texels of
  * mPassThrough / mBypassHistoryUpdate leave the mask untouched, as the reference's early return
- * does. Exact FP32 arithmetic whatever `arith` asks; one size only (an input size different from
- * the output size, or mUpsampling, is TAA_ERR_UNSUPPORTED). Category-2 texels are for the
- * caller's ray tracer; category-1 texels for taa_edge_aa_apply. */
+ * does. Category-2 texels are for the caller's ray tracer; category-1 texels for
+ * taa_edge_aa_apply. TAA_ARITH_FP16 takes the FP16 path's arithmetic where no selection depends on
+ * it (the mask, indices and decisions stay exact), TAA_ARITH_FP32 the exact kernel.
+ * With an input size different from the output size, or mUpsampling, the mask is evaluated at the
+ * output texel as the reference does (taa.comp:775 passes the output coordinates): the fixed
+ * border of TAA_RTFLAG_FXD is measured against the input size, the neighbours are clamped to the
+ * input image, and the centre's material id and normal are read at the output coordinates, which
+ * lie outside the input image for most texels -- a read Vulkan leaves undefined unless the device
+ * bounds-checks images; it reads zero here (material 0, uv/normal (0, 0, 0, 0)), the value
+ * robustImageAccess2 defines. */
 int32_t taa_resolve_frame_segmented(const taa_uniforms_t* uniforms, const taa_frame_images_t* images,
                                     const taa_segmentation_images_t* seg, const taa_cas_consts_t* cas,
                                     int32_t arith, void* stream);

@@ -38,8 +38,9 @@
 
 namespace {
 
-// Segmentation inputs of one frame (all W x H, one size); mask_out receives uSegMask where the
-// reference stores it and keeps its content elsewhere.
+// Segmentation inputs of one frame (material, prev_material, uv_normal at the input size,
+// prev_segmask and mask_out at the output size); mask_out receives uSegMask where the reference
+// stores it and keeps its content elsewhere.
 struct SegIO {
     const uint32_t* material;
     const uint32_t* prev_material;
@@ -54,8 +55,7 @@ int resolve_frame(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, const f
                   T* debug, int32_t* inspect, int cr16, int threads, const SegIO* seg = nullptr) {
     if (!U || W <= 0 || H <= 0 || Wl <= 0 || Hl <= 0 || !color || !depth || !vel4 || !hist || !hdepth || !screen || !hist_out)
         return -1;
-    if (seg && (Wl != W || Hl != H || !seg->material || !seg->prev_material || !seg->uv_normal || !seg->prev_segmask ||
-                !seg->mask_out))
+    if (seg && (!seg->material || !seg->prev_material || !seg->uv_normal || !seg->prev_segmask || !seg->mask_out))
         return -1;
     oracle::Frame<T> F;
     F.W = W; F.H = H; F.Wl = Wl; F.Hl = Hl;
@@ -164,22 +164,23 @@ int oracle_resolve_f64(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, co
                                  threads);
 }
 
-// The resolve with the segmentation mask (one size): the extra images and the mask written.
-int oracle_resolve_seg_f32(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+// The resolve with the segmentation mask: the extra images and the mask written. W x H: output
+// size; Wl x Hl: input size (colour, depth, velocity, history depth, material ids, G-buffer).
+int oracle_resolve_seg_f32(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, const float* color, const float* depth,
                            const float* vel4, const float* hist, const float* hdepth, float* screen, float* hist_out,
                            float* debug, int32_t* inspect, const uint32_t* material, const uint32_t* prev_material,
                            const float* uv_normal, const uint32_t* prev_segmask, uint32_t* mask_out, int threads) {
     SegIO sg{material, prev_material, uv_normal, prev_segmask, mask_out};
-    return resolve_frame<float>(U, W, H, W, H, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, 1,
+    return resolve_frame<float>(U, W, H, Wl, Hl, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, 1,
                                 threads, &sg);
 }
 
-int oracle_resolve_seg_f64(const taa_uniforms_t* U, int W, int H, const float* color, const float* depth,
+int oracle_resolve_seg_f64(const taa_uniforms_t* U, int W, int H, int Wl, int Hl, const float* color, const float* depth,
                            const float* vel4, const double* hist, const float* hdepth, double* screen, double* hist_out,
                            double* debug, int32_t* inspect, const uint32_t* material, const uint32_t* prev_material,
                            const float* uv_normal, const uint32_t* prev_segmask, uint32_t* mask_out, int threads) {
     SegIO sg{material, prev_material, uv_normal, prev_segmask, mask_out};
-    return resolve_frame<double>(U, W, H, W, H, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, 1,
+    return resolve_frame<double>(U, W, H, Wl, Hl, color, depth, vel4, hist, hdepth, screen, hist_out, debug, inspect, 1,
                                  threads, &sg);
 }
 

@@ -101,12 +101,25 @@ template <class T> struct Frame {
         x = iclamp(x, 0, Wl - 1); y = iclamp(y, 0, Hl - 1);
         return T(hdepth[(size_t)y * Wl + x]);
     }
-    // segmentation inputs (one size only: W == Wl); material == nullptr: no mask
-    const uint32_t* material = nullptr;      // R32UI  uCurrentMaterial
-    const uint32_t* prev_material = nullptr; // R32UI  uPreviousMaterial
-    const float* uv_normal = nullptr;        // RGBA32F uCurrentUvNrm
-    const uint32_t* prev_segmask = nullptr;  // R32UI  uPreviousSegMask
-    uint32_t u32_at(const uint32_t* img, int x, int y) const {
+    // segmentation inputs; material == nullptr: no mask. The renderer's images have the input
+    // size (Wl x Hl), the masks the output size (taa.hpp:336-338 creates them at the target size)
+    const uint32_t* material = nullptr;      // R32UI  uCurrentMaterial   (input size)
+    const uint32_t* prev_material = nullptr; // R32UI  uPreviousMaterial  (input size)
+    const float* uv_normal = nullptr;        // RGBA32F uCurrentUvNrm     (input size)
+    const uint32_t* prev_segmask = nullptr;  // R32UI  uPreviousSegMask   (output size)
+    // an input-sized R32UI image at (x, y) clamped to its edge (CLAMP_TO_TEX, taa.comp:42)
+    uint32_t u32_in(const uint32_t* img, int x, int y) const {
+        x = iclamp(x, 0, Wl - 1); y = iclamp(y, 0, Hl - 1);
+        return img[(size_t)y * Wl + x];
+    }
+    // ... at (x, y) unclamped: imageLoad outside the image, which Vulkan leaves undefined unless the
+    // device bounds-checks image accesses; the value robustImageAccess2 defines, 0, is taken
+    uint32_t u32_in_or_zero(const uint32_t* img, int x, int y) const {
+        if (x < 0 || y < 0 || x >= Wl || y >= Hl) return 0u;
+        return img[(size_t)y * Wl + x];
+    }
+    // an output-sized R32UI image at (x, y) clamped to its edge
+    uint32_t u32_out(const uint32_t* img, int x, int y) const {
         x = iclamp(x, 0, W - 1); y = iclamp(y, 0, H - 1);
         return img[(size_t)y * W + x];
     }
@@ -418,13 +431,21 @@ inline void seg_sincos(float a, float& s, float& c) {
     }
 }
 
-// sample_normal (taa.comp:574-579): (cos z cos w, sin z cos w, sin w)
-template <class T> inline v3<T> seg_normal(const Frame<T>& F, int x, int y) {
-    x = iclamp(x, 0, F.W - 1); y = iclamp(y, 0, F.H - 1);
-    const float* p = F.uv_normal + 4 * ((size_t)y * F.W + x);
+// sample_normal (taa.comp:574-579): (cos z cos w, sin z cos w, sin w) of the input-sized G-buffer
+// texel (x, y); `clamp`: the texel clamped to the edge (the neighbours, read through CLAMP_TO_TEX),
+// else a texel outside the image reads (0, 0, 0, 0) as robustImageAccess2 defines (the centre, read
+// at the output texel's own coordinates, taa.comp:641, which upsampling puts outside the input)
+template <class T> inline v3<T> seg_normal(const Frame<T>& F, int x, int y, bool clamp = true) {
+    float z = 0.0f, w = 0.0f;
+    if (clamp) { x = iclamp(x, 0, F.Wl - 1); y = iclamp(y, 0, F.Hl - 1); }
+    if (x >= 0 && y >= 0 && x < F.Wl && y < F.Hl) {
+        const float* p = F.uv_normal + 4 * ((size_t)y * F.Wl + x);
+        z = p[2];
+        w = p[3];
+    }
     T sz, cz, sw, cw;
-    seg_sincos(T(p[2]), sz, cz);
-    seg_sincos(T(p[3]), sw, cw);
+    seg_sincos(T(z), sz, cz);
+    seg_sincos(T(w), sw, cw);
     return {cz * cw, sz * cw, sw};
 }
 
@@ -436,17 +457,22 @@ template <class T> inline T sobel_length(const T (&t)[8]) {
     return std::sqrt(gx * gx + gy * gy);  // length()
 }
 
-// calc_segmentation_value (taa.comp:589-702) at texel iuv with the frame's historyUv; `total`
-// receives the weighted indicator sum when the function gets that far (else 0).
+// calc_segmentation_value (taa.comp:589-702) at output texel iuv with the frame's historyUv;
+// `total` receives the weighted indicator sum when the function gets that far (else 0). As in the
+// reference, iuv is the output texel also where the input (render) size differs: the border test
+// of TAA_RTFLAG_FXD is against the input size (textureSize_loRes, taa.comp:595), the neighbours
+// are clamped to the input image (CLAMP_TO_TEX), the previous material is read at historyUv in
+// input texels (taa.comp:622-624), the centre's material id and normal at iuv itself (zero outside
+// the input image, see u32_in_or_zero), the previous mask at iuv in the output-sized mask.
 template <class T>
 inline uint32_t segmentation_value(const taa_uniforms_t& U, const taa_params_t& P, const Frame<T>& F, iv2 iuv,
                                    v2<T> historyUv, T& total) {
     total = T(0);
     const uint32_t fl = P.mRayTraceAugmentFlags;
-    const int W = F.W, H = F.H;
+    const int Wl = F.Wl, Hl = F.Hl;
     if (fl & TAA_RTFLAG_FXD) {
         const int b = 100;
-        if (iuv.x < b || iuv.y < b || iuv.x >= W - b || iuv.y >= H - b) return 1u;
+        if (iuv.x < b || iuv.y < b || iuv.x >= Wl - b || iuv.y >= Hl - b) return 1u;
     }
     if (fl & TAA_RTFLAG_ALL) return 2u;
     const bool useHistoryCount = (fl & TAA_RTFLAG_CNT) != 0;
@@ -454,18 +480,18 @@ inline uint32_t segmentation_value(const taa_uniforms_t& U, const taa_params_t& 
     if (fl & TAA_RTFLAG_OUT) {
         if (historyUv.x < T(0) || historyUv.y < T(0) || historyUv.x >= T(1) || historyUv.y >= T(1)) return 1u;
     }
-    const uint32_t matId = F.u32_at(F.material, iuv.x, iuv.y);
+    const uint32_t matId = F.u32_in_or_zero(F.material, iuv.x, iuv.y);
     if (fl & TAA_RTFLAG_DIS) {
-        iv2 tcPrev = uv_to_tc<T>(historyUv, iv2{W, H});
-        if (tcPrev.x >= 0 && tcPrev.y >= 0 && tcPrev.x < W && tcPrev.y < H) {
-            const uint32_t prevMatId = F.u32_at(F.prev_material, tcPrev.x, tcPrev.y);
+        iv2 tcPrev = uv_to_tc<T>(historyUv, iv2{Wl, Hl});
+        if (tcPrev.x >= 0 && tcPrev.y >= 0 && tcPrev.x < Wl && tcPrev.y < Hl) {
+            const uint32_t prevMatId = F.u32_in(F.prev_material, tcPrev.x, tcPrev.y);
             if (prevMatId != matId && (prevMatId & 0x80000000u) != 0) return 2u | newCountValue;
         }
     }
     T nrmValue = T(0), dptValue = T(0), matValue = T(0), lumValue = T(0);
     static const int ox[8] = {-1, 0, 1, -1, 1, -1, 0, 1}, oy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
     if (fl & TAA_RTFLAG_NRM) {
-        v3<T> nC = seg_normal(F, iuv.x, iuv.y);
+        v3<T> nC = seg_normal(F, iuv.x, iuv.y, false);
         auto d = [&](int dx, int dy) { v3<T> n = seg_normal(F, iuv.x + dx, iuv.y + dy); return nC.x * n.x + nC.y * n.y + nC.z * n.z; };
         T dL = d(-1, 0), dR = d(1, 0), dT = d(0, -1), dB = d(0, 1);
         T mind = gmax(T(0), gmin(gmin(gmin(dL, dR), dT), dB));
@@ -478,8 +504,8 @@ inline uint32_t segmentation_value(const taa_uniforms_t& U, const taa_params_t& 
         dptValue = sobel_length(t);
     }
     if (fl & TAA_RTFLAG_MID) {
-        if (matId != F.u32_at(F.material, iuv.x - 1, iuv.y) || matId != F.u32_at(F.material, iuv.x + 1, iuv.y) ||
-            matId != F.u32_at(F.material, iuv.x, iuv.y - 1) || matId != F.u32_at(F.material, iuv.x, iuv.y + 1))
+        if (matId != F.u32_in(F.material, iuv.x - 1, iuv.y) || matId != F.u32_in(F.material, iuv.x + 1, iuv.y) ||
+            matId != F.u32_in(F.material, iuv.x, iuv.y - 1) || matId != F.u32_in(F.material, iuv.x, iuv.y + 1))
             matValue = T(1.0);
     }
     if (fl & TAA_RTFLAG_LUM) {
@@ -491,7 +517,7 @@ inline uint32_t segmentation_value(const taa_uniforms_t& U, const taa_params_t& 
             matValue * T(P.mRayTraceAugment_WMId) + lumValue * T(P.mRayTraceAugment_WLum);
     if (total >= T(P.mRayTraceAugment_Thresh)) return 2u | newCountValue;
     if (useHistoryCount) {
-        const uint32_t oldCnt = (F.u32_at(F.prev_segmask, iuv.x, iuv.y) & 0xffff0000u) >> 16;
+        const uint32_t oldCnt = (F.u32_out(F.prev_segmask, iuv.x, iuv.y) & 0xffff0000u) >> 16;
         if (oldCnt > 0) return 2u | ((oldCnt - 1u) << 16);
     }
     return 0u;

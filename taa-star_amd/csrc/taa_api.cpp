@@ -174,14 +174,12 @@ int launch_range(taastar::FrameDesc fd, const taa_uniforms_t& U, int x_begin, in
                  const taastar::SegDesc* sg = nullptr) {
     if (x_end <= x_begin) return TAA_OK;
     const bool cas = fd.cas_on != 0;
-    if (sg) {  // segmentation mask: exact FP32 kernel, default tile shape
-        const taastar::TileShapeDef d = shape_def(cas ? taastar::kSegShapeCas : taastar::kSegShapePlain, cas);
+    if (sg) {  // segmentation mask: its own kernels, tiled by the launcher
         fd.x_begin = x_begin;
         fd.x_end = x_end;
         fd.param_idx = param_idx;
-        fd.tiles_x = taastar::tiles_for(x_end - x_begin, d.tw);
-        fd.tiles_y = taastar::tiles_for(fd.H, d.th);
-        const int e = taastar::launch_resolve_seg(fd, U, *sg, stream);
+        if (U.mUpsampling) fd.up = 1;
+        const int e = taastar::launch_resolve_seg(fd, U, *sg, arith, stream);
         if (e != 0) return hip_fail((hipError_t)e, "resolve launch");
         return TAA_OK;
     }
@@ -214,18 +212,23 @@ int prepare_resolve(const taa_uniforms_t* u, const taa_frame_images_t* img, cons
     rc = make_desc(img, cas, fd);
     if (rc) return rc;
     if (seg) {
-        if (fd.up || u->mUpsampling) return fail(TAA_ERR_UNSUPPORTED, "the segmentation mask is built for one size (no upsampling)");
         if (!seg->material || !seg->prev_material || !seg->uv_normal || !seg->prev_segmask || !seg->segmask)
             return fail(TAA_ERR_INVALID_ARG, "a segmentation image pointer is NULL");
-        if (seg->reserved[0] || seg->reserved[1] || seg->reserved[2]) return fail(TAA_ERR_INVALID_ARG, "reserved must be 0");
+        if (seg->reserved[0] || seg->reserved[1]) return fail(TAA_ERR_INVALID_ARG, "reserved must be 0");
         const int sp = seg->pitch == 0 ? fd.pitch : seg->pitch;
         if (sp < fd.W) return fail(TAA_ERR_INVALID_ARG, "segmentation pitch %d < width %d", sp, fd.W);
         if (!fits_offsets(sp, fd.H, 16)) return fail(TAA_ERR_INVALID_ARG, "a segmentation image would exceed 4 GiB");
+        // the renderer's images: input-sized; at one size with the masks' pitch unless input_pitch says otherwise
+        const bool one_size = fd.Wl == fd.W && fd.Hl == fd.H;
+        if (seg->input_pitch < 0) return fail(TAA_ERR_INVALID_ARG, "segmentation input_pitch %d < 0", seg->input_pitch);
+        const int spl = seg->input_pitch != 0 ? seg->input_pitch : (one_size ? sp : fd.pitch_l);
+        if (spl < fd.Wl) return fail(TAA_ERR_INVALID_ARG, "segmentation input pitch %d < input width %d", spl, fd.Wl);
+        if (!fits_offsets(spl, fd.Hl, 16)) return fail(TAA_ERR_INVALID_ARG, "a segmentation input image would exceed 4 GiB");
         const void* m = seg->segmask;
         if (m == seg->prev_segmask || m == seg->material || m == seg->prev_material || m == seg->uv_normal || m == fd.color ||
             m == fd.depth || m == fd.history || m == fd.hdepth || m == fd.velocity || m == fd.history_out || m == fd.screen)
             return fail(TAA_ERR_INVALID_ARG, "segmask aliases another image");
-        sgd = taastar::SegDesc{seg->material, seg->prev_material, seg->uv_normal, seg->prev_segmask, seg->segmask, sp, 0};
+        sgd = taastar::SegDesc{seg->material, seg->prev_material, seg->uv_normal, seg->prev_segmask, seg->segmask, sp, spl};
     }
     return TAA_OK;
 }

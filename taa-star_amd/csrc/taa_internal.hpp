@@ -40,8 +40,8 @@ struct SegDesc {
     const float* uv_normal;
     const uint32_t* prev_segmask;
     uint32_t* segmask;
-    int32_t pitch;
-    int32_t pad;
+    int32_t pitch;    // the masks' row pitch (output size)
+    int32_t pitch_l;  // the renderer's images' row pitch (input size)
 };
 
 // One frame of a batched launch: frame + device copy of its uniforms + first flattened tile.
@@ -91,8 +91,10 @@ bool prod_params(const FrameDesc& fd, const taa_uniforms_t& U);
 // selection depends on it; tiled by its form's shape (kUpShapeCas / kUpShapeCasProd / kUpShapePlain),
 // whatever fd.tiles_x / tiles_y say
 int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, int arith, void* stream);
-// the resolve with the segmentation mask (exact FP32 arithmetic, default tile shape)
-int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, void* stream);
+// the resolve with the segmentation mask (exact FP32 arithmetic, or with TAA_ARITH_FP16 the FP16 path's
+// where no selection depends on it); fd.up: the upsampling path's kernel, which reads the mask's
+// inputs from global memory. Tiled by its own shape, whatever fd.tiles_x / tiles_y say
+int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, int arith, void* stream);
 int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uint32_t peak_bits, void* stream);
 // Workgroups of a batched launch: one per tile, rounded to whole XCD slots. The batch kernels walk
 // the tile space with a grid stride, so a smaller (persistent) grid is also correct; measured on

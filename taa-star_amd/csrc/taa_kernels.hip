@@ -132,7 +132,7 @@ __device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, c
     if (fl & TAA_RTFLAG_DIS) {
         const int tx = to_int(hu * (float)W), ty = to_int(hv * (float)H);  // uv_to_tc(historyUv)
         if (tx >= 0 && ty >= 0 && tx < W && ty < H) {
-            const uint32_t prevMatId = gload<uint32_t>(sg.prev_material, (uint32_t)(ty * sg.pitch + tx) * 4u);
+            const uint32_t prevMatId = gload<uint32_t>(sg.prev_material, (uint32_t)(ty * sg.pitch_l + tx) * 4u);
             if (prevMatId != matId && (prevMatId & 0x80000000u) != 0u) return 2u | newCountValue;
         }
     }
@@ -159,6 +159,83 @@ __device__ uint32_t segmentation_value(const FrameDesc& fd, const SegDesc& sg, c
         float t[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) t[k] = sp.lum[l + oy[k] * SP + ox[k]];
+        lumValue = sobel_length(t);
+    }
+    total = nrmValue * P.mRayTraceAugment_WNrm + dptValue * P.mRayTraceAugment_WDpt + matValue * P.mRayTraceAugment_WMId +
+            lumValue * P.mRayTraceAugment_WLum;
+    if (total >= P.mRayTraceAugment_Thresh) return 2u | newCountValue;
+    if (useHistoryCount) {
+        const uint32_t oldCnt = (gload<uint32_t>(sg.prev_segmask, (uint32_t)(y * sg.pitch + x) * 4u) & 0xffff0000u) >> 16;
+        if (oldCnt > 0u) return 2u | ((oldCnt - 1u) << 16);
+    }
+    return 0u;
+}
+
+// calc_segmentation_value (taa.comp:589-702) on the upsampling path, at the output texel (x, y) as
+// the reference passes it (taa.comp:775). The renderer's images have the input size and are read
+// from global memory: the neighbours clamped to the input image (CLAMP_TO_TEX, taa.comp:42), the
+// centre's material id and normal at (x, y) itself -- outside the input image for most output
+// texels, where Vulkan leaves the load undefined unless the device bounds-checks images: zero
+// here, the value robustImageAccess2 defines. The border of TAA_RTFLAG_FXD is measured against the
+// input size (textureSize_loRes, taa.comp:595), the previous material read at historyUv in input
+// texels (taa.comp:622-624), the previous mask at (x, y) in the output-sized mask.
+template <bool S16>
+__device__ uint32_t segmentation_value_up(const FrameDesc& fd, const SegDesc& sg, const taa_uniforms_t& U,
+                                          const taa_params_t& P, int x, int y, float hu, float hv, float& total) {
+    total = 0.0f;
+    const uint32_t fl = P.mRayTraceAugmentFlags;
+    const int Wl = fd.Wl, Hl = fd.Hl, pl = fd.pitch_l, sp = sg.pitch_l;
+    if (fl & TAA_RTFLAG_FXD) {
+        const int b = 100;
+        if (x < b || y < b || x >= Wl - b || y >= Hl - b) return 1u;
+    }
+    if (fl & TAA_RTFLAG_ALL) return 2u;
+    const bool useHistoryCount = (fl & TAA_RTFLAG_CNT) != 0u;
+    const uint32_t newCountValue = useHistoryCount ? (uint32_t)P.mRayTraceHistoryCount << 16 : 0u;
+    if ((fl & TAA_RTFLAG_OUT) && (hu < 0.0f || hv < 0.0f || hu >= 1.0f || hv >= 1.0f)) return 1u;
+    const bool inside = x < Wl && y < Hl;  // x, y >= 0
+    // input texel (xx + dx, yy + dy) clamped to the input image: its offset in texels at row pitch q
+    auto cl = [&](int xx, int yy, int q) { return (uint32_t)(clampi(yy, 0, Hl - 1) * q + clampi(xx, 0, Wl - 1)); };
+    const uint32_t matId = inside ? gload<uint32_t>(sg.material, (uint32_t)(y * sp + x) * 4u) : 0u;
+    if (fl & TAA_RTFLAG_DIS) {
+        const int tx = to_int(hu * (float)Wl), ty = to_int(hv * (float)Hl);  // uv_to_tc(historyUv, textureSize_loRes)
+        if (tx >= 0 && ty >= 0 && tx < Wl && ty < Hl) {
+            const uint32_t prevMatId = gload<uint32_t>(sg.prev_material, (uint32_t)(ty * sp + tx) * 4u);
+            if (prevMatId != matId && (prevMatId & 0x80000000u) != 0u) return 2u | newCountValue;
+        }
+    }
+    const int ox[8] = {-1, 0, 1, -1, 1, -1, 0, 1}, oy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+    float nrmValue = 0.0f, dptValue = 0.0f, matValue = 0.0f, lumValue = 0.0f;
+    if (fl & TAA_RTFLAG_NRM) {
+        auto normal_of = [&](float4 q) {  // sample_normal (taa.comp:574-579)
+            float sz, cz, sw, cw;
+            seg_sincos(q.z, sz, cz);
+            seg_sincos(q.w, sw, cw);
+            return f3{cz * cw, sz * cw, sw};
+        };
+        const f3 nC = normal_of(inside ? gload<float4>(sg.uv_normal, (uint32_t)(y * sp + x) * 16u) : make_float4(0.f, 0.f, 0.f, 0.f));
+        auto dn = [&](int dx, int dy) {
+            const f3 n = normal_of(gload<float4>(sg.uv_normal, cl(x + dx, y + dy, sp) * 16u));
+            return nC.x * n.x + nC.y * n.y + nC.z * n.z;
+        };
+        const float dL = dn(-1, 0), dR = dn(1, 0), dT = dn(0, -1), dB = dn(0, 1);
+        nrmValue = 1.0f - fmaxf(0.0f, fminf(fminf(fminf(dL, dR), dT), dB));
+    }
+    if (fl & TAA_RTFLAG_DPT) {
+        float t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            t[k] = linearize_depth(ld_f32(fd.depth, cl(x + ox[k], y + oy[k], pl)), U.mCamNearPlane, U.mCamFarPlane);
+        dptValue = sobel_length(t);
+    }
+    if (fl & TAA_RTFLAG_MID) {
+        auto mat = [&](int dx, int dy) { return gload<uint32_t>(sg.material, cl(x + dx, y + dy, sp) * 4u); };
+        if (matId != mat(-1, 0) || matId != mat(1, 0) || matId != mat(0, -1) || matId != mat(0, 1)) matValue = 1.0f;
+    }
+    if (fl & TAA_RTFLAG_LUM) {
+        float t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t[k] = rgb_to_ycocg(xyz(ld_rgba<S16>(fd.color, cl(x + ox[k], y + oy[k], pl)))).x;
         lumValue = sobel_length(t);
     }
     total = nrmValue * P.mRayTraceAugment_WNrm + dptValue * P.mRayTraceAugment_WDpt + matValue * P.mRayTraceAugment_WMId +
@@ -460,7 +537,8 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     float segTotal = 0.0f;
     if constexpr (SEG) {
         if (P.mRayTraceAugment) {
-            segMaskValue = segmentation_value<SP>(fd, *sg, *spl, U, P, x, y, hu, hv, sd, l, segTotal);
+            if constexpr (UP) segMaskValue = segmentation_value_up<S16>(fd, *sg, U, P, x, y, hu, hv, segTotal);
+            else segMaskValue = segmentation_value<SP>(fd, *sg, *spl, U, P, x, y, hu, hv, sd, l, segTotal);
             if ((segMaskValue & 3u) != 0u) {
                 alpha = P.mRejectionAlpha;
                 rejected = true;
@@ -683,7 +761,7 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
         if constexpr (G::VEL16)
             if (vel_staged) sm.vel[li] = gload<uint32_t>(fd.velocity, gi * 4u);
         if constexpr (G::SEG) {  // sample_normal, sample_luminance and the material id (taa.comp:570-579)
-            const uint32_t si = (uint32_t)(gy * sg->pitch + gx);
+            const uint32_t si = (uint32_t)(gy * sg->pitch_l + gx);
             const float4 q = gload<float4>(sg->uv_normal, si * 16u);
             float sz, cz, sw, cw;
             seg_sincos(q.z, sz, cz);
@@ -796,15 +874,15 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
             // xl, yl grow with the output column and row, so a staged tile's rectangle holds them
             // all: the staged variant reads only LDS
             if (ui.w > 0)
-                r = resolve_pixel_f32<S16, G::SP, true, false, true, false, FA, G::UCW>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
-                                                                               uvx, uvy, owned, sg, &spl, &ui, rx, ry);
+                r = resolve_pixel_f32<S16, G::SP, true, SEG, true, false, FA, G::UCW>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
+                                                                             uvx, uvy, owned, sg, &spl, &ui, rx, ry);
             else
-                r = resolve_pixel_f32<S16, G::SP, true, false, false, false, FA>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
-                                                                      owned, sg, &spl, &ui, rx, ry);
+                r = resolve_pixel_f32<S16, G::SP, true, SEG, false, false, FA>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d, uvx, uvy,
+                                                                    owned, sg, &spl, &ui, rx, ry);
         } else {
-            r = resolve_pixel_f32<S16, G::SP, false, SEG, false, G::VEL16>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
-                                                                            uvx, uvy, owned, sg, &spl, nullptr, 0, 0,
-                                                                            vel_staged ? sm.vel : nullptr);
+            r = resolve_pixel_f32<S16, G::SP, false, SEG, false, G::VEL16, FA>(fd, U, P, cx, cy, l, sm.c0, sm.c1, sm.c2, sm.d,
+                                                                                uvx, uvy, owned, sg, &spl, nullptr, 0, 0,
+                                                                                vel_staged ? sm.vel : nullptr);
         }
         if (owned) {
             const uint32_t gi = (uint32_t)(cy * pitch + cx);
@@ -853,13 +931,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3
     resolve_tile<G, S16, false, false, PROD>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
 }
 
-// the resolve with the segmentation mask of mRayTraceAugment
-template <int TW, int TH, int PAD, bool CAS, bool S16>
+// the resolve with the segmentation mask of mRayTraceAugment (FA: the FP16 path's arithmetic where
+// no selection depends on it)
+template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA>
 __global__ __launch_bounds__(256) void k_resolve_seg(const ResolveArgs a, const SegDesc sg) {
     using G = Geo<TW, TH, PAD, CAS, false, true>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
-    resolve_tile<G, S16, true>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm, &sg);
+    resolve_tile<G, S16, true, FA>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm, &sg);
+}
+
+// the upsampling resolve with the segmentation mask (its inputs read from global memory)
+template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA>
+__global__ __launch_bounds__(256) void k_resolve_up_seg(const ResolveArgs a, const SegDesc sg) {
+    using G = Geo<TW, TH, PAD, CAS, true>;
+    __shared__ typename G::Smem sm;
+    const int n = a.fd.tiles_x * a.fd.tiles_y;
+    resolve_tile<G, S16, true, FA>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm, &sg);
 }
 
 // Upsampling path (input size != output size, or mUpsampling): exact FP32 arithmetic, or with FA
@@ -1015,22 +1103,45 @@ int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, int arith, v
     return (int)e;
 }
 
-int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, void* stream) {
+template <bool UP, bool FA>
+hipError_t launch_resolve_seg_t(ResolveArgs a, const SegDesc& sg, hipStream_t st) {
+    FrameDesc& fd = a.fd;
+    const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
+    constexpr TileShapeDef C = kShapesCas[UP ? kUpShapeCas : kSegShapeCas], Pl = kShapesPlain[UP ? kUpShapePlain : kSegShapePlain];
+    const TileShapeDef d = fd.cas_on ? C : Pl;
+    fd.tiles_x = tiles_for(fd.x_end - fd.x_begin, d.tw);
+    fd.tiles_y = tiles_for(fd.H, d.th);
+    const int n = fd.tiles_x * fd.tiles_y;
+    if constexpr (UP) {
+        if (fd.cas_on) {
+            if (s16) hipLaunchKernelGGL((k_resolve_up_seg<C.tw, C.th, C.pad, true, true, FA>), dim3(n), dim3(256), 0, st, a, sg);
+            else hipLaunchKernelGGL((k_resolve_up_seg<C.tw, C.th, C.pad, true, false, FA>), dim3(n), dim3(256), 0, st, a, sg);
+        } else {
+            if (s16) hipLaunchKernelGGL((k_resolve_up_seg<Pl.tw, Pl.th, Pl.pad, false, true, FA>), dim3(n), dim3(256), 0, st, a, sg);
+            else hipLaunchKernelGGL((k_resolve_up_seg<Pl.tw, Pl.th, Pl.pad, false, false, FA>), dim3(n), dim3(256), 0, st, a, sg);
+        }
+    } else {
+        if (fd.cas_on) {
+            if (s16) hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, true, FA>), dim3(n), dim3(256), 0, st, a, sg);
+            else hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, false, FA>), dim3(n), dim3(256), 0, st, a, sg);
+        } else {
+            if (s16) hipLaunchKernelGGL((k_resolve_seg<Pl.tw, Pl.th, Pl.pad, false, true, FA>), dim3(n), dim3(256), 0, st, a, sg);
+            else hipLaunchKernelGGL((k_resolve_seg<Pl.tw, Pl.th, Pl.pad, false, false, FA>), dim3(n), dim3(256), 0, st, a, sg);
+        }
+    }
+    return hipGetLastError();
+}
+
+int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, int arith, void* stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     ResolveArgs a;
     a.fd = fd;
     a.U = U;
-    const int n = fd.tiles_x * fd.tiles_y;
-    const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
-    constexpr TileShapeDef C = kShapesCas[kSegShapeCas], Pl = kShapesPlain[kSegShapePlain];
-    if (fd.cas_on) {
-        if (s16) hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, true>), dim3(n), dim3(256), 0, st, a, sg);
-        else hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, false>), dim3(n), dim3(256), 0, st, a, sg);
-    } else {
-        if (s16) hipLaunchKernelGGL((k_resolve_seg<Pl.tw, Pl.th, Pl.pad, false, true>), dim3(n), dim3(256), 0, st, a, sg);
-        else hipLaunchKernelGGL((k_resolve_seg<Pl.tw, Pl.th, Pl.pad, false, false>), dim3(n), dim3(256), 0, st, a, sg);
-    }
-    return (int)hipGetLastError();
+    const bool fa = arith == TAA_ARITH_FP16;
+    hipError_t e;
+    if (fd.up) e = fa ? launch_resolve_seg_t<true, true>(a, sg, st) : launch_resolve_seg_t<true, false>(a, sg, st);
+    else e = fa ? launch_resolve_seg_t<false, true>(a, sg, st) : launch_resolve_seg_t<false, false>(a, sg, st);
+    return (int)e;
 }
 
 int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uint32_t peak_bits, void* stream) {

@@ -185,7 +185,8 @@ class SegmentationImages(C.Structure):
     """taa_segmentation_images_t: the segmentation mask's extra images (taa.comp:34-37)."""
     _fields_ = [
         ("material", C.c_void_p), ("prev_material", C.c_void_p), ("uv_normal", C.c_void_p),
-        ("prev_segmask", C.c_void_p), ("segmask", C.c_void_p), ("pitch", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("prev_segmask", C.c_void_p), ("segmask", C.c_void_p), ("pitch", C.c_int32), ("input_pitch", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

@@ -28,18 +28,20 @@ def vel_shape_dtype(vfmt):
 
 
 class FrameBuffers:
-    """Every image one resolve touches, as device tensors [H, W, C]."""
+    """Every image one resolve touches, as device tensors [H, W, C]. input_size (Wl, Hl): the
+    render size of colour, depth, velocity and previous depth when it differs from the output."""
 
-    def __init__(self, width, height, fmt, vfmt, debug=False, inspect=False, device="cuda"):
+    def __init__(self, width, height, fmt, vfmt, debug=False, inspect=False, device="cuda", input_size=None):
         self.W, self.H, self.fmt, self.vfmt = width, height, fmt, vfmt
+        self.Wl, self.Hl = input_size if input_size is not None else (width, height)
         cd = color_dtype(fmt)
         vc, vd = vel_shape_dtype(vfmt)
         z = lambda *s, dtype: torch.zeros(*s, dtype=dtype, device=device)
-        self.color = z(height, width, 4, dtype=cd)
-        self.depth = z(height, width, dtype=torch.float32)
-        self.velocity = z(height, width, vc, dtype=vd)
+        self.color = z(self.Hl, self.Wl, 4, dtype=cd)
+        self.depth = z(self.Hl, self.Wl, dtype=torch.float32)
+        self.velocity = z(self.Hl, self.Wl, vc, dtype=vd)
         self.history = [z(height, width, 4, dtype=cd), z(height, width, 4, dtype=cd)]
-        self.prev_depth = z(height, width, dtype=torch.float32)
+        self.prev_depth = z(self.Hl, self.Wl, dtype=torch.float32)
         self.screen = z(height, width, 4, dtype=cd)
         self.debug = z(height, width, 4, dtype=torch.float16) if debug else None
         self.inspect = z(height, width, INSPECT_WORDS, dtype=torch.int32) if inspect else None
@@ -72,6 +74,8 @@ class FrameBuffers:
         img.result_history = self.hist_out.data_ptr()
         img.inspect = self.inspect.data_ptr() if self.inspect is not None else None
         img.width, img.height, img.pitch = self.W, self.H, self.W
+        if (self.Wl, self.Hl) != (self.W, self.H):
+            img.input_width, img.input_height, img.input_pitch = self.Wl, self.Hl, self.Wl
         img.color_format, img.velocity_format = self.fmt, self.vfmt
         return img
 

@@ -125,9 +125,12 @@ def velocity(R, depth, ids=None, objects=None, dtype=np.float32, threads=0):
 
 def resolve_seg(U, color4, depth, vel4, hist4, hdepth, material, prev_material, uv_normal, prev_segmask,
                 mask_init=None, dtype=np.float32, threads=0):
-    """One frame of the oracle with the segmentation mask (one size). Returns the resolve dict plus
-    `segmask`: mask_init (default zeros) with the texels the reference stores replaced."""
-    H, W = depth.shape
+    """One frame of the oracle with the segmentation mask. The output size is the history's, the
+    input size the depth's (material ids and G-buffer at the input size, masks at the output
+    size). Returns the resolve dict plus `segmask`: mask_init (default zeros) with the texels the
+    reference stores replaced."""
+    Hl, Wl = depth.shape
+    H, W = np.shape(hist4)[:2]
     threads = threads or os.cpu_count() or 1
     f, u = np.float32, np.uint32
     arrs = [np.ascontiguousarray(a, t) for a, t in ((color4, f), (depth, f), (vel4, f), (hdepth, f))]
@@ -141,9 +144,9 @@ def resolve_seg(U, color4, depth, vel4, hist4, hdepth, material, prev_material, 
     L = lib()
     fn = L.oracle_resolve_seg_f32 if dtype == np.float32 else L.oracle_resolve_seg_f64
     fp, tp, up = C.POINTER(C.c_float), C.POINTER(T), C.POINTER(C.c_uint32)
-    fn.argtypes = [C.c_void_p, C.c_int, C.c_int, fp, fp, fp, tp, fp, tp, tp, tp, C.POINTER(C.c_int32), up, up, fp, up, up,
-                   C.c_int]
-    rc = fn(C.byref(U), W, H, _p(arrs[0], C.c_float), _p(arrs[1], C.c_float), _p(arrs[2], C.c_float), _p(hist4, T),
+    fn.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp, tp, fp, tp, tp, tp, C.POINTER(C.c_int32),
+                   up, up, fp, up, up, C.c_int]
+    rc = fn(C.byref(U), W, H, Wl, Hl, _p(arrs[0], C.c_float), _p(arrs[1], C.c_float), _p(arrs[2], C.c_float), _p(hist4, T),
             _p(arrs[3], C.c_float), _p(scr, T), _p(his, T), _p(dbg, T), _p(ins, C.c_int32), _p(mats[0], C.c_uint32),
             _p(mats[1], C.c_uint32), _p(uvn, C.c_float), _p(mats[2], C.c_uint32), _p(mask, C.c_uint32), threads)
     assert rc == 0

@@ -1,8 +1,11 @@
 """The resolve with the segmentation mask (taa_resolve_frame_segmented, mRayTraceAugment;
-taa.comp:589-702, 773-784, 932-938, 959) on the GPU against the FP32 transcription.
+taa.comp:589-702, 773-784, 932-938, 959) on the GPU against the FP32 transcription, at one size
+and upsampling (the mask evaluated at the output texel, its inputs at the render size).
 
 Bar: as the FP32 path — the mask words bit-exact, the weighted indicator sum bit-exact, texel
-indices, flags and box bit-exact, history and screen within 1e-5 relative (observed 0).
+indices, flags and box bit-exact, history and screen within 1e-5 relative (observed 0). In the
+FP16 arithmetic: the mask, the indicator sum, the texel indices, taps and decisions bit-exact, the
+sharpened screen > 55 dB against the transcription run on the GPU's previous history.
 """
 import ctypes as C
 
@@ -13,9 +16,9 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from gpu_helpers import assert_parity, round_storage, to_f16  # noqa: E402
-from taastar import (FMT_RGBA16F, FMT_RGBA32F, INSPECT, RTFLAG, VEL_RG16F, VEL_RG32F, SegmentationImages,  # noqa: E402
-                     check, lib)
+from gpu_helpers import INDEX_WORDS, assert_parity, psnr, round_storage, to_f16  # noqa: E402
+from taastar import (ARITH_FP16, FLAG_RECTIFIED, FMT_RGBA16F, FMT_RGBA32F, INSPECT, RTFLAG, VEL_RG16F,  # noqa: E402
+                     VEL_RG32F, SegmentationImages, check, lib)
 from taastar.device import FrameBuffers  # noqa: E402
 from taastar.workloads import (cas_for, decode, encode_color, encode_velocity, params_c2, scene_for,  # noqa: E402
                                segmentation_inputs, uniforms_for, velocity_as_read, workload)
@@ -31,20 +34,22 @@ class SegSequence:
     """Frames of one seeded scene with material ids and normals, resolved with the segmentation
     mask on the GPU and by the FP32 oracle side by side."""
 
-    def __init__(self, w, width, height, cas=True, params=None, split=None):
+    def __init__(self, w, width, height, cas=True, params=None, split=None, input_size=None, upsampling=False,
+                 arith=0):
         self.w, self.W, self.H = w, width, height
+        Wl, Hl = input_size if input_size is not None else (width, height)
         self.fmt, self.vfmt = w["fmt"], w["vel"]
-        self.scene = scene_for(w, width=width, height=height, pan_mode=1, pan=3.0)
+        self.scene = scene_for(w, width=Wl, height=Hl, pan_mode=1, pan=3.0)
         self.cas = cas_for(w, width, height) if cas else None
-        self.params, self.split = params, split
-        self.bufs = FrameBuffers(width, height, self.fmt, self.vfmt, debug=True, inspect=True)
+        self.params, self.split, self.upsampling, self.arith = params, split, upsampling, arith
+        self.bufs = FrameBuffers(width, height, self.fmt, self.vfmt, debug=True, inspect=True, input_size=(Wl, Hl))
         z = lambda *s, dtype: torch.zeros(*s, dtype=dtype, device="cuda")
-        self.mat = [z(height, width, dtype=torch.int32), z(height, width, dtype=torch.int32)]
-        self.uvn = z(height, width, 4, dtype=torch.float32)
+        self.mat = [z(Hl, Wl, dtype=torch.int32), z(Hl, Wl, dtype=torch.int32)]
+        self.uvn = z(Hl, Wl, 4, dtype=torch.float32)
         self.mask = [z(height, width, dtype=torch.int32), z(height, width, dtype=torch.int32)]
         self.hist_o = np.zeros((height, width, 4), np.float32)
-        self.prev_depth = np.zeros((height, width), np.float32)
-        self.prev_mat_o = np.zeros((height, width), np.uint32)
+        self.prev_depth = np.zeros((Hl, Wl), np.float32)
+        self.prev_mat_o = np.zeros((Hl, Wl), np.uint32)
         self.mask_o = np.zeros((height, width), np.uint32)
         self.k = 0
 
@@ -57,6 +62,7 @@ class SegSequence:
         self.mat[k & 1].copy_(torch.from_numpy(mat.view(np.int32)))
         self.uvn.copy_(torch.from_numpy(uvn))
         U = uniforms_for(self.w, k, jit, params=self.params)
+        U.mUpsampling = int(self.upsampling)
         if self.split is not None:
             U.splitScreen, U.splitX = 1, self.split
             U.param[1].mRayTraceAugment = 0
@@ -66,7 +72,7 @@ class SegSequence:
             self.mask[k & 1].data_ptr()
         img = self.bufs.images()
         check(lib().taa_resolve_frame_segmented(C.byref(U), C.byref(img), C.byref(sg),
-                                                C.byref(self.cas) if self.cas else None, 0, None))
+                                                C.byref(self.cas) if self.cas else None, self.arith, None))
         self.bufs.k += 1
         torch.cuda.synchronize()
         o = oracle.resolve_seg(U, decode(col_s), dep, velocity_as_read(vel_s), self.hist_o, self.prev_depth, mat,
@@ -81,7 +87,10 @@ class SegSequence:
                    gpu_mask=self.mask[k & 1].cpu().numpy().view(np.uint32),
                    oracle_history=o_hist, oracle_screen=o_scr, oracle_inspect=o["inspect"],
                    oracle_debug=to_f16(o["debug"]).astype(np.float32), oracle_mask=o["segmask"])
-        self.hist_o, self.prev_depth = o_hist, dep
+        # the FP16 arithmetic's history departs from the transcription's in its low bits: the
+        # oracle continues from the GPU's history, so that every frame is held to one step's bar
+        self.hist_o = out["gpu_history"] if self.arith == ARITH_FP16 else o_hist
+        self.prev_depth = dep
         self.prev_mat_o = mat
         self.mask_o = o["segmask"]
         self.k += 1
@@ -100,6 +109,19 @@ def assert_seg_parity(out, split=None):
     bad = np.count_nonzero(gm != om)
     assert bad == 0, f"segmentation mask: {bad} texels differ"
     assert np.array_equal(out["gpu_debug"], out["oracle_debug"])
+
+
+def assert_seg_fp16(out):
+    """The FP16 arithmetic's bar: mask, indicator sum, indices, taps and decisions exact; the
+    sharpened screen > 55 dB."""
+    gi, oi = out["gpu_inspect"], out["oracle_inspect"]
+    for wd in INDEX_WORDS + [INSPECT["SEG_TOTAL"]]:
+        g, o = gi[..., wd], oi[..., wd]
+        if wd == INSPECT["FLAGS"]:
+            g, o = g & ~FLAG_RECTIFIED, o & ~FLAG_RECTIFIED
+        assert np.count_nonzero(g != o) == 0, f"inspect word {wd}"
+    assert np.count_nonzero(out["gpu_mask"] != out["oracle_mask"]) == 0, "segmentation mask"
+    assert psnr(out["gpu_screen"][..., :3], out["oracle_screen"][..., :3]) > 55.0
 
 
 def seg_params(flags=None, **kw):
@@ -158,6 +180,49 @@ def test_segmentation_split_screen(oracle, cas):
         assert_seg_parity(seq.step(oracle), split=97)
 
 
+UP_CASES = {"2x": ((160, 96), (320, 192)), "1.5x": ((200, 120), (300, 180)), "1.33x_ragged": ((75, 41), (101, 57))}
+
+
+@pytest.mark.parametrize("fmt", [FMT_RGBA32F, FMT_RGBA16F])
+@pytest.mark.parametrize("name", sorted(UP_CASES))
+def test_segmentation_upsampled(oracle, name, fmt):
+    """TAAU (mUpsampling) with the mask: the reference's default flags, every texel's mask, sum and
+    resolve against the transcription."""
+    print(f"case seg upsampled {name} fmt={fmt}", flush=True)
+    lo, hi = UP_CASES[name]
+    w = workload("c2", fmt=fmt, vel=VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F)
+    seq = SegSequence(w, hi[0], hi[1], params=seg_params(), input_size=lo, upsampling=True)
+    cats = set()
+    for _ in range(4):
+        out = seq.step(oracle)
+        assert_seg_parity(out)
+        cats |= set(np.unique(out["gpu_mask"] & 3).tolist())
+    assert {0, 2} <= cats, cats
+
+
+@pytest.mark.parametrize("name", sorted(VARIANTS))
+def test_segmentation_upsampled_flag_variants(oracle, name):
+    """The flag variants on the upsampling path; 240x220 input, so that the fixed border (measured
+    against the input size) shows inside the 360x330 output."""
+    print(f"case seg upsampled variant {name}", flush=True)
+    v = dict(VARIANTS[name])
+    seq = SegSequence(workload("c2"), 360, 330, cas=False, params=seg_params(v.pop("flags"), **v), input_size=(240, 220),
+                      upsampling=True)
+    for _ in range(3):
+        assert_seg_parity(seq.step(oracle))
+
+
+@pytest.mark.parametrize("up", [False, True])
+def test_segmentation_fp16_arithmetic(oracle, up):
+    """TAA_ARITH_FP16 with the mask, at one size and upsampling: the mask and every selection exact."""
+    print(f"case seg fp16 up={up}", flush=True)
+    w = workload("c3")
+    seq = SegSequence(w, 320, 192, params=seg_params(), input_size=(160, 96) if up else None, upsampling=up,
+                      arith=ARITH_FP16)
+    for _ in range(4):
+        assert_seg_fp16(seq.step(oracle))
+
+
 def test_segmentation_refusals():
     from taastar._abi import TAA_ERR_INVALID_ARG as EINVAL, TAA_ERR_UNSUPPORTED as EUNSUP
     w = workload("c2")
@@ -173,7 +238,13 @@ def test_segmentation_refusals():
     sg.material = sg.prev_material = sg.prev_segmask = m.data_ptr()
     sg.uv_normal, sg.segmask = n.data_ptr(), m.data_ptr()
     assert L.taa_resolve_frame_segmented(C.byref(U), C.byref(img), C.byref(sg), None, 0, None) == EINVAL  # alias
-    img.input_width, img.input_height = 32, 16
     m2 = torch.zeros(32, 64, dtype=torch.int32, device="cuda")
     sg.segmask = m2.data_ptr()
-    assert L.taa_resolve_frame_segmented(C.byref(U), C.byref(img), C.byref(sg), None, 0, None) == EUNSUP  # TAAU
+    sg.input_pitch = -1
+    assert L.taa_resolve_frame_segmented(C.byref(U), C.byref(img), C.byref(sg), None, 0, None) == EINVAL
+    sg.input_pitch, sg.reserved[0] = 0, 1
+    assert L.taa_resolve_frame_segmented(C.byref(U), C.byref(img), C.byref(sg), None, 0, None) == EINVAL
+    sg.reserved[0] = 0
+    img.input_width, img.input_height = 32, 16
+    sg.input_pitch = 31  # narrower than the input width
+    assert L.taa_resolve_frame_segmented(C.byref(U), C.byref(img), C.byref(sg), None, 0, None) == EINVAL

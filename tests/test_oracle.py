@@ -932,6 +932,56 @@ def test_seg_decision_paths(oracle):
     assert np.all(m[:, :10] == 2 | (2 << 16)) and np.all(m[:, 10:20] == 2) and np.all(m[:, 20:] == 0)
 
 
+def seg_case_up(Hl, Wl, H, W, rng, flags, **weights):
+    """seg_case at the input size Hl x Wl with an output-sized (H x W) history and previous mask."""
+    c = seg_case(Hl, Wl, rng, flags, **weights)
+    c["hist"] = np.concatenate([rng.uniform(0.05, 2.0, (H, W, 3)), np.zeros((H, W, 1))], -1)
+    c["prev_mask"] = np.zeros((H, W), np.uint32)
+    return c
+
+
+def test_seg_upsampled_mask_at_output_coordinates(oracle):
+    """An input smaller than the output: calc_segmentation_value still runs at the output texel
+    (taa.comp:775) -- the fixed border against the input size (textureSize_loRes, :595), the
+    neighbours clamped to the input image (CLAMP_TO_TEX, :42), the centre's material id and normal
+    read at the output coordinates, zero outside the input image (robustImageAccess2's value for
+    that undefined read: material 0, normal of angles (0, 0) = (1, 0, 0)), the previous mask at the
+    output texel."""
+    R = taastar.RTFLAG
+    rng = np.random.default_rng(86)
+    Hl, Wl, H, W = 220, 240, 330, 360
+    m = run_seg(oracle, seg_case_up(Hl, Wl, H, W, rng, R["FXD"]))["segmask"]
+    yy, xx = np.mgrid[0:H, 0:W]
+    np.testing.assert_array_equal(m, ((xx < 100) | (yy < 100) | (xx >= Wl - 100) | (yy >= Hl - 100)).astype(np.uint32))
+
+    Hl, Wl, H, W = 24, 40, 48, 80
+    yy, xx = np.mgrid[0:H, 0:W]
+    inside = (xx < Wl) & (yy < Hl)
+    cx = lambda d: np.clip(xx + d, 0, Wl - 1)
+    cy = lambda d: np.clip(yy + d, 0, Hl - 1)
+    # material edges: own id (0 outside the input) against the clamped 4-neighbours
+    c = seg_case_up(Hl, Wl, H, W, rng, R["MID"], mid=1.0)
+    mat = c["mat"]
+    own = np.where(inside, mat[np.minimum(yy, Hl - 1), np.minimum(xx, Wl - 1)], 0)
+    edge = np.zeros((H, W), bool)
+    for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+        edge |= mat[cy(dy), cx(dx)] != own
+    np.testing.assert_array_equal(seg_total(run_seg(oracle, c)), edge.astype(np.float64))
+    assert edge[inside].any() and not edge[inside].all() and edge[~inside].all()
+    # normal dots: the centre's normal (1, 0, 0) outside the input
+    c = seg_case_up(Hl, Wl, H, W, rng, R["NRM"], nrm=1.0)
+    z, w = c["uvn"][..., 2].astype(np.float64), c["uvn"][..., 3].astype(np.float64)
+    n = np.stack([np.cos(z) * np.cos(w), np.sin(z) * np.cos(w), np.sin(w)], -1)
+    nc = np.where(inside[..., None], n[np.minimum(yy, Hl - 1), np.minimum(xx, Wl - 1)], np.array([1.0, 0.0, 0.0]))
+    dots = [np.sum(nc * n[cy(dy), cx(dx)], -1) for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1))]
+    np.testing.assert_allclose(seg_total(run_seg(oracle, c)), 1.0 - np.maximum(0.0, np.min(dots, 0)), atol=1e-6)
+    # history count from the output-sized previous mask
+    c = seg_case_up(Hl, Wl, H, W, rng, R["CNT"])
+    c["prev_mask"][:, 50:] = 2 | (3 << 16)
+    m = run_seg(oracle, c)["segmask"]
+    assert np.all(m[:, 50:] == 2 | (2 << 16)) and np.all(m[:, :50] == 0)
+
+
 def test_seg_passthrough_leaves_the_mask(oracle):
     rng = np.random.default_rng(84)
     c = seg_case(16, 20, rng, taastar.RTFLAG["ALL"])
