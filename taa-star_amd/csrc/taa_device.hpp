@@ -203,12 +203,15 @@ struct BilinearAxis {
     int a, b;
     float t, u;
 };
-__device__ __forceinline__ BilinearAxis bilinear_axis(int W, float u) {
+// `origin` receives the first tap before the edge clamp (pre-clamped to [-1, W] so that origin + 1
+// cannot overflow for absurd positions)
+__device__ __forceinline__ BilinearAxis bilinear_axis(int W, float u, int* origin = nullptr) {
     const float px = u * (float)W - 0.5f;
     const float fx = floorf(px);
     BilinearAxis r;
     r.t = px - fx;
     const int x0 = clampi(to_int(fx), -1, W);
+    if (origin) *origin = x0;
     r.a = clampi(x0, 0, W - 1);
     r.b = clampi(x0 + 1, 0, W - 1);
     r.u = 1.0f - r.t;
@@ -224,14 +227,19 @@ __device__ __forceinline__ BilinearTaps bilinear_taps(const BilinearAxis& x, con
 __device__ __forceinline__ BilinearTaps bilinear_taps(int W, int H, float u, float v) {
     return bilinear_taps(bilinear_axis(W, u), bilinear_axis(H, v));
 }
-template <class F>
-__device__ __forceinline__ float4 bilinear4_at(const BilinearTaps& t, F&& at) {
+// the filter of the four taps a (xa, ya), b (xb, ya), c (xa, yb), d (xb, yb): lerp in x, then in y
+__device__ __forceinline__ float4 bilinear4_quad(float4 a, float4 b, float4 c, float4 d, const BilinearTaps& t) {
     const float tx = t.tx, ty = t.ty, ux = t.ux, uy = t.uy;
-    float4 a = at(t.xa, t.ya), b = at(t.xb, t.ya);
-    float4 c = at(t.xa, t.yb), d = at(t.xb, t.yb);
     float4 top = make_float4(a.x * ux + b.x * tx, a.y * ux + b.y * tx, a.z * ux + b.z * tx, a.w * ux + b.w * tx);
     float4 bot = make_float4(c.x * ux + d.x * tx, c.y * ux + d.y * tx, c.z * ux + d.z * tx, c.w * ux + d.w * tx);
     return make_float4(top.x * uy + bot.x * ty, top.y * uy + bot.y * ty, top.z * uy + bot.z * ty, top.w * uy + bot.w * ty);
+}
+__device__ __forceinline__ float bilinear1_quad(float a, float b, float c, float d, const BilinearTaps& t) {
+    return (a * t.ux + b * t.tx) * t.uy + (c * t.ux + d * t.tx) * t.ty;
+}
+template <class F>
+__device__ __forceinline__ float4 bilinear4_at(const BilinearTaps& t, F&& at) {
+    return bilinear4_quad(at(t.xa, t.ya), at(t.xb, t.ya), at(t.xa, t.yb), at(t.xb, t.yb), t);
 }
 template <class F>
 __device__ __forceinline__ float4 bilinear4_at(int W, int H, float u, float v, F&& at) {
@@ -239,9 +247,7 @@ __device__ __forceinline__ float4 bilinear4_at(int W, int H, float u, float v, F
 }
 template <class F>
 __device__ __forceinline__ float bilinear1_at(const BilinearTaps& t, F&& at) {
-    float a = at(t.xa, t.ya), b = at(t.xb, t.ya);
-    float c = at(t.xa, t.yb), d = at(t.xb, t.yb);
-    return (a * t.ux + b * t.tx) * t.uy + (c * t.ux + d * t.tx) * t.ty;
+    return bilinear1_quad(at(t.xa, t.ya), at(t.xb, t.ya), at(t.xa, t.yb), at(t.xb, t.yb), t);
 }
 template <class F>
 __device__ __forceinline__ float bilinear1_at(int W, int H, float u, float v, F&& at) {

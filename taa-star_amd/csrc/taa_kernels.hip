@@ -292,16 +292,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         if constexpr (STG) return up->d[staged(xx, yy)];
         else return ld_f32(fd.depth, (uint32_t)(yy * pl + xx));
     };
-    // the taps of the bilinear depth and velocity samples: (x, y) as the tile's tables hold them --
-    // STG: the position in the staged rectangle, x - x0 and (y - y0) * UCW; else the clamped texel
-    auto depth_tap = [&](int xx, int yy) -> float {
-        if constexpr (STG) return up->d[yy + xx];
-        else return ld_f32(fd.depth, (uint32_t)(yy * pl + xx));
-    };
-    auto vel_tap_up = [&](int xx, int yy) -> float4 {
-        if constexpr (STG) return up->v[yy + xx];
-        else return ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(yy * pl + xx));
-    };
+    // the taps of the bilinear depth and velocity samples (non-staged tiles): the clamped texels
     // STG: the output texel's own input texel (xl, yl) in the staged rectangle
     const int sbase = STG ? staged(xl, yl) : 0;
     // velocity texel (x + dx, y + dy) clamped to the image (global index gidx), dx, dy in -1..1:
@@ -319,6 +310,28 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     // (SAMPLE_TEX at uv + toU * offset), from the tile's tables
     auto up_taps = [&](int dx, int dy) {
         return bilinear_taps(up->ax[(dx + 1) * up->rw + rx], up->ay[(dy + 1) * up->rh + ry]);
+    };
+    // bilinear depth and velocity samples at uv + (dx, dy) input texels. STG: a staged tile's tables
+    // hold the first tap's position p = (x - x0) + (y - y0) * UCW before the edge clamp, the other
+    // taps at p + 1, p + UCW, p + UCW + 1 -- the rectangle holds the clamped texel at every
+    // unclamped position, so these are the four texels of the clamped form
+    auto up_depth = [&](int dx, int dy) -> float {
+        const BilinearTaps t = up_taps(dx, dy);
+        if constexpr (STG) {
+            const float* q = up->d + (t.ya + t.xa);
+            return bilinear1_quad(q[0], q[1], q[UCW], q[UCW + 1], t);
+        } else {
+            return bilinear1_at(t, [&](int xx, int yy) { return ld_f32(fd.depth, (uint32_t)(yy * pl + xx)); });
+        }
+    };
+    auto up_vel = [&](int dx, int dy) -> float4 {
+        const BilinearTaps t = up_taps(dx, dy);
+        if constexpr (STG) {
+            const float4* q = up->v + (t.ya + t.xa);
+            return bilinear4_quad(q[0], q[1], q[UCW], q[UCW + 1], t);
+        } else {
+            return bilinear4_at(t, [&](int xx, int yy) { return ld_vel(fd.velocity, fd.vel_fmt, (uint32_t)(yy * pl + xx)); });
+        }
     };
     const float toUx = 1.0f / (float)Wl, toUy = 1.0f / (float)Hl;  // 1 / textureSize(depth, velocity)
 
@@ -448,7 +461,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     {
         // SAMPLE_TEX(uCurrentVelocity, uv): a texel fetch at one size, bilinear when upsampling
         float4 vs;
-        if constexpr (UP) vs = bilinear4_at(up_taps(0, 0), vel_tap_up);
+        if constexpr (UP) vs = up_vel(0, 0);
         else vs = vel_tap(0, 0, gi);
         bool canUseVelocity = !(P.mUseVelocityVectors == 0 || (P.mUseVelocityVectors == 1 && vs.w < 0.5f));
         if (canUseVelocity) {
@@ -456,7 +469,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
             if (P.mVelocitySampleMode == 1) {
                 // taa.comp:398-412: the longest vector is tracked but the last tap is assigned
                 float4 last;
-                if constexpr (UP) last = bilinear4_at(up_taps(1, 1), vel_tap_up);
+                if constexpr (UP) last = up_vel(1, 1);
                 else last = vel_tap(1, 1, (uint32_t)(min(y + 1, H - 1) * fd.pitch + min(x + 1, W - 1)));
                 vs.x = last.x;
                 vs.y = last.y;
@@ -464,12 +477,12 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
                 // findClosestUvAndZ_3x3 (taa.comp:371-389): scan order row-major, strict <
                 int best = 0;
                 if constexpr (UP) {  // depth taps at the output uv, interpolated
-                    float dC = bilinear1_at(up_taps(-1, -1), depth_tap);
+                    float dC = up_depth(-1, -1);
                     for (int k = 1; k < 9; ++k) {
-                        float d = bilinear1_at(up_taps(k % 3 - 1, k / 3 - 1), depth_tap);
+                        float d = up_depth(k % 3 - 1, k / 3 - 1);
                         if (d < dC) { dC = d; best = k; }
                     }
-                    vs = bilinear4_at(up_taps(best % 3 - 1, best / 3 - 1), vel_tap_up);
+                    vs = up_vel(best % 3 - 1, best / 3 - 1);
                 } else {
                     float dC = sd[l - SP - 1];
 #pragma unroll
@@ -560,7 +573,7 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
         const float eps = 1e-5f;
         auto mov = [&](int dx, int dy) {
             float4 v;
-            if constexpr (UP) v = bilinear4_at(up_taps(dx, dy), vel_tap_up);
+            if constexpr (UP) v = up_vel(dx, dy);
             else v = vel_tap(dx, dy, (uint32_t)(clampi(y + dy, 0, H - 1) * fd.pitch + clampi(x + dx, 0, W - 1)));
             return (fabsf(v.x) > eps || fabsf(v.y) > eps) && (fabsf(v.w) >= 0.5f);
         };
@@ -818,16 +831,12 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
                 const int c = col ? clampi(x0 - G::RA + j, 0, W - 1) : clampi(y0 - G::RA + j, 0, H - 1);
                 const float uv = col ? ((float)c + 0.5f) / (float)W : ((float)c + 0.5f) / (float)H;
                 if (q < 3) {
-                    BilinearAxis ba = col ? bilinear_axis(Wl, uv + toUx * (float)(q - 1))
-                                          : bilinear_axis(Hl, uv + toUy * (float)(q - 1));
-                    if (stage) {
-                        if (col) {
-                            ba.a -= ui.x0;
-                            ba.b -= ui.x0;
-                        } else {
-                            ba.a = (ba.a - ui.y0) * G::UCW;
-                            ba.b = (ba.b - ui.y0) * G::UCW;
-                        }
+                    int o0;
+                    BilinearAxis ba = col ? bilinear_axis(Wl, uv + toUx * (float)(q - 1), &o0)
+                                          : bilinear_axis(Hl, uv + toUy * (float)(q - 1), &o0);
+                    if (stage) {  // the first tap's position before the edge clamp; the second follows it
+                        ba.a = col ? o0 - ui.x0 : (o0 - ui.y0) * G::UCW;
+                        ba.b = col ? ba.a + 1 : ba.a + G::UCW;
                     }
                     (col ? sm.uax : sm.uay)[q * n + j] = ba;
                 } else {
