@@ -193,7 +193,7 @@ def time_resolve(name, arith_name, steps, warmup, frames, autotune=False, over=N
     r = dict(workload=name, width=W, height=H, arith=arith_name, steps=K, warmup=warmup,
              gpix_per_s=K * px / (total_ms * 1e-3) / 1e9, avg_ms=avg_ms, median_ms=per[K // 2],
              bytes_per_pixel=bpp, achieved_gbs=bpp * px / (avg_ms * 1e-3) / 1e9, wall_s=wall,
-             kernel={0: "exact", 1: "fp16", 2: "upsample"}[kern[0]],
+             kernel={0: "exact", 1: "fp16", 2: "upsample", 3: "upsample_fp16"}[kern[0]],
              tile_shape=L.taa_tile_shape_selected(W, H, w["fmt"], arith, int(cas is not None)), w=w, host=host)
     del cols, deps, vels, hist, screen
     torch.cuda.empty_cache()
