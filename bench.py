@@ -446,14 +446,14 @@ def extras(a):
     F16, F32 = taastar.FMT_RGBA16F, taastar.FMT_RGBA32F
     A16, A32 = taastar.ARITH_FP16, taastar.ARITH_FP32
     x = {}
-    for name, ar, st in (("c3", "fp32", 50), ("c2", "fp32", 100), ("c2", "fp16", 100)):
-        r = time_resolve(name, ar, st, 5, 4)
+    for name, ar, st in (("c3", "fp32", 200), ("c2", "fp32", 200), ("c2", "fp16", 200)):
+        r = time_resolve(name, ar, st, 20, 4)
         x[f"resolve_{name}_{r['width']}x{r['height']}_{ar}arith"] = {
             k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items() if k not in ("w", "host")}
         print(f"# {name} {ar}: {r['gpix_per_s']:.3f} Gpix/s", file=sys.stderr, flush=True)
     # the reference's own defaults (taa.hpp:31-76: min/max box, clamp, bilinear history, movers-only
     # velocity) with its RGBA16F velocity image (shader_cpu_common.h:77), at 4K on the FP16 kernel
-    r = time_resolve("c3", "fp16", 100, 5, 4, over=dict(vel=taastar.VEL_RGBA16F, params=taastar.default_params))
+    r = time_resolve("c3", "fp16", 200, 20, 4, over=dict(vel=taastar.VEL_RGBA16F, params=taastar.default_params))
     x["resolve_reference_defaults_3840x2160_rgba16f_velocity_fp16arith"] = {
         k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items() if k not in ("w", "host")}
     print(f"# reference defaults fp16: {r['gpix_per_s']:.3f} Gpix/s ({r['kernel']} kernel)", file=sys.stderr, flush=True)

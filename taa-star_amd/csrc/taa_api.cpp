@@ -780,6 +780,7 @@ struct BatchLayout {
     int shape = 0;  // tile shape of the main launch
     bool cas = false, s16 = false, fast = false;
     int form = 0;  // the FP16 kernel's form (taastar::fast_path_form): the largest over the frames
+    bool prod_main = false, prod_up = false;  // every frame of that launch has the production feature set
     int n_main = 0, total_main = 0;
     int n_up = 0, total_up = 0;
 };
@@ -836,6 +837,10 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
         L.form = std::max(L.form, f);
     }
     if (L.fast && n_rgba != 0 && n_rgba != L.n_main) L.fast = false;
+    L.prod_main = !L.fast && L.n_main > 0;
+    for (int i = 0; i < L.n_main && L.prod_main; ++i) L.prod_main = taastar::exact_prod_form(ents[i].fd, ents[i].U);
+    L.prod_up = L.n_up > 0;
+    for (int i = L.n_main; i < L.n_main + L.n_up && L.prod_up; ++i) L.prod_up = taastar::prod_params(ents[i].fd, ents[i].U);
     return TAA_OK;
 }
 
@@ -845,11 +850,12 @@ static int launch_layout(const taastar::BatchEntry* d_ents, const BatchLayout& L
         const int grid = taastar::batch_grid(L.total_main);
         int le = L.fast ? taastar::launch_batch_fast(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, L.form, grid,
                                                      stream)
-                        : taastar::launch_batch(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, arith, grid, stream);
+                        : taastar::launch_batch(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, arith, L.prod_main,
+                                                grid, stream);
         if (le) return le;
     }
     if (L.n_up)
-        return taastar::launch_batch_up(d_ents + L.n_main, L.n_up, L.total_up, L.cas, L.s16, arith,
+        return taastar::launch_batch_up(d_ents + L.n_main, L.n_up, L.total_up, L.cas, L.s16, arith, L.prod_up,
                                         taastar::batch_grid(L.total_up), stream);
     return 0;
 }
@@ -919,7 +925,8 @@ void plan_free(taa_batch_plan* p) {
 }
 
 bool same_launches(const BatchLayout& a, const BatchLayout& b) {
-    return a.shape == b.shape && a.fast == b.fast && a.form == b.form && a.n_main == b.n_main &&
+    return a.shape == b.shape && a.fast == b.fast && a.form == b.form && a.prod_main == b.prod_main &&
+           a.prod_up == b.prod_up && a.n_main == b.n_main &&
            a.total_main == b.total_main &&
            a.n_up == b.n_up && a.total_up == b.total_up;
 }

@@ -102,12 +102,13 @@ int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uin
 // faster (55.7 vs 48.1 Gpix/s) than an occupancy-sized persistent grid with static strides.
 inline int batch_grid(int total_tiles) { return (total_tiles + 7) & ~7; }
 
+// prod: every frame has the production feature set (exact_prod_form): the production form
 int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas,
-                 bool s16, int arith, int grid, void* stream);
+                 bool s16, int arith, bool prod, int grid, void* stream);
 // the upsampling frames of a batch (the upsampling resolve in the batch's arithmetic, kUpShapeCas /
-// kUpShapePlain tiles)
-int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int arith, int grid,
-                    void* stream);
+// kUpShapePlain tiles; prod: every frame has the production feature set, prod_params)
+int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int arith, bool prod,
+                    int grid, void* stream);
 
 // Throughput kernel (taa_fast.hip): variance clip + Catmull-Rom + simple/closest-depth velocity
 // with RG velocity, fp16-fed mixed arithmetic. fast_path_supported() says whether a parameter

@@ -974,9 +974,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FA && S16 ?
 // Batched resolve: the grid walks the flattened tile space of every frame in the batch with a
 // grid stride (one tile per workgroup at the launch size batch_grid() picks). XCD k takes the
 // k-th contiguous eighth of the tiles.
-template <int TW, int TH, int PAD, bool CAS, bool S16>
+// PROD: every frame of the batch has the production feature set (exact_prod_form); its RG16F
+// velocity is then staged like k_resolve's.
+template <int TW, int TH, int PAD, bool CAS, bool S16, bool PROD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3))) void k_resolve_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
-    using G = Geo<TW, TH, PAD, CAS>;
+    using G = Geo<TW, TH, PAD, CAS, false, false, PROD && S16>;
     __shared__ typename G::Smem sm;
     const int per = (total_tiles + 7) >> 3;
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
@@ -984,14 +986,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3
         const int t = xcd * per + j;
         if (t >= total_tiles) break;
         const BatchEntry& e = batch_entry(entries, n_entries, t);
-        resolve_tile<G, S16>(e.fd, e.U, t - e.tile_begin, sm);
+        resolve_tile<G, S16, false, false, PROD>(e.fd, e.U, t - e.tile_begin, sm);
         __syncthreads();  // LDS is reused by the next tile
     }
 }
 
 // Batched upsampling resolve: the frames of a batch whose input size differs from the output size
 // (or that set mUpsampling), walked like k_resolve_batch, each tile as in k_resolve_up.
-template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA>
+// PROD: every frame of the batch has the production feature set (prod_params)
+template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA, bool PROD>
 __global__ __launch_bounds__(256) void k_resolve_up_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = Geo<TW, TH, PAD, CAS, true>;
     __shared__ typename G::Smem sm;
@@ -1001,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_resolve_up_batch(const BatchEntry* __re
         const int t = xcd * per + j;
         if (t >= total_tiles) break;
         const BatchEntry& e = batch_entry(entries, n_entries, t);
-        resolve_tile<G, S16, false, FA>(e.fd, e.U, t - e.tile_begin, sm);
+        resolve_tile<G, S16, false, FA, PROD>(e.fd, e.U, t - e.tile_begin, sm);
         __syncthreads();  // LDS is reused by the next tile
     }
 }
@@ -1037,14 +1040,14 @@ hipError_t launch_resolve_t(const FrameDesc& fd, const taa_uniforms_t& U, int sh
     return hipGetLastError();
 }
 
-template <bool CAS, bool S16>
+template <bool CAS, bool S16, bool PROD>
 hipError_t launch_batch_t(const BatchEntry* e, int n, int total, int shape, int grid, hipStream_t st) {
     constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
     switch (shape) {
-        case 0: hipLaunchKernelGGL((k_resolve_batch<S[0].tw, S[0].th, S[0].pad, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
-        case 1: hipLaunchKernelGGL((k_resolve_batch<S[1].tw, S[1].th, S[1].pad, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
-        case 2: hipLaunchKernelGGL((k_resolve_batch<S[2].tw, S[2].th, S[2].pad, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
-        default: hipLaunchKernelGGL((k_resolve_batch<S[3].tw, S[3].th, S[3].pad, CAS, S16>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 0: hipLaunchKernelGGL((k_resolve_batch<S[0].tw, S[0].th, S[0].pad, CAS, S16, PROD>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 1: hipLaunchKernelGGL((k_resolve_batch<S[1].tw, S[1].th, S[1].pad, CAS, S16, PROD>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        case 2: hipLaunchKernelGGL((k_resolve_batch<S[2].tw, S[2].th, S[2].pad, CAS, S16, PROD>), dim3(grid), dim3(256), 0, st, e, n, total); break;
+        default: hipLaunchKernelGGL((k_resolve_batch<S[3].tw, S[3].th, S[3].pad, CAS, S16, PROD>), dim3(grid), dim3(256), 0, st, e, n, total); break;
     }
     return hipGetLastError();
 }
@@ -1161,36 +1164,49 @@ int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uin
     return (int)hipGetLastError();
 }
 
-template <bool FA>
+template <bool FA, bool PROD>
 hipError_t launch_batch_up_t(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int grid,
                              hipStream_t st) {
     constexpr TileShapeDef C = kShapesCas[kUpShapeCas], Pl = kShapesPlain[kUpShapePlain];
     if (cas) {
-        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, true, FA>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
-        else hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, false, FA>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, true, FA, PROD>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        else hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, false, FA, PROD>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
     } else {
-        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, true, FA>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
-        else hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, false, FA>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        if (s16) hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, true, FA, PROD>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
+        else hipLaunchKernelGGL((k_resolve_up_batch<Pl.tw, Pl.th, Pl.pad, false, false, FA, PROD>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
     }
     return hipGetLastError();
 }
 
-int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int arith, int grid,
-                    void* stream) {
+int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int arith, bool prod,
+                    int grid, void* stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
-    return (int)(arith == TAA_ARITH_FP16 ? launch_batch_up_t<true>(d_entries, n_entries, total_tiles, cas, s16, grid, st)
-                                         : launch_batch_up_t<false>(d_entries, n_entries, total_tiles, cas, s16, grid, st));
+    hipError_t e;
+    if (arith == TAA_ARITH_FP16)
+        e = prod ? launch_batch_up_t<true, true>(d_entries, n_entries, total_tiles, cas, s16, grid, st)
+                 : launch_batch_up_t<true, false>(d_entries, n_entries, total_tiles, cas, s16, grid, st);
+    else
+        e = prod ? launch_batch_up_t<false, true>(d_entries, n_entries, total_tiles, cas, s16, grid, st)
+                 : launch_batch_up_t<false, false>(d_entries, n_entries, total_tiles, cas, s16, grid, st);
+    return (int)e;
 }
 
 int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, int arith,
-                 int grid, void* stream) {
+                 bool prod, int grid, void* stream) {
     (void)arith;
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e;
-    if (cas) e = s16 ? launch_batch_t<true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
-                     : launch_batch_t<true, false>(d_entries, n_entries, total_tiles, shape, grid, st);
-    else e = s16 ? launch_batch_t<false, true>(d_entries, n_entries, total_tiles, shape, grid, st)
-                 : launch_batch_t<false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+    if (prod) {
+        if (cas) e = s16 ? launch_batch_t<true, true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
+                         : launch_batch_t<true, false, true>(d_entries, n_entries, total_tiles, shape, grid, st);
+        else e = s16 ? launch_batch_t<false, true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
+                     : launch_batch_t<false, false, true>(d_entries, n_entries, total_tiles, shape, grid, st);
+    } else {
+        if (cas) e = s16 ? launch_batch_t<true, true, false>(d_entries, n_entries, total_tiles, shape, grid, st)
+                         : launch_batch_t<true, false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+        else e = s16 ? launch_batch_t<false, true, false>(d_entries, n_entries, total_tiles, shape, grid, st)
+                     : launch_batch_t<false, false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+    }
     return (int)e;
 }
 
