@@ -27,6 +27,8 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC = {
     # profiles/k_resolve_fast_3840x2160_fp16_cas.md: (126,148.2 KiB x 1.99976 + 129,770.9 KiB x 1.0) x 1024
     ("c3", "fp16"): 391205776,
+    # profiles/k_resolve_3840x2160_rgba16f_exact_fp32.md: (126,024.0 KiB x 1.99982 + 129,668.2 KiB x 1.0) x 1024
+    ("c3", "fp32"): 390854160,
 }
 
 
