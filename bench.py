@@ -202,6 +202,49 @@ def time_resolve(name, arith_name, steps, warmup, frames, autotune=False, over=N
     return r
 
 
+def accuracy_vs_oracle(w, frames_host, arith_name, frames=4):
+    """The measured arithmetic against the FP64 oracle on the bench sequence's first frames (from a
+    reset history), with the oracle as the checker (CPU-baseline leg): per frame, the full-frame
+    PSNR of the sharpened screen (display-referred, peak 1.0), the PSNR outside the reference's own
+    divergence footprint (the unclamped Lottes weight, taa.comp:865-869) and the FP32 transcription
+    on the same storage for comparison -- the figures the FP16 parity tests hold to their bar."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import torch
+    import taastar
+    import _oracle
+    from gpu_helpers import FP16OracleChain
+    from taastar.device import FrameBuffers
+    from taastar.workloads import cas_for, decode, uniforms_for
+    W, H = w["width"], w["height"]
+    arith = taastar.ARITH_FP16 if arith_name == "fp16" else taastar.ARITH_FP32
+    k = cas_for(w)
+    bufs = FrameBuffers(W, H, w["fmt"], w["vel"])
+    chain = FP16OracleChain(_oracle, W, H, k, "bench accuracy")
+    for f in range(frames):
+        col_s, dep, vel_s, jit = frames_host[f % len(frames_host)]
+        U = uniforms_for(w, f, jit)
+        bufs.upload(col_s, dep, vel_s)
+        bufs.resolve(U, k, arith)
+        torch.cuda.synchronize()
+        chain.measure(U, col_s, dep, vel_s, decode(bufs.screen.cpu().numpy()))
+    del bufs
+    torch.cuda.empty_cache()
+    reps = chain.reports
+    r2 = lambda v: round(float(v), 2)
+    return dict(
+        reference="FP64 oracle (the scalar transcription of taa.comp + CasFilter in double precision)",
+        frames=f"frames 0-{frames - 1} of the bench sequence, reset history, {W}x{H}, {arith_name} arithmetic",
+        full_frame_psnr_db_min=r2(min(r["full_frame_psnr_db"] for r in reps)),
+        outside_footprint_psnr_db_min=r2(min(r["screen_psnr_db"] for r in reps)),
+        fp32_transcription_full_frame_psnr_db_min=r2(min(r["fp32_fp16storage_psnr_db"] for r in reps)),
+        footprint_pixels_max=max(r["masked_pixels"] for r in reps),
+        per_frame=[dict(frame=r["frame"], full_frame_psnr_db=r2(r["full_frame_psnr_db"]),
+                        outside_footprint_psnr_db=r2(r["screen_psnr_db"]),
+                        fp32_transcription_psnr_db=r2(r["fp32_fp16storage_psnr_db"]),
+                        footprint_pixels=r["masked_pixels"]) for r in reps])
+
+
 def time_velocity(W, H, vfmt_name, objects, steps, warmup):
     """K back-to-back velocity-producer launches (taa_velocity_from_depth) over one W x H depth
     image resident in HBM, on a private stream, timed with HIP events on that stream."""
@@ -678,6 +721,7 @@ def main():
         out.update(serving_summary(out["extras"], r["gpix_per_s"]))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, r["host"][:2], a.cpu_seconds)
+        out["accuracy"] = accuracy_vs_oracle(w, r["host"], a.arith)
     print(json.dumps(out), flush=True)
 
 

@@ -182,6 +182,15 @@ class FP16OracleChain:
         self.reports = []
 
     def check(self, U, col_s, dep, vel_s, g_scr, g_ins=None):
+        rep = self.measure(U, col_s, dep, vel_s, g_scr, g_ins)
+        f, p_scr, p_all, p_base = rep["frame"], rep["screen_psnr_db"], rep["full_frame_psnr_db"], rep["fp32_fp16storage_psnr_db"]
+        assert p_scr >= FP16_PSNR_MIN_DB, f"{self.name} frame {f}: screen PSNR {p_scr:.2f} dB < {FP16_PSNR_MIN_DB}: {rep}"
+        assert p_all >= min(p_base - 3.0, 60.0), f"{self.name} frame {f}: full-frame {p_all:.2f} dB vs FP32 path {p_base:.2f} dB"
+        assert rep["masked_pixels"] <= FP16_MASK_MAX_FRACTION * self.W * self.H, rep
+        return rep
+
+    def measure(self, U, col_s, dep, vel_s, g_scr, g_ins=None):
+        """One frame's figures (the report check() holds to the bar); advances the chains."""
         from scipy.ndimage import binary_dilation
         from taastar import FLAG_RECTIFIED
         oracle, k, f = self.oracle, self.cas, self.frame
@@ -212,15 +221,12 @@ class FP16OracleChain:
         p_base = psnr(s32[..., :3], s64[..., :3], 1.0)
         p_scr = psnr(g_scr[stable][:, :3], s64[stable][:, :3], 1.0) if stable.any() else float("inf")
         err = float(np.abs(g_scr[stable][:, :3] - s64[stable][:, :3]).max()) if stable.any() else 0.0
-        rep = dict(frame=f, screen_psnr_db=round(p_scr, 2), screen_max_abs=err, full_frame_psnr_db=round(p_all, 2),
-                   fp32_fp16storage_psnr_db=round(p_base, 2), masked_pixels=int(unstable.sum()))
+        rep = dict(frame=f, screen_psnr_db=p_scr, screen_max_abs=err, full_frame_psnr_db=p_all,
+                   fp32_fp16storage_psnr_db=p_base, masked_pixels=int(unstable.sum()))
         self.reports.append(rep)
         print(f"{self.name} {self.W}x{self.H} frame {f}: PSNR {p_scr:.2f} dB outside {int(unstable.sum())} masked "
               f"pixels, full frame {p_all:.2f} dB (FP32 transcription on FP16 storage: {p_base:.2f} dB)", flush=True)
         self.worst = min(self.worst, p_scr)
-        assert p_scr >= FP16_PSNR_MIN_DB, f"{self.name} frame {f}: screen PSNR {p_scr:.2f} dB < {FP16_PSNR_MIN_DB}: {rep}"
-        assert p_all >= min(p_base - 3.0, 60.0), f"{self.name} frame {f}: full-frame {p_all:.2f} dB vs FP32 path {p_base:.2f} dB"
-        assert unstable.sum() <= FP16_MASK_MAX_FRACTION * self.W * self.H, rep
         self.h64 = o64["history"]
         self.h32 = to_f16(o32["history"]).astype(np.float32)
         self.prev = dep
