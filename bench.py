@@ -238,9 +238,11 @@ def accuracy_vs_oracle(w, frames_host, arith_name, frames=4):
         full_frame_psnr_db_min=r2(min(r["full_frame_psnr_db"] for r in reps)),
         outside_footprint_psnr_db_min=r2(min(r["screen_psnr_db"] for r in reps)),
         fp32_transcription_full_frame_psnr_db_min=r2(min(r["fp32_fp16storage_psnr_db"] for r in reps)),
+        outside_footprint_max_abs=float(f"{max(r['screen_max_abs'] for r in reps):.3g}"),
         footprint_pixels_max=max(r["masked_pixels"] for r in reps),
         per_frame=[dict(frame=r["frame"], full_frame_psnr_db=r2(r["full_frame_psnr_db"]),
                         outside_footprint_psnr_db=r2(r["screen_psnr_db"]),
+                        outside_footprint_max_abs=float(f"{r['screen_max_abs']:.3g}"),
                         fp32_transcription_psnr_db=r2(r["fp32_fp16storage_psnr_db"]),
                         footprint_pixels=r["masked_pixels"]) for r in reps])
 
