@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "taa_footprint.hpp"
 #include "taa_internal.hpp"
 
 namespace taastar {
@@ -329,10 +330,10 @@ __device__ float4 history_catmullrom(const FrameDesc& fd, float hu, float hv) {
                    -1.5f * f3x + 2.0f * f2x + 0.5f * fx, 0.5f * f3x - 0.5f * f2x};
     float wy[4] = {-0.5f * f3y + f2y - 0.5f * fy, 1.5f * f3y - 2.5f * f2y + 1.0f,
                    -1.5f * f3y + 2.0f * f2y + 0.5f * fy, 0.5f * f3y - 0.5f * f2y};
-    int kx = clampi(to_int(fbx), -2, W + 1), ky = clampi(to_int(fby), -2, H + 1);
-    // kx + 2 cannot overflow (kx is clamped to [-2, W + 1]). Written as kx <= W - 3 the same test
-    // put the batched RGBA32F launch into a schedule 18% slower (4.88 -> 5.93 ms per 48-frame round).
-    const bool interior = kx >= 1 && kx + 2 <= W - 1 && ky >= 1 && ky + 2 <= H - 1;
+    int kx = cr_base(to_int(fbx), W), ky = cr_base(to_int(fby), H);
+    // cr_inside forms kx + 2 of the pre-clamped base. Written as kx <= W - 3 the same test put the
+    // batched RGBA32F launch into a schedule 18% slower (4.88 -> 5.93 ms per 48-frame round).
+    const bool interior = cr_inside(kx, W) && cr_inside(ky, H);
     int xs[4], ys[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {

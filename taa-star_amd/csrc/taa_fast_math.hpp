@@ -75,9 +75,8 @@ __device__ __forceinline__ Hist history_cr_fast(const void* hist, int W, int H, 
     f3 acc = {0.f, 0.f, 0.f};
     float acca = 0.f;
     // the whole 4x4 footprint inside the image. kx, ky come unclamped from to_int(floor(...)) and
-    // saturate at INT_MIN / INT_MAX for huge or infinite history positions, so the test is written
-    // without forming kx + 2 (which would wrap and pass for kx = INT_MAX)
-    const bool interior = kx >= 1 && kx <= W - 3 && ky >= 1 && ky <= H - 3;
+    // saturate at INT_MIN / INT_MAX for huge or infinite history positions: the raw form of the test
+    const bool interior = cr_inside_raw(kx, W) && cr_inside_raw(ky, H);
     if (interior) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -112,7 +111,7 @@ __device__ __forceinline__ Hist history_cr_fast(const void* hist, int W, int H, 
             if constexpr (ALPHA) acca = __builtin_fmaf(rowa, wy.w[j], acca);
         }
     } else {
-        const int kxc = clampi(kx, -2, W + 1), kyc = clampi(ky, -2, H + 1);
+        const int kxc = cr_base(kx, W), kyc = cr_base(ky, H);
         int xs[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) xs[i] = clampi(kxc - 1 + i, 0, W - 1);
