@@ -64,8 +64,10 @@ enum { TAA_VEL_RG32F = 0, TAA_VEL_RG16F = 1, TAA_VEL_RGBA32F = 2, TAA_VEL_RGBA16
  *         velocity-based alpha, Lottes weighting, the near-clamp reduction, the debug image and
  *         fused CAS, with RG or RGBA velocity of the colour storage's precision -- the reference's
  *         default block with its RGBA16F velocity included. Shaped or chroma-shrunk boxes, slow
- *         clip, B-spline history, Karis tone mapping, unjitter, noise, pass-through, bypass and
- *         upsampling are resolved by the FP32 path (taa_resolve_kernel tells which). */
+ *         clip, B-spline history, Karis tone mapping, unjitter, noise, pass-through and bypass
+ *         are resolved by the FP32 path (taa_resolve_kernel tells which). Upsampling runs the
+ *         upsampling resolve in the FP16 path's arithmetic (TAA_KERNEL_UPSAMPLE_FP16), with every
+ *         sample position, texel index and decision exact. */
 enum { TAA_ARITH_FP32 = 0, TAA_ARITH_FP16 = 1 };
 /* the kernel a resolve runs (taa_resolve_kernel) */
 enum {
@@ -293,8 +295,8 @@ typedef struct taa_edge_aa_consts {
  * NULL, the CAS pass of source/taa.hpp:1127-1134 fused into the same kernel: result_screen then
  * receives the sharpened screen, exactly what the reference's separate pass would write.
  * Stream-ordered and asynchronous.
- * An input size different from the output size, or mUpsampling, takes the upsampling path (exact
- * FP32 arithmetic whatever `arith` asks). TAA_ERR_UNSUPPORTED: mRayTraceAugment (it needs the
+ * An input size different from the output size, or mUpsampling, takes the upsampling path (in the
+ * arithmetic `arith` asks). TAA_ERR_UNSUPPORTED: mRayTraceAugment (it needs the
  * extra images of taa_resolve_frame_segmented).
  * Split screen (uniforms->splitScreen): columns x > splitX use param[1], the others param[0]
  * (taa.comp:717). With CAS the frame is sharpened after both parts are resolved, in the
@@ -443,8 +445,9 @@ typedef struct taa_batch_item {
  * (TAA_ERR_CAPACITY beyond). Sessions must be distinct, and so must the
  * result_screen pointers; no result_screen may lie inside a batched session's buffers
  * (TAA_ERR_INVALID_ARG). Stream-ordered.
- * Upsampling frames (scaled sessions, mUpsampling) go through a second batched launch of the exact
- * FP32 upsampling resolve in the same call (and in the same captured graph of a plan). */
+ * Upsampling frames (scaled sessions, mUpsampling) go through a second batched launch of the
+ * upsampling resolve, in the call's arithmetic, in the same call (and in the same captured graph of
+ * a plan). */
 int32_t taa_batch_submit(taa_ctx_t* ctx, int32_t n, const taa_batch_item_t* items,
                          int32_t arith, void* stream);
 
