@@ -396,7 +396,7 @@ void session_desc(taa_session* s, void* screen, const taa_cas_consts_t* cas, taa
     fd.up = (s->Wl != s->W || s->Hl != s->H) ? 1 : 0;
 }
 
-// upsampling frames are not batched: they run their own launch (exact FP32 upsampling kernel)
+// frames that take the upsampling resolve (a launch of their own in a batch, in the batch's arithmetic)
 bool session_upsamples(const taa_session* s, const taa_uniforms_t* u) {
     return s->Wl != s->W || s->Hl != s->H || u->mUpsampling;
 }
