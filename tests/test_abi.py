@@ -34,7 +34,7 @@ def test_every_declared_symbol_is_exported(taastar_lib):
 
 
 def test_struct_layouts_match_reference():
-    # source/taa.hpp:30-77, 113-129, 88-91 (offsets verified by the survey's layout probe)
+    # source/taa.hpp:30-77, 113-129, 88-91 (the std140 offsets of the GLSL mirror, taa.comp:50-118)
     assert C.sizeof(taastar.Params) == 176
     assert C.sizeof(taastar.Uniforms) == 544
     assert C.sizeof(taastar.CasConsts) == 32
