@@ -47,7 +47,11 @@ struct Geo {
     static constexpr int UCW = UP ? RW + 8 : 1, UCH = UP ? RH + 8 : 1;
     struct Smem {
         float c0[UP ? 1 : SH * SP], c1[UP ? 1 : SH * SP], c2[UP ? 1 : SH * SP], d[UP ? 1 : SH * SP];
-        float r0[CAS ? RH * RP : 1], r1[CAS ? RH * RP : 1], r2[CAS ? RH * RP : 1];
+        // the resolved region for the fused CAS; the FP16 arithmetic on FP16 storage keeps each
+        // texel's stored fp16 bits (8 bytes) in the space of r0 and r1 instead (packed())
+        alignas(8) float r0[CAS ? RH * RP : 1];
+        float r1[CAS ? RH * RP : 1], r2[CAS ? RH * RP : 1];
+        __device__ uint2* packed() { return reinterpret_cast<uint2*>(r0); }
         float ux[SW], uy[SH];  // uv of the tile's (clamped) columns and rows
         // segmentation inputs of the staged region: unit normal, raw YCoCg luma, material id
         float nx[SEG ? SH * SP : 1], ny[SEG ? SH * SP : 1], nz[SEG ? SH * SP : 1], lum[SEG ? SH * SP : 1];
@@ -900,12 +904,16 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
         }
         if constexpr (G::CAS) {
             // the CAS pass reads the stored screen image: round to the storage format first
-            f3 s = r.screen;
-            if constexpr (S16) s = {round_f16(s.x), round_f16(s.y), round_f16(s.z)};
             const int li = ry * G::RP + rx;
-            sm.r0[li] = s.x;
-            sm.r1[li] = s.y;
-            sm.r2[li] = s.z;
+            if constexpr (FA && S16) {  // its fp16 bits, for the FP16 path's packed-half CAS
+                sm.packed()[li] = make_uint2(pack_f2h(r.screen.x, r.screen.y), f2h(r.screen.z));
+            } else {
+                f3 s = r.screen;
+                if constexpr (S16) s = {round_f16(s.x), round_f16(s.y), round_f16(s.z)};
+                sm.r0[li] = s.x;
+                sm.r1[li] = s.y;
+                sm.r2[li] = s.z;
+            }
         }
     }
 
@@ -918,6 +926,12 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
             const int px = x0 + ox, py = y0 + oy;
             if (px >= fd.x_end || py >= H) continue;
             const int li = (oy + 1) * G::RP + (ox + 1);
+            if constexpr (FA && S16) {
+                const uint2* q = sm.packed();
+                gstore<uint2>(fd.screen, (uint32_t)(py * pitch + px) * 8u,
+                              cas_filter_h2(q[li - G::RP], q[li - 1], q[li], q[li + 1], q[li + G::RP], peak));
+                continue;
+            }
             auto at = [&](int k) { return f3{sm.r0[k], sm.r1[k], sm.r2[k]}; };
             f3 o = FA ? cas_filter_fma(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak)
                       : cas_filter(at(li - G::RP), at(li - 1), at(li), at(li + 1), at(li + G::RP), peak);
