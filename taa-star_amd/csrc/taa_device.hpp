@@ -282,24 +282,20 @@ __device__ __forceinline__ float upsample_axis(float l, float h, float tj, int t
 }
 constexpr float kUpsampleCorner = 0.999999f;  // the second corner offset of the search (taa.comp:236-239)
 
-// The sample of the four corners (0,0), (c,0), (0,c), (c,c) -- the last that lands in the output
-// texel on both axes -- fetched at its texel through `color_at(x, y)` (converted colour, clamped
-// coordinates); beta = 0 when there is none. sx/hx: upsample_axis at offsets 0 and c for the
-// column, sy/hy for the row.
-template <class F>
-__device__ f3 current_upsampled(const float (&sx)[2], const bool (&hx)[2], const float (&sy)[2], const bool (&hy)[2],
-                                int Wl, int Hl, float& beta, F&& color_at) {
-    float fx = -1.0f, fy = -1.0f;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (hx[k & 1] && hy[k >> 1]) { fx = sx[k & 1]; fy = sy[k >> 1]; }
-    if (fx >= 0.0f) {
-        beta = 1.0f;
-        const int ix = clampi(to_int(floorf(fx)), 0, Wl - 1), iy = clampi(to_int(floorf(fy)), 0, Hl - 1);
-        return color_at(ix, iy);
-    }
-    beta = 0.0f;
-    return f3{0.0f, 0.0f, 0.0f};
+// getCurrentUpsampledColor's search (taa.comp:236-257) on one axis of output texel tc: the two
+// candidates at the corner offsets 0 and c, the second taken when it lands in the output texel,
+// else the first. The reference's four corners are the products of the two axes' candidates and it
+// keeps the last that lands on both (order (0,0), (c,0), (0,c), (c,c)): that is this pick on each
+// axis whenever each axis has one. Returns the input texel it fetches (floor, clamped to the edge),
+// or -1 when no candidate lands -- or, on the x axis (`x`), when the pick is negative, since the
+// reference tests only foundTc.x >= 0.
+__device__ __forceinline__ int upsample_pick(float l, float h, float tj, int tc, int n, bool x) {
+    bool h0, h1;
+    const float s0 = upsample_axis(l, h, tj, tc, 0.0f, h0);
+    const float s1 = upsample_axis(l, h, tj, tc, kUpsampleCorner, h1);
+    const float s = h1 ? s1 : s0;
+    if (!(h0 || h1) || (x && !(s >= 0.0f))) return -1;
+    return clampi(to_int(floorf(s)), 0, n - 1);
 }
 
 // RN(h * a) for the fp16 value h in the low (HI false) or high half of `bits`: one mixed FMA whose

@@ -66,8 +66,7 @@ struct Geo {
         // samples at uv + (d - 1) input texels, d = 0..2, and the current-sample search at the
         // corner offsets 0 and c (position, hit)
         BilinearAxis uax[UP ? 3 * RW : 1], uay[UP ? 3 * RH : 1];
-        float usx[UP ? 2 * RW : 1], usy[UP ? 2 * RH : 1];
-        int uhx[UP ? 2 * RW : 1], uhy[UP ? 2 * RH : 1];
+        int ucx[UP ? RW : 1], ucy[UP ? RH : 1];
     };
 };
 
@@ -84,10 +83,8 @@ struct UpIn {
     // the per-column and per-row tables (column j of the resolve region at [d * rw + j])
     const BilinearAxis* ax;
     const BilinearAxis* ay;
-    const float* sx;
-    const float* sy;
-    const int* hx;
-    const int* hy;
+    const int* cx;  // getCurrentUpsampledColor's input column per output column, -1: none found
+    const int* cy;  // ... input row per output row, -1: none
     int rw, rh;
 };
 
@@ -465,10 +462,14 @@ __device__ __forceinline__ PixelResult resolve_pixel_f32(const FrameDesc& fd, co
     bool upsampled = false;
     if constexpr (UP) {
         if (U.mUpsampling) {
-            const float sx[2] = {up->sx[rx], up->sx[up->rw + rx]}, sy[2] = {up->sy[ry], up->sy[up->rh + ry]};
-            const bool hx[2] = {up->hx[rx] != 0, up->hx[up->rw + rx] != 0};
-            const bool hy[2] = {up->hy[ry] != 0, up->hy[up->rh + ry] != 0};
-            currentColor = current_upsampled(sx, hx, sy, hy, Wl, Hl, beta, color_at);
+            const int ix = up->cx[rx], iy = up->cy[ry];
+            if (ix >= 0 && iy >= 0) {
+                beta = 1.0f;
+                currentColor = color_at(ix, iy);
+            } else {
+                beta = 0.0f;
+                currentColor = f3{0.0f, 0.0f, 0.0f};
+            }
             upsampled = true;
         }
     }
@@ -830,7 +831,7 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
     // the output texel's footprint (+-1); the rectangle keeps 3. An input larger than the output
     // stages nothing and every fetch reads global memory.
     UpIn ui{sm.uc0, sm.uc1, sm.uc2, sm.ud, sm.uvel, 0, 0, 0, 0, G::UCW,
-            sm.uax, sm.uay, sm.usx, sm.usy, sm.uhx, sm.uhy, G::RW, G::RH};
+            sm.uax, sm.uay, sm.ucx, sm.ucy, G::RW, G::RH};
     if constexpr (G::UP) {
         const int Wl = fd.Wl, Hl = fd.Hl, pl = fd.pitch_l;
         // the staged rectangle: the input texels under the resolve region, +3 on every side
@@ -850,19 +851,19 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
             ui.h = h;
         }
         // per-column and per-row tables of the resolve region (clamped texel, uv as tabulated
-        // below): the bilinear axes of SAMPLE_TEX at uv + (d - 1) / input size, and one axis of
-        // getCurrentUpsampledColor's search at both corner offsets. A staged tile keeps the bilinear
-        // taps as positions in the staged rectangle (x - x0, (y - y0) * UCW), which hold the same
-        // texels.
+        // below): the bilinear axes of SAMPLE_TEX at uv + (d - 1) / input size, and the input
+        // column (row) getCurrentUpsampledColor fetches from (upsample_pick). A staged tile keeps the
+        // bilinear taps as positions in the staged rectangle (x - x0, (y - y0) * UCW), which hold
+        // the same texels.
         {
             const float toUx = 1.0f / (float)Wl, toUy = 1.0f / (float)Hl;
             const float jux = U.mJitterNdc[0] * 0.5f * P.mUnjitterFactor, juy = U.mJitterNdc[1] * 0.5f * P.mUnjitterFactor;
             const float lx = (float)Wl, ly = (float)Hl, hxs = (float)W, hys = (float)H;
             const float tjx = jux * lx * -1.0f, tjy = juy * ly * -1.0f;
-            for (int i = tid; i < 5 * (G::RW + G::RH); i += G::NT) {
-                const bool col = i < 5 * G::RW;
-                const int n = col ? G::RW : G::RH, e = col ? i : i - 5 * G::RW;
-                const int q = e / n, j = e - q * n;  // q = 0..2: bilinear offset, 3..4: search corner
+            for (int i = tid; i < 4 * (G::RW + G::RH); i += G::NT) {
+                const bool col = i < 4 * G::RW;
+                const int n = col ? G::RW : G::RH, e = col ? i : i - 4 * G::RW;
+                const int q = e / n, j = e - q * n;  // q = 0..2: bilinear offset, 3: the sample search
                 const int c = col ? clampi(x0 - G::RA + j, 0, W - 1) : clampi(y0 - G::RA + j, 0, H - 1);
                 const float uv = col ? ((float)c + 0.5f) / (float)W : ((float)c + 0.5f) / (float)H;
                 if (q < 3) {
@@ -875,11 +876,8 @@ __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_u
                     }
                     (col ? sm.uax : sm.uay)[q * n + j] = ba;
                 } else {
-                    bool hit;
-                    const float o = q == 3 ? 0.0f : kUpsampleCorner;
-                    const float sp = col ? upsample_axis(lx, hxs, tjx, c, o, hit) : upsample_axis(ly, hys, tjy, c, o, hit);
-                    (col ? sm.usx : sm.usy)[(q - 3) * n + j] = sp;
-                    (col ? sm.uhx : sm.uhy)[(q - 3) * n + j] = hit ? 1 : 0;
+                    (col ? sm.ucx : sm.ucy)[j] = col ? upsample_pick(lx, hxs, tjx, c, Wl, true)
+                                                     : upsample_pick(ly, hys, tjy, c, Hl, false);
                 }
             }
         }
