@@ -1005,9 +1005,11 @@ __global__ __launch_bounds__(256) void k_resolve_up_seg(const ResolveArgs a, con
 // Upsampling path (input size != output size, or mUpsampling): exact FP32 arithmetic, or with FA
 // the FP16 path's (TAA_ARITH_FP16); the input texels under the tile are staged in LDS (converted
 // colour, depth, velocity), the rest of the resolve as above, fused CAS from LDS.
-// PROD: the production feature set with its switches fixed (prod_fix)
+// PROD: the production feature set with its switches fixed (prod_fix). Occupancy hint: the FP16
+// arithmetic's general form fits five waves per SIMD in registers; the production form's 30x14
+// tiles hold four workgroups per CU in LDS, so four
 template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA, bool PROD>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FA && S16 ? 5 : 1))) void k_resolve_up(const ResolveArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FA && S16 ? (PROD ? 4 : 5) : 1))) void k_resolve_up(const ResolveArgs a) {
     using G = Geo<TW, TH, PAD, CAS, true>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
