@@ -126,14 +126,16 @@ def test_split_screen(oracle):
 def test_split_screen_with_cas(oracle, fmt, arith, split_x):
     """splitScreen with CAS in one call: both parts resolved, then one CAS pass over the whole
     frame (taa.hpp:1008-1027, 1111-1139), against the oracle's resolve + CasFilter of the same
-    frame; splitX -1 puts every column on param[1], the last column every column on param[0]."""
+    frame; splitX -1 puts every column on param[1], the last column every column on param[0]. The
+    same frames without the inspect image (param[0]'s columns then take the production form of the
+    exact kernel) write the same bits."""
     import ctypes as C
     import taastar
     from gpu_helpers import FP16_PSNR_MIN_DB, INDEX_WORDS, psnr
     from taastar.workloads import uniforms_for
     print(f"case split+cas fmt={fmt} arith={arith} splitX={split_x}", flush=True)
     w = workload("c2", fmt=fmt, vel=VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F)
-    seq = Sequence(w, width=203, height=81, cas=True, arith=arith, debug=arith == 0)
+    seq = Sequence(w, width=203, height=81, cas=True, arith=arith, debug=False, shipped=True)
     for f in range(3):
         col, dep, vel4, jit = seq.scene.render(seq.frame)
         U = uniforms_for(w, seq.frame, jit)
