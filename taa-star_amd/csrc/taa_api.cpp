@@ -841,22 +841,30 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
     for (int i = 0; i < L.n_main && L.prod_main; ++i) L.prod_main = taastar::exact_prod_form(ents[i].fd, ents[i].U);
     L.prod_up = L.n_up > 0;
     for (int i = L.n_main; i < L.n_main + L.n_up && L.prod_up; ++i) L.prod_up = taastar::prod_params(ents[i].fd, ents[i].U);
+    if (L.prod_up && L.cas) {  // the production form's own CAS tile, as in taa_resolve_frame
+        const taastar::TileShapeDef sp = shape_def(taastar::kUpShapeCasProd, true);
+        L.total_up = 0;
+        for (int i = L.n_main; i < L.n_main + L.n_up; ++i) {
+            taastar::BatchEntry& e = ents[i];
+            e.fd.tiles_x = taastar::tiles_for(e.fd.W, sp.tw);
+            e.fd.tiles_y = taastar::tiles_for(e.fd.H, sp.th);
+            e.tile_begin = L.total_up;
+            L.total_up += e.fd.tiles_x * e.fd.tiles_y;
+        }
+    }
     return TAA_OK;
 }
 
 // The launches of one batched call on `stream` (descriptors already on the device at d_ents).
 static int launch_layout(const taastar::BatchEntry* d_ents, const BatchLayout& L, int arith, void* stream) {
     if (L.n_main) {
-        const int grid = taastar::batch_grid(L.total_main);
-        int le = L.fast ? taastar::launch_batch_fast(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, L.form, grid,
-                                                     stream)
+        int le = L.fast ? taastar::launch_batch_fast(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, L.form, stream)
                         : taastar::launch_batch(d_ents, L.n_main, L.total_main, L.shape, L.cas, L.s16, arith, L.prod_main,
-                                                grid, stream);
+                                                stream);
         if (le) return le;
     }
     if (L.n_up)
-        return taastar::launch_batch_up(d_ents + L.n_main, L.n_up, L.total_up, L.cas, L.s16, arith, L.prod_up,
-                                        taastar::batch_grid(L.total_up), stream);
+        return taastar::launch_batch_up(d_ents + L.n_main, L.n_up, L.total_up, L.cas, L.s16, arith, L.prod_up, stream);
     return 0;
 }
 

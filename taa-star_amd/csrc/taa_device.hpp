@@ -484,5 +484,14 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return xcd * q + min(xcd, r) + slot;
 }
 
+// The tile of a batched launch's workgroup: the grid is batch_grid(total_tiles) = 8 * per
+// workgroups, one tile each, and XCD k (blockIdx % 8) takes the k-th contiguous eighth of the
+// flattened tile space. -1: a workgroup of the rounding, with no tile.
+__device__ __forceinline__ int batch_tile(int total_tiles) {
+    const int per = (total_tiles + 7) >> 3;
+    const int slot = (int)(blockIdx.x >> 3), t = (int)(blockIdx.x & 7) * per + slot;
+    return slot < per && t < total_tiles ? t : -1;
+}
+
 }  // namespace
 }  // namespace taastar

@@ -493,15 +493,10 @@ template <int TW, int TH, bool CAS, bool S16, int FORM>
 __global__ __launch_bounds__(256) void k_resolve_fast_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = FGeo<TW, TH, CAS, S16, FORM == 3>;
     __shared__ typename G::Smem sm;
-    const int per = (total_tiles + 7) >> 3;
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
-    for (int j = slot; j < per; j += slots) {
-        const int t = xcd * per + j;
-        if (t >= total_tiles) break;
-        const BatchEntry& e = batch_entry(entries, n_entries, t);
-        fast_tile<G, false, FORM != 0, FORM >= 2>(e.fd, e.U, t - e.tile_begin, sm);
-        __syncthreads();
-    }
+    const int t = batch_tile(total_tiles);
+    if (t < 0) return;
+    const BatchEntry& e = batch_entry(entries, n_entries, t);
+    fast_tile<G, false, FORM != 0, FORM >= 2>(e.fd, e.U, t - e.tile_begin, sm);
 }
 
 // FORM: 0 the production form (variance box, Catmull-Rom, RG velocity, no branch on either),
@@ -524,7 +519,8 @@ hipError_t launch_fast_t(const FrameDesc& fd, const taa_uniforms_t& U, int shape
 }
 
 template <bool CAS, bool S16, int FORM>
-hipError_t launch_fast_batch_t(const BatchEntry* e, int n, int total, int shape, int grid, hipStream_t st) {
+hipError_t launch_fast_batch_t(const BatchEntry* e, int n, int total, int shape, hipStream_t st) {
+    const int grid = batch_grid(total);
     constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
     switch (shape) {
         case 0: hipLaunchKernelGGL((k_resolve_fast_batch<S[0].tw, S[0].th, CAS, S16, FORM>), dim3(grid), dim3(256), 0, st, e, n, total); break;
@@ -596,23 +592,23 @@ int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape,
 }
 
 template <bool CAS, bool S16>
-hipError_t launch_fast_batch_f(const BatchEntry* e, int n, int total, int shape, int form, int grid, hipStream_t st) {
+hipError_t launch_fast_batch_f(const BatchEntry* e, int n, int total, int shape, int form, hipStream_t st) {
     switch (form) {
-        case 0: return launch_fast_batch_t<CAS, S16, 0>(e, n, total, shape, grid, st);
-        case 1: return launch_fast_batch_t<CAS, S16, 1>(e, n, total, shape, grid, st);
-        case 2: return launch_fast_batch_t<CAS, S16, 2>(e, n, total, shape, grid, st);
-        default: return launch_fast_batch_t<CAS, S16, 3>(e, n, total, shape, grid, st);
+        case 0: return launch_fast_batch_t<CAS, S16, 0>(e, n, total, shape, st);
+        case 1: return launch_fast_batch_t<CAS, S16, 1>(e, n, total, shape, st);
+        case 2: return launch_fast_batch_t<CAS, S16, 2>(e, n, total, shape, st);
+        default: return launch_fast_batch_t<CAS, S16, 3>(e, n, total, shape, st);
     }
 }
 
 int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, int form,
-                      int grid, void* stream) {
+                      void* stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e;
-    if (cas) e = s16 ? launch_fast_batch_f<true, true>(d_entries, n_entries, total_tiles, shape, form, grid, st)
-                     : launch_fast_batch_f<true, false>(d_entries, n_entries, total_tiles, shape, form, grid, st);
-    else e = s16 ? launch_fast_batch_f<false, true>(d_entries, n_entries, total_tiles, shape, form, grid, st)
-                 : launch_fast_batch_f<false, false>(d_entries, n_entries, total_tiles, shape, form, grid, st);
+    if (cas) e = s16 ? launch_fast_batch_f<true, true>(d_entries, n_entries, total_tiles, shape, form, st)
+                     : launch_fast_batch_f<true, false>(d_entries, n_entries, total_tiles, shape, form, st);
+    else e = s16 ? launch_fast_batch_f<false, true>(d_entries, n_entries, total_tiles, shape, form, st)
+                 : launch_fast_batch_f<false, false>(d_entries, n_entries, total_tiles, shape, form, st);
     return (int)e;
 }
 

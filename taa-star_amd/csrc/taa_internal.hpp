@@ -96,19 +96,22 @@ int launch_resolve_up(const FrameDesc& fd, const taa_uniforms_t& U, int arith, v
 // inputs from global memory. Tiled by its own shape, whatever fd.tiles_x / tiles_y say
 int launch_resolve_seg(const FrameDesc& fd, const taa_uniforms_t& U, const SegDesc& sg, int arith, void* stream);
 int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uint32_t peak_bits, void* stream);
-// Workgroups of a batched launch: one per tile, rounded to whole XCD slots. The batch kernels walk
-// the tile space with a grid stride, so a smaller (persistent) grid is also correct; measured on
-// the 64-session mix, one workgroup per tile with the hardware dispatcher balancing the XCDs is
-// faster (55.7 vs 48.1 Gpix/s) than an occupancy-sized persistent grid with static strides.
+// Workgroups of a batched launch: one per tile, rounded to whole XCD slots; the batch kernels take
+// exactly this grid and resolve one tile per workgroup (batch_tile). Measured on the 64-session
+// mix, one workgroup per tile with the hardware dispatcher balancing the XCDs is faster (55.7 vs
+// 48.1 Gpix/s) than an occupancy-sized persistent grid with static strides; and a kernel body
+// without a loop over tiles keeps the single-frame kernel's register budget (the loop cost the
+// FP16 batch kernel 12 VGPRs and a wave per SIMD: 62.6 against 70.9 Gpix/s on the 48-stream mix).
 inline int batch_grid(int total_tiles) { return (total_tiles + 7) & ~7; }
 
 // prod: every frame has the production feature set (exact_prod_form): the production form
 int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas,
-                 bool s16, int arith, bool prod, int grid, void* stream);
+                 bool s16, int arith, bool prod, void* stream);
 // the upsampling frames of a batch (the upsampling resolve in the batch's arithmetic, kUpShapeCas /
-// kUpShapePlain tiles; prod: every frame has the production feature set, prod_params)
+// kUpShapePlain tiles, kUpShapeCasProd with prod: every frame has the production feature set,
+// prod_params)
 int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int arith, bool prod,
-                    int grid, void* stream);
+                    void* stream);
 
 // Throughput kernel (taa_fast.hip): variance clip + Catmull-Rom + simple/closest-depth velocity
 // with RG velocity, fp16-fed mixed arithmetic. fast_path_supported() says whether a parameter
@@ -123,7 +126,7 @@ bool fast_path_supported(const FrameDesc& fd, const taa_uniforms_t& U, int param
 int fast_path_form(const FrameDesc& fd, const taa_uniforms_t& U, int param_idx);
 int launch_resolve_fast(const FrameDesc& fd, const taa_uniforms_t& U, int shape, void* stream);
 int launch_batch_fast(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16,
-                      int form, int grid, void* stream);
+                      int form, void* stream);
 
 // Edge-AA fallback on category-1 texels of a segmentation mask (taa_edge_aa.hip).
 int launch_edge_aa(const void* src, const uint32_t* mask, void* dst, int W, int H, int pitch, int seg_pitch, int fmt,

@@ -1016,42 +1016,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FA && S16 ?
     resolve_tile<G, S16, false, FA, PROD>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm);
 }
 
-// Batched resolve: the grid walks the flattened tile space of every frame in the batch with a
-// grid stride (one tile per workgroup at the launch size batch_grid() picks). XCD k takes the
-// k-th contiguous eighth of the tiles.
+// Batched resolve: one workgroup per tile of the flattened tile space of every frame in the batch
+// (batch_tile: XCD k takes the k-th contiguous eighth of the tiles).
 // PROD: every frame of the batch has the production feature set (exact_prod_form); its RG16F
 // velocity is then staged like k_resolve's.
 template <int TW, int TH, int PAD, bool CAS, bool S16, bool PROD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3))) void k_resolve_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = Geo<TW, TH, PAD, CAS, false, false, PROD && S16>;
     __shared__ typename G::Smem sm;
-    const int per = (total_tiles + 7) >> 3;
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
-    for (int j = slot; j < per; j += slots) {
-        const int t = xcd * per + j;
-        if (t >= total_tiles) break;
-        const BatchEntry& e = batch_entry(entries, n_entries, t);
-        resolve_tile<G, S16, false, false, PROD>(e.fd, e.U, t - e.tile_begin, sm);
-        __syncthreads();  // LDS is reused by the next tile
-    }
+    const int t = batch_tile(total_tiles);
+    if (t < 0) return;
+    const BatchEntry& e = batch_entry(entries, n_entries, t);
+    resolve_tile<G, S16, false, false, PROD>(e.fd, e.U, t - e.tile_begin, sm);
 }
 
 // Batched upsampling resolve: the frames of a batch whose input size differs from the output size
-// (or that set mUpsampling), walked like k_resolve_batch, each tile as in k_resolve_up.
+// (or that set mUpsampling), walked like k_resolve_batch, each tile as in k_resolve_up (its tile
+// shapes and occupancy hint).
 // PROD: every frame of the batch has the production feature set (prod_params)
 template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA, bool PROD>
-__global__ __launch_bounds__(256) void k_resolve_up_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FA && S16 ? (PROD ? 4 : 5) : 1))) void k_resolve_up_batch(const BatchEntry* __restrict__ entries, int n_entries, int total_tiles) {
     using G = Geo<TW, TH, PAD, CAS, true>;
     __shared__ typename G::Smem sm;
-    const int per = (total_tiles + 7) >> 3;
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
-    for (int j = slot; j < per; j += slots) {
-        const int t = xcd * per + j;
-        if (t >= total_tiles) break;
-        const BatchEntry& e = batch_entry(entries, n_entries, t);
-        resolve_tile<G, S16, false, FA, PROD>(e.fd, e.U, t - e.tile_begin, sm);
-        __syncthreads();  // LDS is reused by the next tile
-    }
+    const int t = batch_tile(total_tiles);
+    if (t < 0) return;
+    const BatchEntry& e = batch_entry(entries, n_entries, t);
+    resolve_tile<G, S16, false, FA, PROD>(e.fd, e.U, t - e.tile_begin, sm);
 }
 
 // sharpen_cas.comp on its own (one thread per pixel, clamped loads)
@@ -1086,7 +1076,8 @@ hipError_t launch_resolve_t(const FrameDesc& fd, const taa_uniforms_t& U, int sh
 }
 
 template <bool CAS, bool S16, bool PROD>
-hipError_t launch_batch_t(const BatchEntry* e, int n, int total, int shape, int grid, hipStream_t st) {
+hipError_t launch_batch_t(const BatchEntry* e, int n, int total, int shape, hipStream_t st) {
+    const int grid = batch_grid(total);
     constexpr const TileShapeDef* S = CAS ? kShapesCas : kShapesPlain;
     switch (shape) {
         case 0: hipLaunchKernelGGL((k_resolve_batch<S[0].tw, S[0].th, S[0].pad, CAS, S16, PROD>), dim3(grid), dim3(256), 0, st, e, n, total); break;
@@ -1210,9 +1201,10 @@ int launch_cas(const void* src, void* dst, int W, int H, int pitch, int fmt, uin
 }
 
 template <bool FA, bool PROD>
-hipError_t launch_batch_up_t(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int grid,
+hipError_t launch_batch_up_t(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16,
                              hipStream_t st) {
-    constexpr TileShapeDef C = kShapesCas[kUpShapeCas], Pl = kShapesPlain[kUpShapePlain];
+    constexpr TileShapeDef C = kShapesCas[PROD ? kUpShapeCasProd : kUpShapeCas], Pl = kShapesPlain[kUpShapePlain];
+    const int grid = batch_grid(total_tiles);
     if (cas) {
         if (s16) hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, true, FA, PROD>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
         else hipLaunchKernelGGL((k_resolve_up_batch<C.tw, C.th, C.pad, true, false, FA, PROD>), dim3(grid), dim3(256), 0, st, d_entries, n_entries, total_tiles);
@@ -1224,33 +1216,33 @@ hipError_t launch_batch_up_t(const BatchEntry* d_entries, int n_entries, int tot
 }
 
 int launch_batch_up(const BatchEntry* d_entries, int n_entries, int total_tiles, bool cas, bool s16, int arith, bool prod,
-                    int grid, void* stream) {
+                    void* stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e;
     if (arith == TAA_ARITH_FP16)
-        e = prod ? launch_batch_up_t<true, true>(d_entries, n_entries, total_tiles, cas, s16, grid, st)
-                 : launch_batch_up_t<true, false>(d_entries, n_entries, total_tiles, cas, s16, grid, st);
+        e = prod ? launch_batch_up_t<true, true>(d_entries, n_entries, total_tiles, cas, s16, st)
+                 : launch_batch_up_t<true, false>(d_entries, n_entries, total_tiles, cas, s16, st);
     else
-        e = prod ? launch_batch_up_t<false, true>(d_entries, n_entries, total_tiles, cas, s16, grid, st)
-                 : launch_batch_up_t<false, false>(d_entries, n_entries, total_tiles, cas, s16, grid, st);
+        e = prod ? launch_batch_up_t<false, true>(d_entries, n_entries, total_tiles, cas, s16, st)
+                 : launch_batch_up_t<false, false>(d_entries, n_entries, total_tiles, cas, s16, st);
     return (int)e;
 }
 
 int launch_batch(const BatchEntry* d_entries, int n_entries, int total_tiles, int shape, bool cas, bool s16, int arith,
-                 bool prod, int grid, void* stream) {
+                 bool prod, void* stream) {
     (void)arith;
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e;
     if (prod) {
-        if (cas) e = s16 ? launch_batch_t<true, true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
-                         : launch_batch_t<true, false, true>(d_entries, n_entries, total_tiles, shape, grid, st);
-        else e = s16 ? launch_batch_t<false, true, true>(d_entries, n_entries, total_tiles, shape, grid, st)
-                     : launch_batch_t<false, false, true>(d_entries, n_entries, total_tiles, shape, grid, st);
+        if (cas) e = s16 ? launch_batch_t<true, true, true>(d_entries, n_entries, total_tiles, shape, st)
+                         : launch_batch_t<true, false, true>(d_entries, n_entries, total_tiles, shape, st);
+        else e = s16 ? launch_batch_t<false, true, true>(d_entries, n_entries, total_tiles, shape, st)
+                     : launch_batch_t<false, false, true>(d_entries, n_entries, total_tiles, shape, st);
     } else {
-        if (cas) e = s16 ? launch_batch_t<true, true, false>(d_entries, n_entries, total_tiles, shape, grid, st)
-                         : launch_batch_t<true, false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
-        else e = s16 ? launch_batch_t<false, true, false>(d_entries, n_entries, total_tiles, shape, grid, st)
-                     : launch_batch_t<false, false, false>(d_entries, n_entries, total_tiles, shape, grid, st);
+        if (cas) e = s16 ? launch_batch_t<true, true, false>(d_entries, n_entries, total_tiles, shape, st)
+                         : launch_batch_t<true, false, false>(d_entries, n_entries, total_tiles, shape, st);
+        else e = s16 ? launch_batch_t<false, true, false>(d_entries, n_entries, total_tiles, shape, st)
+                     : launch_batch_t<false, false, false>(d_entries, n_entries, total_tiles, shape, st);
     }
     return (int)e;
 }
