@@ -677,6 +677,7 @@ def main():
                 tune(*RESOLUTIONS[cls], fmt, ar, True)
             r = simulate(4, list(MIX48), 48, fmt, ar, a.sim_seconds)
             print(json.dumps(dict(load="batched48", storage_and_arith=name, mix=MIX48_TEXT, **r)), flush=True)
+        copy_calibration(a.copy_calib)
         return
     r = time_resolve(a.workload, a.arith, a.steps, a.warmup, a.frames, autotune=a.autotune)
     w = r["w"]
