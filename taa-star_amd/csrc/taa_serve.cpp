@@ -2,7 +2,7 @@
 //
 // One host thread drives the GPU: taa_server_pump() harvests finished batches (event queries) and
 // launches every queued frame whose session is idle into a free in-flight slot (its own stream),
-// as one persistent batched resolve. Completion times are GPU timestamps (event elapsed time from
+// as one batched resolve (one workgroup per tile of all its frames). Completion times are GPU timestamps (event elapsed time from
 // a reference event whose host time is known), so latency does not include host polling delays.
 // Closed sessions go to a per-resolution recycle list instead of hipFree, so churn never stalls the
 // device; their bytes stay counted against the budget until reused or released.
@@ -749,7 +749,7 @@ extern "C" int32_t taa_serve_simulate(const taa_serve_sim_config_t* cfg, taa_ser
         out->gpu_busy_frac = union_length(srv->spans, t_begin, t_end) / (t_end - t_begin);
         out->hbm_peak_bytes = st1.hbm_used;
     } else if (rc == 0 && cfg->mode == 4) {
-        // every session advances one frame per graph launch (descriptor upload + persistent kernel)
+        // every session advances one frame per graph launch (descriptor upload + batched kernel)
         hipStream_t st;
         hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
         std::vector<taa_batch_item_t> items(ss.size());
