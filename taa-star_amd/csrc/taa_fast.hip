@@ -481,8 +481,11 @@ struct FastArgs {
     taa_uniforms_t U;
 };
 
+// Occupancy hint: five waves per SIMD for the extended forms (the RGBA-velocity form on FP16
+// storage then holds 96 VGPRs and 24 bytes of spills instead of 101 VGPRs at four waves: 4K with
+// the reference's defaults 56.7 -> 57.7 Gpix/s); the production form reaches 92 VGPRs unhinted.
 template <int TW, int TH, bool CAS, bool S16, bool INSP, int FORM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FORM == 2 ? 5 : 1))) void k_resolve_fast(const FastArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FORM == 2 || (FORM == 3 && S16) ? 5 : 1))) void k_resolve_fast(const FastArgs a) {
     using G = FGeo<TW, TH, CAS, S16, FORM == 3>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
