@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "taa_footprint.hpp"
+#include "taa_tiles.hpp"
 #include "taa_internal.hpp"
 
 namespace taastar {
@@ -478,20 +479,12 @@ __device__ __forceinline__ const BatchEntry& batch_entry(const BatchEntry* entri
     return *(const BatchEntry*)(ents + lo);
 }
 
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-    const int q = n >> 3, r = n & 7;
-    const int xcd = b & 7, slot = b >> 3;
-    return xcd * q + min(xcd, r) + slot;
-}
+__device__ __forceinline__ int xcd_tile(int b, int n) { return xcd_tile_of(b, n); }
 
 // The tile of a batched launch's workgroup: the grid is batch_grid(total_tiles) = 8 * per
 // workgroups, one tile each, and XCD k (blockIdx % 8) takes the k-th contiguous eighth of the
 // flattened tile space. -1: a workgroup of the rounding, with no tile.
-__device__ __forceinline__ int batch_tile(int total_tiles) {
-    const int per = (total_tiles + 7) >> 3;
-    const int slot = (int)(blockIdx.x >> 3), t = (int)(blockIdx.x & 7) * per + slot;
-    return slot < per && t < total_tiles ? t : -1;
-}
+__device__ __forceinline__ int batch_tile(int total_tiles) { return batch_tile_of((int)blockIdx.x, total_tiles); }
 
 }  // namespace
 }  // namespace taastar
