@@ -11,6 +11,7 @@ Prints one JSON line (rank 0). `--gpus N` with N > 1 exits with a message: this 
 one GPU; sessions shard across GPUs without any exchange.
 """
 import argparse
+import contextlib
 import ctypes as C
 import json
 import os
@@ -724,7 +725,8 @@ def main():
         out.update(serving_summary(out["extras"], r["gpix_per_s"]))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, r["host"][:2], a.cpu_seconds)
-        out["accuracy"] = accuracy_vs_oracle(w, r["host"], a.arith)
+        with contextlib.redirect_stdout(sys.stderr):  # the per-frame report lines; stdout holds the one JSON line
+            out["accuracy"] = accuracy_vs_oracle(w, r["host"], a.arith)
     print(json.dumps(out), flush=True)
 
 
