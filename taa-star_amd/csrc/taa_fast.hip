@@ -120,7 +120,10 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
     float* const sux = sm.ux();
     float* const suy = sm.uy();
 
-    // 1. stage the region: YCoCg (fp32), depth, velocity; uv of the staged columns and rows
+    // 1. stage the region: YCoCg (fp32), depth, velocity; uv of the staged columns and rows. The
+    // staging loads issue at the highest wave priority, ahead of other workgroups' arithmetic
+    // (4K: +0.5% in three same-box triples; priority 2 here: +0.1%)
+    __builtin_amdgcn_s_setprio(3);
     {
         constexpr int DY = G::NT / G::SW, DX = G::NT % G::SW;
         int sy = tid / G::SW, sx = tid - sy * G::SW;
@@ -143,6 +146,7 @@ __device__ void fast_tile(const FrameDesc& fd, const taa_uniforms_t& U, int tile
             if (sx >= G::SW) { sx -= G::SW; sy += 1; }
         }
     }
+    __builtin_amdgcn_s_setprio(0);
     if (tid < G::SW) sux[tid] = ((float)clampi(gx0 + tid, 0, W - 1) + 0.5f) / fW;              // taa.comp:131
     else if (tid < G::SW + G::SH) suy[tid - G::SW] = ((float)clampi(gy0 + tid - G::SW, 0, H - 1) + 0.5f) / fH;
     __syncthreads();
