@@ -26,8 +26,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each divided by what the same
 # counter reads for a device-to-device copy of known size), keyed by (workload, arith)
 TRAFFIC = {
-    # profiles/k_resolve_fast_3840x2160_fp16_cas.md: (126,118.5 KiB x 1.99977 + 129,778.5 KiB x 1.0) x 1024
-    ("c3", "fp16"): 391152543,
+    # profiles/k_resolve_fast_3840x2160_fp16_cas.md: (125,827.1 KiB x 1.99976 + 129,777.5 KiB x 1.0) x 1024
+    ("c3", "fp16"): 390555001,
     # profiles/k_resolve_3840x2160_rgba16f_exact_fp32.md: (126,024.0 KiB x 1.99982 + 129,668.2 KiB x 1.0) x 1024
     ("c3", "fp32"): 390854160,
 }
