@@ -453,7 +453,9 @@ int32_t taa_batch_submit(taa_ctx_t* ctx, int32_t n, const taa_batch_item_t* item
 
 /* A batch plan captures descriptor upload + the batched kernel in a hipGraph for a fixed set
  * of sessions; each launch re-reads the items' uniforms and advances every session by one frame.
- * Tile shape per plan comes from the context's autotuner table. */
+ * Tile shape per plan comes from the context's autotuner table: a single one-size frame takes its
+ * class's shape, several take the shape tuned for the largest class of at most their total
+ * pixel count (a launch over many frames behaves like one frame of their total size). */
 typedef struct taa_batch_plan taa_batch_plan_t;
 int32_t taa_batch_plan_create(taa_ctx_t* ctx, int32_t n, const taa_batch_item_t* items,
                               int32_t arith, taa_batch_plan_t** out);
@@ -475,7 +477,8 @@ int32_t taa_tile_shape_get(int32_t index, taa_bool32 cas, taa_tile_shape_t* out)
 
 /* Time every built tile shape for this frame class (width, height, colour format, arith, CAS)
  * on the context's device with a synthetic frame, and record the fastest in the process-wide
- * table that taa_resolve_frame, taa_batch_submit and batch plans consult. */
+ * table that taa_resolve_frame, taa_batch_submit and batch plans consult (a batch of several
+ * frames uses the entry of the largest class of at most its total pixel count). */
 int32_t taa_autotune(taa_ctx_t* ctx, int32_t width, int32_t height, int32_t color_format,
                      int32_t arith, taa_bool32 cas, int32_t iters, int32_t* best_index,
                      float* best_us);

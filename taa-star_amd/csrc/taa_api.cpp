@@ -80,6 +80,26 @@ int selected_shape(int W, int H, int fmt, int arith, bool cas) {
     return cas ? taastar::kDefaultShapeCas : taastar::kDefaultShapePlain;
 }
 
+// The shape for a launch over the tiles of several frames: such a launch behaves like one frame of
+// their total size, so it takes the shape tuned for the largest class of at most that many pixels
+// (the default when none is tuned). Measured with every class tuned on single frames: 64 sessions
+// of 640x360 took 14x14 tiles from their own class, 48.2 Gpix/s batched, against 61.9 with 30x30.
+int selected_shape_for_pixels(long long pixels, int fmt, int arith, bool cas) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    long long best_px = -1;
+    int shape = cas ? taastar::kDefaultShapeCas : taastar::kDefaultShapePlain;
+    for (const auto& e : g_tune) {
+        const ClassKey& k = e.first;
+        if (std::get<2>(k) != fmt || std::get<3>(k) != arith || std::get<4>(k) != (cas ? 1 : 0)) continue;
+        const long long px = (long long)std::get<0>(k) * std::get<1>(k);
+        if (px <= pixels && px > best_px) {
+            best_px = px;
+            shape = e.second;
+        }
+    }
+    return shape;
+}
+
 taastar::TileShapeDef shape_def(int idx, bool cas) { return cas ? taastar::kShapesCas[idx] : taastar::kShapesPlain[idx]; }
 
 // Fill the kernel-side frame descriptor; validates the images.
@@ -791,8 +811,9 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
     L = BatchLayout{};
     L.cas = items[0].cas != nullptr;
     L.s16 = items[0].session->cf == TAA_FMT_RGBA16F;
-    // one tile shape per main launch: the one tuned for the largest one-size frame class
-    int best_px = -1;
+    // one tile shape per main launch: a single frame's tuned shape; for several frames, the shape
+    // of one frame of their total size (selected_shape_for_pixels)
+    long long total_px = 0;
     for (int i = 0; i < n; ++i) {
         const taa_batch_item_t& it = items[i];
         if (int rc = taastar::check_batch_uniforms(it.uniforms)) return rc;
@@ -801,12 +822,10 @@ static int build_batch(taa_ctx_t* c, int n, const taa_batch_item_t* items, int a
             continue;
         }
         ++L.n_main;
-        const int px = it.session->W * it.session->H;
-        if (px > best_px) {
-            best_px = px;
-            L.shape = selected_shape(it.session->W, it.session->H, it.session->cf, arith, L.cas);
-        }
+        total_px += (long long)it.session->W * it.session->H;
+        if (L.n_main == 1) L.shape = selected_shape(it.session->W, it.session->H, it.session->cf, arith, L.cas);
     }
+    if (L.n_main > 1) L.shape = selected_shape_for_pixels(total_px, items[0].session->cf, arith, L.cas);
     const taastar::TileShapeDef sd = shape_def(L.shape, L.cas);
     const taastar::TileShapeDef su = shape_def(L.cas ? taastar::kUpShapeCas : taastar::kUpShapePlain, L.cas);
     int k_main = 0, k_up = L.n_main;
