@@ -455,7 +455,8 @@ int32_t taa_batch_submit(taa_ctx_t* ctx, int32_t n, const taa_batch_item_t* item
  * of sessions; each launch re-reads the items' uniforms and advances every session by one frame.
  * Tile shape per plan comes from the context's autotuner table: a single one-size frame takes its
  * class's shape, several take the shape tuned for the largest class of at most their total
- * pixel count (a launch over many frames behaves like one frame of their total size). */
+ * pixel count (a launch over many frames behaves like one frame of their total size), or the
+ * default shape when it resolves 2% fewer texels over the batch's frames. */
 typedef struct taa_batch_plan taa_batch_plan_t;
 int32_t taa_batch_plan_create(taa_ctx_t* ctx, int32_t n, const taa_batch_item_t* items,
                               int32_t arith, taa_batch_plan_t** out);
