@@ -483,7 +483,9 @@ int32_t taa_tile_shape_get(int32_t index, taa_bool32 cas, taa_tile_shape_t* out)
 int32_t taa_autotune(taa_ctx_t* ctx, int32_t width, int32_t height, int32_t color_format,
                      int32_t arith, taa_bool32 cas, int32_t iters, int32_t* best_index,
                      float* best_us);
-/* Force a tile shape for a frame class; index -1 restores the built-in default. */
+/* Force a tile shape for a frame class; index -1 restores the built-in default (with CAS: 30x30,
+ * or for frames of fewer 30x30 tiles than the device holds workgroups at once, 30x14 and below
+ * 512 such tiles 14x14). */
 int32_t taa_tile_shape_select(int32_t width, int32_t height, int32_t color_format, int32_t arith,
                               taa_bool32 cas, int32_t index);
 /* The shape index the table holds for a frame class. */
