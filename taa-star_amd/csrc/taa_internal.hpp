@@ -85,8 +85,9 @@ int launch_resolve(const FrameDesc& fd, const taa_uniforms_t& U, int shape, int 
 // with the other branches compiled out
 bool exact_prod_form(const FrameDesc& fd, const taa_uniforms_t& U);
 // the same switches, whatever the sizes: launch_resolve_up runs the upsampling kernel's production
-// form when they hold
-bool prod_params(const FrameDesc& fd, const taa_uniforms_t& U);
+// form when they hold; with seg, the same with mRayTraceAugment set (the masked resolve's
+// production form, whose mask keeps its own parameters)
+bool prod_params(const FrameDesc& fd, const taa_uniforms_t& U, bool seg = false);
 // the upsampling (TAAU) path: exact FP32 arithmetic, or with TAA_ARITH_FP16 the FP16 path's where no
 // selection depends on it; tiled by its form's shape (kUpShapeCas / kUpShapeCasProd / kUpShapePlain),
 // whatever fd.tiles_x / tiles_y say

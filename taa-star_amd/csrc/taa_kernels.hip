@@ -748,7 +748,7 @@ struct ProdFix {
     taa_uniforms_t U;
     taa_params_t P;
 };
-__device__ __forceinline__ void prod_fix(ProdFix& f, const FrameDesc& fd, const taa_uniforms_t& U, bool s16) {
+__device__ __forceinline__ void prod_fix(ProdFix& f, const FrameDesc& fd, const taa_uniforms_t& U, bool s16, bool seg) {
     f.fd = fd;
     f.fd.debug = nullptr;
     f.fd.inspect = nullptr;
@@ -769,7 +769,7 @@ __device__ __forceinline__ void prod_fix(ProdFix& f, const FrameDesc& fd, const 
     p.mDepthCulling = 1;
     p.mInterpolationMode = 2;
     p.mDynamicAntiGhosting = 0;
-    p.mRayTraceAugment = 0;
+    p.mRayTraceAugment = seg ? 1 : 0;  // the masked resolve keeps the mask's own parameters
     p.mColorClampingOrClipping = 2;
     p.mVelBasedAlpha = 0;
     p.mLumaWeightingLottes = 1;
@@ -783,7 +783,7 @@ template <class G, bool S16, bool SEG = G::SEG, bool FA = false, bool PROD = fal
 __device__ __forceinline__ void resolve_tile(const FrameDesc& fd_in, const taa_uniforms_t& U_in, int tile, typename G::Smem& sm,
                                              const SegDesc* sg = nullptr) {
     ProdFix pf;
-    if constexpr (PROD) prod_fix(pf, fd_in, U_in, S16);
+    if constexpr (PROD) prod_fix(pf, fd_in, U_in, S16, SEG);
     const FrameDesc& fd = PROD ? pf.fd : fd_in;
     const taa_uniforms_t& U = PROD ? pf.U : U_in;
     const taa_params_t& P = PROD ? pf.P : U.param[fd.param_idx];
@@ -984,13 +984,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S16 ? 4 : 3
 }
 
 // the resolve with the segmentation mask of mRayTraceAugment (FA: the FP16 path's arithmetic where
-// no selection depends on it)
-template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA>
+// no selection depends on it; PROD: the production feature set with the mask, prod_params(.., true))
+template <int TW, int TH, int PAD, bool CAS, bool S16, bool FA, bool PROD = false>
 __global__ __launch_bounds__(256) void k_resolve_seg(const ResolveArgs a, const SegDesc sg) {
     using G = Geo<TW, TH, PAD, CAS, false, true>;
     __shared__ typename G::Smem sm;
     const int n = a.fd.tiles_x * a.fd.tiles_y;
-    resolve_tile<G, S16, true, FA>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm, &sg);
+    resolve_tile<G, S16, true, FA, PROD>(a.fd, a.U, xcd_tile(blockIdx.x, n), sm, &sg);
 }
 
 // the upsampling resolve with the segmentation mask (its inputs read from global memory)
@@ -1090,14 +1090,15 @@ hipError_t launch_batch_t(const BatchEntry* e, int n, int total, int shape, hipS
 
 }  // namespace
 
-bool prod_params(const FrameDesc& fd, const taa_uniforms_t& U) {
+bool prod_params(const FrameDesc& fd, const taa_uniforms_t& U, bool seg) {
     const taa_params_t& p = U.param[fd.param_idx];
     const bool s16 = fd.color_fmt == TAA_FMT_RGBA16F;
     return !fd.debug && !fd.inspect && !U.mBypassHistoryUpdate &&
            fd.vel_fmt == (s16 ? TAA_VEL_RG16F : TAA_VEL_RG32F) && !p.mPassThrough && !p.mUnjitterNeighbourhood &&
            !p.mUnjitterCurrentSample && p.mVarianceClipping && !p.mShapedNeighbourhood && p.mUseYCoCg &&
            !p.mShrinkChromaAxis && p.mUseVelocityVectors == 2 && p.mVelocitySampleMode == 2 && p.mDepthCulling &&
-           p.mInterpolationMode == 2 && !p.mDynamicAntiGhosting && !p.mRayTraceAugment && p.mColorClampingOrClipping == 2 &&
+           p.mInterpolationMode == 2 && !p.mDynamicAntiGhosting && (p.mRayTraceAugment != 0) == seg &&
+           p.mColorClampingOrClipping == 2 &&
            !p.mVelBasedAlpha && p.mLumaWeightingLottes && !p.mReduceBlendNearClamp && p.mRejectOutside && !p.mAddNoise &&
            !p.mToneMapLumaKaris;
 }
@@ -1169,7 +1170,10 @@ hipError_t launch_resolve_seg_t(ResolveArgs a, const SegDesc& sg, hipStream_t st
             else hipLaunchKernelGGL((k_resolve_up_seg<Pl.tw, Pl.th, Pl.pad, false, false, FA>), dim3(n), dim3(256), 0, st, a, sg);
         }
     } else {
-        if (fd.cas_on) {
+        if (fd.cas_on && prod_params(fd, a.U, true)) {  // the production feature set with the mask
+            if (s16) hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, true, FA, true>), dim3(n), dim3(256), 0, st, a, sg);
+            else hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, false, FA, true>), dim3(n), dim3(256), 0, st, a, sg);
+        } else if (fd.cas_on) {
             if (s16) hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, true, FA>), dim3(n), dim3(256), 0, st, a, sg);
             else hipLaunchKernelGGL((k_resolve_seg<C.tw, C.th, C.pad, true, false, FA>), dim3(n), dim3(256), 0, st, a, sg);
         } else {

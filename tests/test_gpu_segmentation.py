@@ -16,7 +16,7 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from gpu_helpers import INDEX_WORDS, assert_parity, psnr, round_storage, to_f16  # noqa: E402
+from gpu_helpers import INDEX_WORDS, assert_parity, assert_same_bits, psnr, round_storage, to_f16  # noqa: E402
 from taastar import (ARITH_FP16, FLAG_RECTIFIED, FMT_RGBA16F, FMT_RGBA32F, INSPECT, RTFLAG, VEL_RG16F,  # noqa: E402
                      VEL_RG32F, SegmentationImages, check, lib)
 from taastar.device import FrameBuffers  # noqa: E402
@@ -35,7 +35,7 @@ class SegSequence:
     mask on the GPU and by the FP32 oracle side by side."""
 
     def __init__(self, w, width, height, cas=True, params=None, split=None, input_size=None, upsampling=False,
-                 arith=0):
+                 arith=0, shipped=False):
         self.w, self.W, self.H = w, width, height
         Wl, Hl = input_size if input_size is not None else (width, height)
         self.fmt, self.vfmt = w["fmt"], w["vel"]
@@ -47,6 +47,11 @@ class SegSequence:
         self.mat = [z(Hl, Wl, dtype=torch.int32), z(Hl, Wl, dtype=torch.int32)]
         self.uvn = z(Hl, Wl, 4, dtype=torch.float32)
         self.mask = [z(height, width, dtype=torch.int32), z(height, width, dtype=torch.int32)]
+        # shipped: the same frames also through buffers without the debug and inspect images (with the
+        # production feature set and CAS the one-size masked resolve then runs its production form);
+        # step() asserts that both write the same history, screen and mask bits
+        self.ship = FrameBuffers(width, height, self.fmt, self.vfmt, input_size=(Wl, Hl)) if shipped else None
+        self.mask_s = [z(height, width, dtype=torch.int32), z(height, width, dtype=torch.int32)] if shipped else None
         self.hist_o = np.zeros((height, width, 4), np.float32)
         self.prev_depth = np.zeros((Hl, Wl), np.float32)
         self.prev_mat_o = np.zeros((Hl, Wl), np.uint32)
@@ -74,7 +79,19 @@ class SegSequence:
         check(lib().taa_resolve_frame_segmented(C.byref(U), C.byref(img), C.byref(sg),
                                                 C.byref(self.cas) if self.cas else None, self.arith, None))
         self.bufs.k += 1
+        if self.ship is not None:
+            self.ship.upload(col_s, dep, vel_s)
+            s2 = SegmentationImages()
+            s2.material, s2.prev_material, s2.uv_normal = sg.material, sg.prev_material, sg.uv_normal
+            s2.prev_segmask, s2.segmask = self.mask_s[(k + 1) & 1].data_ptr(), self.mask_s[k & 1].data_ptr()
+            check(lib().taa_resolve_frame_segmented(C.byref(U), C.byref(self.ship.images()), C.byref(s2),
+                                                    C.byref(self.cas) if self.cas else None, self.arith, None))
+            self.ship.k += 1
         torch.cuda.synchronize()
+        if self.ship is not None:
+            assert_same_bits(self.bufs.screen, self.ship.screen, f"frame {k}: screen without debug and inspect images")
+            assert_same_bits(self.bufs.hist_in, self.ship.hist_in, f"frame {k}: history without debug and inspect images")
+            assert torch.equal(self.mask[k & 1], self.mask_s[k & 1]), f"frame {k}: mask without debug and inspect images"
         o = oracle.resolve_seg(U, decode(col_s), dep, velocity_as_read(vel_s), self.hist_o, self.prev_depth, mat,
                                self.prev_mat_o, uvn, self.mask_o)
         o_hist = round_storage(o["history"], self.fmt)
@@ -139,10 +156,11 @@ def seg_params(flags=None, **kw):
 @pytest.mark.parametrize("fmt", [FMT_RGBA32F, FMT_RGBA16F])
 def test_segmentation_default_flags(oracle, fmt):
     """The reference's default flags (all indicators, outside, disocclusion, history count) on a
-    panning scene: categories 0, 1 and 2 all occur, with the fused CAS."""
+    panning scene: categories 0, 1 and 2 all occur, with the fused CAS; without the debug and inspect
+    images (the production form of the masked resolve) the same bits."""
     print(f"case seg default fmt={fmt}", flush=True)
     w = workload("c2", fmt=fmt, vel=VEL_RG16F if fmt == FMT_RGBA16F else VEL_RG32F)
-    seq = SegSequence(w, 320, 180, params=seg_params())
+    seq = SegSequence(w, 320, 180, params=seg_params(), shipped=True)
     cats = set()
     for _ in range(5):
         out = seq.step(oracle)
@@ -214,11 +232,12 @@ def test_segmentation_upsampled_flag_variants(oracle, name):
 
 @pytest.mark.parametrize("up", [False, True])
 def test_segmentation_fp16_arithmetic(oracle, up):
-    """TAA_ARITH_FP16 with the mask, at one size and upsampling: the mask and every selection exact."""
+    """TAA_ARITH_FP16 with the mask, at one size and upsampling: the mask and every selection exact;
+    without the debug and inspect images the same bits."""
     print(f"case seg fp16 up={up}", flush=True)
     w = workload("c3")
     seq = SegSequence(w, 320, 192, params=seg_params(), input_size=(160, 96) if up else None, upsampling=up,
-                      arith=ARITH_FP16)
+                      arith=ARITH_FP16, shipped=True)
     for _ in range(4):
         assert_seg_fp16(seq.step(oracle))
 
