@@ -283,6 +283,30 @@ def test_tile_shapes_table():
     assert lib.taa_tile_shape_select(640, 360, 0, 0, 0, 99) == _abi.TAA_ERR_INVALID_ARG
 
 
+def test_default_cas_tile_shape_follows_the_frame_size():
+    """Untuned classes with CAS: 30x30 once a frame holds 1,280 such tiles (one per workgroup the
+    device holds at once), 30x14 below that, 14x14 below 512; an entry of the table overrides it and
+    -1 restores the built-in choice."""
+    lib = taastar.lib()
+
+    def shape(i):
+        s = taastar.TileShape()
+        assert lib.taa_tile_shape_get(i, 1, C.byref(s)) == 0
+        return s.tile_w, s.tile_h
+
+    fmt, arith = taastar.FMT_RGBA16F, taastar.ARITH_FP16
+    for (w, h), want in (((640, 360), (14, 14)), ((203, 81), (14, 14)), ((1280, 720), (30, 14)),
+                         ((1920, 1080), (30, 30)), ((3840, 2160), (30, 30)), ((7680, 4320), (30, 30))):
+        n30 = -(-w // 30) * -(-h // 30)
+        assert (n30 < 512) == (want == (14, 14)) and (512 <= n30 < 1280) == (want == (30, 14))
+        assert shape(lib.taa_tile_shape_selected(w, h, fmt, arith, 1)) == want, (w, h)
+    assert lib.taa_tile_shape_selected(640, 360, fmt, arith, 0) == lib.taa_tile_shape_selected(3840, 2160, fmt, arith, 0)
+    assert lib.taa_tile_shape_select(1280, 720, fmt, arith, 1, 2) == 0
+    assert shape(lib.taa_tile_shape_selected(1280, 720, fmt, arith, 1)) == (30, 30)
+    assert lib.taa_tile_shape_select(1280, 720, fmt, arith, 1, -1) == 0
+    assert shape(lib.taa_tile_shape_selected(1280, 720, fmt, arith, 1)) == (30, 14)
+
+
 def test_scene_is_deterministic_and_well_formed():
     sc = taastar.Scene(width=160, height=96, num_polygons=40, seed=123)
     c1, d1, v1, j1 = sc.render(3, threads=1)
